@@ -1,0 +1,235 @@
+"""FSP engine benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A "step" is one complete FSP solve of the 3 x 512 x 512 gene-expression
+network (786,432 states): Krylov expv (m = 30, tol = 1e-8) from
+p0 = delta(G=0, n=0, m=0) to t = 60 with 30 equispaced output times, FP64
+merged-CSR generator.  `value` = solves/s with all inputs resident in HBM.
+
+Also reported on the same line:
+  roofline      merged-CSR FP64 SpMV on the 16,777,216-state five-species
+                network (the HBM-scale shape): algorithmic bytes / launch
+                time from HIP events on the launch stream, against 8 TB/s
+  spmv          the same for HYB / KRON / mixed precision
+  mixed         the same solve with the FP32-valued / FP32-basis form:
+                solves/s and worst L1 error vs FP64 over the output times
+  cpu_baseline  the CPU oracle (restatement of the reference algorithm) on a
+                bounded sample of the same solve, extrapolated to solves/s
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from cuda_fsp_amd import capi, models as M  # noqa: E402
+
+METRIC = "FSP solves/s + SpMV HBM GB/s; mixed/batched vs plain FP64 at 1e-6 L1"
+HBM_PEAK_GBS = 8000.0
+
+
+class HipEvents:
+    """HIP events through the runtime the engine already loaded."""
+
+    def __init__(self):
+        capi.lib()
+        self.h = C.CDLL("libamdhip64.so.7")
+        self.h.hipEventCreate.argtypes = [C.c_void_p]
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.h.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream=None):
+        assert self.h.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        ms = C.c_float(0)
+        assert self.h.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+def solve_outputs(E, v, p0_dev, n, t_out, out_buf):
+    capi.check(capi.lib().fsp_memcpy_d2d(v.data_ptr(), p0_dev.data_ptr(), n * 8, None))
+    E.restart(t_out[0])
+    for k, t in enumerate(t_out):
+        E.set_final_time(t)
+        E.solve(v)
+        capi.check(capi.lib().fsp_memcpy_d2d(out_buf.offset(k * n), v.data_ptr(), n * 8, None))
+    capi.check(capi.lib().fsp_stream_sync(None))
+
+
+def bench_solve(net, precision, steps, warmup, t_out):
+    A = capi.Mat(net, "csr", precision)
+    n = A.n_rows
+    p0 = M.delta_p0(net)
+    p0_dev = capi.DeviceBuffer.from_host(p0)
+    v = capi.DeviceBuffer.zeros(n)
+    out = capi.DeviceBuffer.zeros(n * len(t_out))
+    E = capi.Expv(A, n, t_out[0], m=30, tol=1e-8)
+    for _ in range(warmup):
+        solve_outputs(E, v, p0_dev, n, t_out, out)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solve_outputs(E, v, p0_dev, n, t_out, out)
+    dt = time.perf_counter() - t0
+    st = E.stats()
+    res = out.to_host().reshape(len(t_out), n)
+    return dt, st, res
+
+
+def bench_spmv(net, fmt, precision, reps, ev):
+    A = capi.Mat(net, fmt, precision)
+    n = A.n_rows
+    x = capi.DeviceBuffer.from_host(np.random.default_rng(0).random(n))
+    y = capi.DeviceBuffer.zeros(n)
+    for _ in range(3):
+        A.apply(x, y, accumulate=False)
+    a, b = ev.event(), ev.event()
+    ev.record(a)
+    for _ in range(reps):
+        A.apply(x, y, accumulate=False)
+    ev.record(b)
+    ms = ev.elapsed_ms(a, b) / reps
+    bytes_ = A.info.bytes_per_apply
+    out = dict(format=fmt, precision=precision, n=n, nnz=int(A.info.nnz),
+               us_per_apply=round(ms * 1e3, 2), algorithmic_bytes=int(bytes_),
+               achieved_GBs=round(bytes_ / (ms * 1e-3) / 1e9, 1))
+    A.close()
+    return out
+
+
+def cpu_baseline(net, t_out, budget_s=15.0, threads=None):
+    """Oracle (CPU restatement of KryExpvFSP) on the same solve, bounded:
+    the first Krylov steps until ~budget_s, extrapolated to a full solve by
+    the step count of the full solve."""
+    from oracle import oracle as O
+    threads = threads or min(os.cpu_count() or 1, 16)
+    O.set_threads(threads)
+    rp, c, v = O.merged_csr(net)
+    p = M.delta_p0(net)
+    lib = O.lib()
+    st = O.ExpvStats()
+
+    class _Ctx(C.Structure):
+        _fields_ = [("n", C.c_int64), ("rp", C.c_void_p), ("cols", C.c_void_p),
+                    ("vals", C.c_void_p)]
+    # step the oracle directly through its C API
+    L = lib
+    L.orc_expv_create.restype = C.c_void_p
+    L.orc_expv_create.argtypes = [C.c_int64, C.c_double, C.c_void_p, C.c_void_p, C.c_int,
+                                  C.c_double, C.c_int, C.c_int, C.c_double]
+    L.orc_expv_step.argtypes = [C.c_void_p, C.c_void_p]
+    L.orc_expv_destroy.argtypes = [C.c_void_p]
+    L.orc_expv_get_stats.argtypes = [C.c_void_p, C.c_void_p]
+    L.orc_expv_final_time_reached.argtypes = [C.c_void_p]
+    MV = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p)
+    n = len(rp) - 1
+    L.orc_csr_apply.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                C.c_void_p, C.c_void_p]
+    mv = MV(lambda ctx, x, y: L.orc_csr_apply(n, rp.ctypes.data, c.ctypes.data, v.ctypes.data,
+                                              1.0, x, y))
+    e = L.orc_expv_create(n, float(t_out[-1]), C.cast(mv, C.c_void_p), None, 30, 1e-8, 0, 2, 1.0)
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < budget_s and not L.orc_expv_final_time_reached(e):
+        L.orc_expv_step(e, p.ctypes.data)
+        steps += 1
+    dt = time.perf_counter() - t0
+    L.orc_expv_get_stats(e, C.byref(st))
+    L.orc_expv_destroy(e)
+    return dt, steps, threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this engine has no multi-GPU path "
+              "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if capi.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    t_out = [2.0 * k for k in range(1, 31)]
+    net = M.gene_expression()
+    print(f"[bench] FP64 solve: {net.name} n={net.n_states}", flush=True)
+    dt, st, ref = bench_solve(net, "f64", args.steps, args.warmup, t_out)
+    solves_s = args.steps / dt
+    print(f"[bench] FP64 {solves_s:.3f} solves/s, {st.i_step} Krylov steps/solve", flush=True)
+
+    print("[bench] mixed solve", flush=True)
+    dtm, stm, mixed = bench_solve(net, "mixed", args.steps, args.warmup, t_out)
+    l1_mixed = float(np.abs(mixed - ref).sum(axis=1).max())
+    print(f"[bench] mixed {args.steps / dtm:.3f} solves/s, worst L1 {l1_mixed:.2e}", flush=True)
+
+    ev = HipEvents()
+    big = M.five_species_chain()
+    spmv = []
+    for fmt, prec in (("csr", "f64"), ("hyb", "f64"), ("kron", "f64"), ("csr", "mixed"),
+                      ("hyb", "mixed"), ("kron", "mixed")):
+        print(f"[bench] SpMV {fmt}/{prec} on n={big.n_states}", flush=True)
+        spmv.append(bench_spmv(big, fmt, prec, args.spmv_reps, ev))
+    head = spmv[0]
+    roofline = {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(head["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                "traffic": None,
+                "kernel": "k_apply_csr<double> (merged CSR SpMV, FP64)",
+                "workload": f"five_species_chain n={big.n_states} nnz={head['nnz']}",
+                "algorithmic_bytes": head["algorithmic_bytes"],
+                "us_per_launch": head["us_per_apply"]}
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] CPU oracle sample", flush=True)
+        cdt, csteps, threads = cpu_baseline(net, t_out)
+        frac = csteps / max(st.i_step, 1)
+        cpu = {"value": round(frac / cdt, 6), "unit": "solves/s", "cores": threads,
+               "kind": "port",
+               "sample": f"first {csteps} of {st.i_step} Krylov steps of one solve "
+                         f"({cdt:.1f} s, {threads} OpenMP threads), extrapolated",
+               "cpu": platform.processor() or platform.machine()}
+
+    line = {
+        "metric": METRIC, "value": round(solves_s, 4), "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "gene_expression 3x512x512 (786,432 states): Krylov expv "
+                               "m=30 tol=1e-8 to t=60 at 30 output times, FP64 merged CSR",
+                   "krylov_steps_per_solve": int(st.i_step),
+                   "matvecs_per_solve": int(st.n_matvec)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "spmv": spmv,
+        "mixed": {"solves_per_s": round(args.steps / dtm, 4),
+                  "speedup_vs_f64": round(dt / dtm, 3), "worst_l1_vs_f64": l1_mixed,
+                  "krylov_steps_per_solve": int(stm.i_step)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,430 @@
+// fsp_capi.hip -- C ABI entry points (include/fsp_capi.h): errors, device
+// memory helpers, state enumeration, vector reductions and fsp_mat.
+#include <cstring>
+#include <string>
+
+#include "fsp_formats.h"
+
+namespace fsp {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+fsp_status fail(fsp_status st, const std::string &msg)
+{
+    set_error(msg);
+    return st;
+}
+
+namespace {
+
+// fsp_get_states (src/cme_util.cu:24-41): AoS int[n][d], species 0 fastest.
+template <int D>
+__global__ void k_enumerate(const DevModel *__restrict__ dm, uint32_t n, int *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    int xs[D];
+    decode<D>(i, dm, xs);
+#pragma unroll
+    for (int s = 0; s < D; ++s) out[(size_t)i * D + s] = xs[s];
+}
+
+// state2indx (src/cme_util.cu:13-22) over a batch; outside the box -> -1.
+__global__ void k_to_index(const DevModel *__restrict__ dm, const int *__restrict__ states,
+                           int64_t count, int *__restrict__ idx)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const int d = dm->d;
+    int64_t v = 0;
+    bool ok = true;
+    for (int s = 0; s < d; ++s) {
+        const int x = states[k * d + s];
+        ok = ok && x >= 0 && x < dm->radix[s];
+        v += (int64_t)x * dm->stride[s];
+    }
+    idx[k] = ok ? (int)v : -1;
+}
+
+__global__ void k_sum_partials(const double *__restrict__ x, int64_t n, double *partials)
+{
+    __shared__ double red[kBlock / kWave];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        s += x[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void k_sum_final(const double *__restrict__ partials, int G, double *out)
+{
+    __shared__ double red[kBlock / kWave];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < G; b += kBlock) s += partials[b];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void k_axpy(double a, const double *__restrict__ x, double *__restrict__ y, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        y[i] = a * x[i] + y[i];
+}
+
+__global__ void k_fill(double *__restrict__ x, double v, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        x[i] = v;
+}
+
+// Geometry-only model (bounds) for the state-space calls.
+fsp_status geom_model(int d, const int *bounds, HostModel &hm)
+{
+    if (d < 1 || d > kMaxSpecies) return fail(FSP_ERR_UNSUPPORTED, "n_species must be in 1..8");
+    if (!bounds) return fail(FSP_ERR_INVALID, "bounds is NULL");
+    std::vector<int> sv(1, 1), sc(1, 0), sr = {0, 0};
+    std::vector<double> f;
+    int stride = 0;
+    for (int s = 0; s < d; ++s) {
+        if (bounds[s] < 0) return fail(FSP_ERR_INVALID, "negative bound");
+        stride += bounds[s] + 1;
+    }
+    f.assign((size_t)stride, 1.0);
+    fsp_model m{d, 1, bounds, sv.data(), sc.data(), sr.data(), f.data(), nullptr};
+    return build_host_model(&m, hm);
+}
+
+}  // namespace
+
+fsp_status vec_sum(const double *d_x, int64_t n, double *h_sum, hipStream_t s)
+{
+    const int G = grid_for((n + kBlock - 1) / kBlock, 1024);
+    double *tmp = nullptr;
+    FSP_HIP(hipMalloc(&tmp, sizeof(double) * (size_t)(G + 1)));
+    k_sum_partials<<<G, kBlock, 0, s>>>(d_x, n, tmp);
+    k_sum_final<<<1, kBlock, 0, s>>>(tmp, G, tmp + G);
+    FSP_HIP(hipGetLastError());
+    FSP_HIP(hipMemcpyAsync(h_sum, tmp + G, sizeof(double), hipMemcpyDeviceToHost, s));
+    FSP_HIP(hipStreamSynchronize(s));
+    hipFree(tmp);
+    return FSP_OK;
+}
+
+}  // namespace fsp
+
+using namespace fsp;
+
+struct fsp_mat_s {
+    MatImpl impl;
+};
+
+extern "C" {
+
+const char *fsp_last_error(void) { return g_last_error.c_str(); }
+
+int fsp_version(void) { return 1; }
+
+fsp_status fsp_device_count(int *count)
+{
+    if (!count) return fail(FSP_ERR_INVALID, "count is NULL");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    *count = c;
+    return FSP_OK;
+}
+
+fsp_status fsp_malloc(void **d_ptr, size_t bytes)
+{
+    if (!d_ptr) return fail(FSP_ERR_INVALID, "d_ptr is NULL");
+    hipError_t e = hipMalloc(d_ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(FSP_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return FSP_OK;
+}
+
+fsp_status fsp_free(void *d_ptr)
+{
+    if (d_ptr) FSP_HIP(hipFree(d_ptr));
+    return FSP_OK;
+}
+
+fsp_status fsp_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes)
+{
+    FSP_HIP(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+    return FSP_OK;
+}
+
+fsp_status fsp_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes)
+{
+    FSP_HIP(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+    return FSP_OK;
+}
+
+fsp_status fsp_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream)
+{
+    FSP_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return FSP_OK;
+}
+
+fsp_status fsp_memset_zero(void *d_ptr, size_t bytes, void *stream)
+{
+    FSP_HIP(hipMemsetAsync(d_ptr, 0, bytes, (hipStream_t)stream));
+    return FSP_OK;
+}
+
+fsp_status fsp_stream_sync(void *stream)
+{
+    FSP_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return FSP_OK;
+}
+
+fsp_status fsp_vec_sum(const double *d_x, int64_t n, double *h_sum, void *stream)
+{
+    if (!d_x || !h_sum || n < 0) return fail(FSP_ERR_INVALID, "bad argument");
+    if (n == 0) {
+        *h_sum = 0.0;
+        return FSP_OK;
+    }
+    return vec_sum(d_x, n, h_sum, (hipStream_t)stream);
+}
+
+fsp_status fsp_vec_axpy(double a, const double *d_x, double *d_y, int64_t n, void *stream)
+{
+    if (n <= 0) return FSP_OK;
+    const int G = grid_for((n + kBlock - 1) / kBlock, 4096);
+    k_axpy<<<G, kBlock, 0, (hipStream_t)stream>>>(a, d_x, d_y, n);
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
+}
+
+fsp_status fsp_vec_fill(double *d_x, double value, int64_t n, void *stream)
+{
+    if (n <= 0) return FSP_OK;
+    const int G = grid_for((n + kBlock - 1) / kBlock, 4096);
+    k_fill<<<G, kBlock, 0, (hipStream_t)stream>>>(d_x, value, n);
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
+}
+
+fsp_status fsp_num_states(int n_species, const int *bounds, int64_t *n_states)
+{
+    if (!n_states) return fail(FSP_ERR_INVALID, "n_states is NULL");
+    if (!bounds || n_species < 1) return fail(FSP_ERR_INVALID, "bad bounds");
+    int64_t n = 1;
+    for (int s = 0; s < n_species; ++s) {
+        if (bounds[s] < 0) return fail(FSP_ERR_INVALID, "negative bound");
+        n *= (int64_t)bounds[s] + 1;
+        if (n > 2147483647LL) return fail(FSP_ERR_OVERFLOW, "more than 2^31-1 states");
+    }
+    *n_states = n;
+    return FSP_OK;
+}
+
+fsp_status fsp_states_enumerate(int n_species, const int *bounds, int *d_states, void *stream)
+{
+    HostModel hm;
+    FSP_TRY(geom_model(n_species, bounds, hm));
+    if (!d_states) return fail(FSP_ERR_INVALID, "d_states is NULL");
+    DevModel *dm = nullptr;
+    FSP_HIP(hipMalloc(&dm, sizeof(DevModel)));
+    FSP_HIP(hipMemcpy(dm, &hm.dm, sizeof(DevModel), hipMemcpyHostToDevice));
+    const uint32_t n = (uint32_t)hm.dm.n;
+    const unsigned blocks = (n + kBlock - 1) / kBlock;
+    hipStream_t s = (hipStream_t)stream;
+    FSP_SWITCH_D(n_species, (k_enumerate<D_><<<blocks, kBlock, 0, s>>>(dm, n, d_states)));
+    FSP_HIP(hipGetLastError());
+    FSP_HIP(hipStreamSynchronize(s));
+    hipFree(dm);
+    return FSP_OK;
+}
+
+fsp_status fsp_states_to_index(int n_species, const int *bounds, const int *d_states,
+                               int64_t count, int *d_index, void *stream)
+{
+    HostModel hm;
+    FSP_TRY(geom_model(n_species, bounds, hm));
+    if (count < 0 || (count > 0 && (!d_states || !d_index)))
+        return fail(FSP_ERR_INVALID, "bad argument");
+    if (count == 0) return FSP_OK;
+    DevModel *dm = nullptr;
+    FSP_HIP(hipMalloc(&dm, sizeof(DevModel)));
+    FSP_HIP(hipMemcpy(dm, &hm.dm, sizeof(DevModel), hipMemcpyHostToDevice));
+    hipStream_t s = (hipStream_t)stream;
+    k_to_index<<<(unsigned)((count + kBlock - 1) / kBlock), kBlock, 0, s>>>(dm, d_states, count,
+                                                                           d_index);
+    FSP_HIP(hipGetLastError());
+    FSP_HIP(hipStreamSynchronize(s));
+    hipFree(dm);
+    return FSP_OK;
+}
+
+fsp_status fsp_mat_internal(fsp_mat A, MatImpl **impl)
+{
+    if (!A || !impl) return fail(FSP_ERR_INVALID, "NULL matrix");
+    *impl = &A->impl;
+    return FSP_OK;
+}
+
+fsp_status fsp_mat_create(const fsp_model *model, fsp_format format, fsp_precision precision,
+                          int with_sink, const double *coefs, void *stream, fsp_mat *out)
+{
+    if (!out) return fail(FSP_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (format != FSP_CSR && format != FSP_HYB && format != FSP_KRON)
+        return fail(FSP_ERR_UNSUPPORTED, "requested format is not supported");
+    if (precision != FSP_F64 && precision != FSP_MIXED)
+        return fail(FSP_ERR_UNSUPPORTED, "requested precision is not supported");
+    if (!coefs) return fail(FSP_ERR_INVALID, "coefs is NULL");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(FSP_ERR_NO_DEVICE, "no HIP device visible");
+    auto *h = new fsp_mat_s();
+    MatImpl &M = h->impl;
+    fsp_status st = build_host_model(model, M.hm);
+    if (st != FSP_OK) {
+        delete h;
+        return st;
+    }
+    if (format == FSP_KRON && !M.hm.has_factors && !M.hm.prop_table_dev) {
+        delete h;
+        return fail(FSP_ERR_INVALID, "KRON needs factor tables");
+    }
+    M.format = format;
+    M.prec = precision;
+    M.with_sink = with_sink ? 1 : 0;
+    M.coefs.assign(coefs, coefs + M.hm.dm.R);
+    st = mat_assemble(M, (hipStream_t)stream);
+    if (st != FSP_OK) {
+        mat_free(M);
+        delete h;
+        return st;
+    }
+    *out = h;
+    return FSP_OK;
+}
+
+fsp_status fsp_mat_destroy(fsp_mat A)
+{
+    if (!A) return FSP_OK;
+    mat_free(A->impl);
+    delete A;
+    return FSP_OK;
+}
+
+fsp_status fsp_mat_get_info(fsp_mat A, fsp_mat_info *info)
+{
+    if (!A || !info) return fail(FSP_ERR_INVALID, "NULL argument");
+    const MatImpl &M = A->impl;
+    info->n_states = M.n;
+    info->n_rows = M.n_rows;
+    info->nnz = M.nnz_rows + M.nnz_sink;
+    info->n_slots = M.hm.dm.U;
+    info->format = M.format;
+    info->precision = M.prec;
+    info->with_sink = M.with_sink;
+    info->max_row = M.max_row;
+    info->bytes_per_apply = mat_bytes_per_apply(M);
+    return FSP_OK;
+}
+
+fsp_status fsp_mat_set_coefs(fsp_mat A, const double *h_coefs, void *stream)
+{
+    if (!A || !h_coefs) return fail(FSP_ERR_INVALID, "NULL argument");
+    return mat_revalue(A->impl, h_coefs, (hipStream_t)stream);
+}
+
+fsp_status fsp_mat_apply(fsp_mat A, const double *h_coefs, const double *d_x, double *d_y,
+                         int accumulate, void *stream)
+{
+    if (!A || !d_x || !d_y) return fail(FSP_ERR_INVALID, "NULL argument");
+    if (h_coefs) FSP_TRY(mat_revalue(A->impl, h_coefs, (hipStream_t)stream));
+    return mat_apply(A->impl, d_x, d_y, accumulate, (hipStream_t)stream);
+}
+
+fsp_status fsp_mat_apply_transpose(fsp_mat A, const double *h_coefs, const double *d_x,
+                                   double *d_y, int accumulate, void *stream)
+{
+    if (!A || !d_x || !d_y) return fail(FSP_ERR_INVALID, "NULL argument");
+    if (h_coefs) FSP_TRY(mat_revalue(A->impl, h_coefs, (hipStream_t)stream));
+    return mat_apply_transpose(A->impl, d_x, d_y, accumulate, (hipStream_t)stream);
+}
+
+fsp_status fsp_mat_apply_term(fsp_mat A, int reaction, double c, const double *d_x, double *d_y,
+                              int accumulate, void *stream)
+{
+    if (!A || !d_x || !d_y) return fail(FSP_ERR_INVALID, "NULL argument");
+    return mat_apply_term(A->impl, reaction, c, d_x, d_y, accumulate, (hipStream_t)stream);
+}
+
+fsp_status fsp_mat_export_csr(fsp_mat A, int *d_row_ptrs, int *d_cols, double *d_vals,
+                              void *stream)
+{
+    if (!A) return fail(FSP_ERR_INVALID, "NULL matrix");
+    const MatImpl &M = A->impl;
+    if (M.format != FSP_CSR || M.prec != FSP_F64)
+        return fail(FSP_ERR_UNSUPPORTED, "export_csr needs an FP64 CSR matrix");
+    hipStream_t s = (hipStream_t)stream;
+    // rows 0..n-1 as stored; the sink row (if any) appended as row n
+    if (d_row_ptrs) {
+        FSP_HIP(hipMemcpyAsync(d_row_ptrs, M.rp, sizeof(int) * (size_t)(M.n + 1),
+                               hipMemcpyDeviceToDevice, s));
+        if (M.with_sink) {
+            const int end = (int)(M.nnz_rows + M.nnz_sink);
+            FSP_HIP(hipStreamSynchronize(s));
+            FSP_HIP(hipMemcpy(d_row_ptrs + M.n + 1, &end, sizeof(int), hipMemcpyHostToDevice));
+        }
+    }
+    if (d_cols) {
+        FSP_HIP(hipMemcpyAsync(d_cols, M.cols, sizeof(int) * (size_t)M.nnz_rows,
+                               hipMemcpyDeviceToDevice, s));
+        if (M.with_sink && M.nnz_sink)
+            FSP_HIP(hipMemcpyAsync(d_cols + M.nnz_rows, M.sink_cols, sizeof(int) * (size_t)M.nnz_sink,
+                                   hipMemcpyDeviceToDevice, s));
+    }
+    if (d_vals) {
+        FSP_HIP(hipMemcpyAsync(d_vals, M.vals, sizeof(double) * (size_t)M.nnz_rows,
+                               hipMemcpyDeviceToDevice, s));
+        if (M.with_sink && M.nnz_sink)
+            FSP_HIP(hipMemcpyAsync(d_vals + M.nnz_rows, M.sink_vals,
+                                   sizeof(double) * (size_t)M.nnz_sink, hipMemcpyDeviceToDevice, s));
+    }
+    FSP_HIP(hipStreamSynchronize(s));
+    return FSP_OK;
+}
+
+fsp_status fsp_mat_export_hyb(fsp_mat A, double *d_diag, double *d_slot_vals, int *d_slot_cols,
+                              double *d_sink_out, void *stream)
+{
+    if (!A) return fail(FSP_ERR_INVALID, "NULL matrix");
+    const MatImpl &M = A->impl;
+    if (M.format != FSP_HYB || M.prec != FSP_F64)
+        return fail(FSP_ERR_UNSUPPORTED, "export_hyb needs an FP64 HYB matrix");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t slots = (size_t)M.hm.dm.U * (size_t)M.n;
+    if (d_diag)
+        FSP_HIP(hipMemcpyAsync(d_diag, M.diag, sizeof(double) * (size_t)M.n, hipMemcpyDeviceToDevice, s));
+    if (d_slot_vals && slots)
+        FSP_HIP(hipMemcpyAsync(d_slot_vals, M.slot_vals, sizeof(double) * slots, hipMemcpyDeviceToDevice, s));
+    if (d_slot_cols && slots)
+        FSP_HIP(hipMemcpyAsync(d_slot_cols, M.slot_cols, sizeof(int) * slots, hipMemcpyDeviceToDevice, s));
+    if (d_sink_out && M.with_sink)
+        FSP_HIP(hipMemcpyAsync(d_sink_out, M.sink_out, sizeof(double) * (size_t)M.n, hipMemcpyDeviceToDevice, s));
+    FSP_HIP(hipStreamSynchronize(s));
+    return FSP_OK;
+}
+
+fsp_status fsp_term_csr(const fsp_model *model, int reaction, int *d_row_ptrs, int *d_cols,
+                        double *d_vals, int64_t *h_nnz, void *stream)
+{
+    if (!h_nnz) return fail(FSP_ERR_INVALID, "h_nnz is NULL");
+    HostModel hm;
+    FSP_TRY(build_host_model(model, hm));
+    return term_csr(hm, reaction, d_row_ptrs, d_cols, d_vals, h_nnz, (hipStream_t)stream);
+}
+
+}  // extern "C"

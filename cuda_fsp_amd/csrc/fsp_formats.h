@@ -1,0 +1,241 @@
+// fsp_formats.h -- generator storage formats and their row kernels.
+//
+// Every row kernel works on tiles of kBlock consecutive rows, one thread per
+// row, grid-stride over tiles.  An "x accessor" X supplies input values:
+// x(c) returns the FP64 value of input entry c (a plain load, or a scaled
+// and rounded Krylov vector).  Each tile function returns the row sum for
+// row tile*kBlock + threadIdx.x and adds the thread's share of the sink row
+// (outflow from the box) to `sink`.
+#pragma once
+
+#include <type_traits>
+
+#include "fsp_device.h"
+
+namespace fsp {
+
+// --------------------------------------------------------------------- CSR
+// Merged CSR.  F64: the diagonal is an ordinary row entry (columns
+// ascending), diag == nullptr.  MIXED: rows hold only off-diagonals in FP32
+// and the FP64 diagonal is separate, so every column of the stored
+// generator sums to exactly zero.  Sink row: entries sorted by column;
+// sink_off[i] = first sink entry with column >= i (i = 0..n).
+template <typename V>
+struct CsrF {
+    const int *__restrict__ rp;
+    const int *__restrict__ cols;
+    const V *__restrict__ vals;
+    const double *__restrict__ diag;
+    const int *__restrict__ sink_off;
+    const int *__restrict__ sink_cols;
+    const double *__restrict__ sink_vals;
+    uint32_t n;
+    int cap;  // LDS product slots per chunk (>= kBlock)
+};
+
+// Phase 1 streams the tile's nonzeros (coalesced) into LDS as products;
+// phase 2 lets each thread add its row's products in column order.
+template <typename V, class X>
+__device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
+                                  double *prod, double &sink)
+{
+    const uint32_t r0 = tile * kBlock;
+    const uint32_t r1 = min(r0 + (uint32_t)kBlock, A.n);
+    const uint32_t i = r0 + threadIdx.x;
+    const int k0 = A.rp[r0], k1 = A.rp[r1];
+    const int kb = (i < A.n) ? A.rp[i] : k1;
+    const int ke = (i < A.n) ? A.rp[i + 1] : k1;
+    double y = 0.0;
+    for (int c0 = k0; c0 < k1; c0 += A.cap) {
+        const int c1 = min(c0 + A.cap, k1);
+        __syncthreads();
+        for (int k = c0 + (int)threadIdx.x; k < c1; k += kBlock)
+            prod[k - c0] = (double)A.vals[k] * x(A.cols[k]);
+        __syncthreads();
+        const int a = max(kb, c0), b = min(ke, c1);
+        for (int k = a; k < b; ++k) y += prod[k - c0];
+    }
+    if (A.diag && i < A.n) y = A.diag[i] * x(i) + y;
+    if (A.sink_off) {
+        const int s0 = A.sink_off[r0], s1 = A.sink_off[r1];
+        for (int k = s0 + (int)threadIdx.x; k < s1; k += kBlock)
+            sink += A.sink_vals[k] * x(A.sink_cols[k]);
+    }
+    return y;
+}
+
+// --------------------------------------------------------------------- HYB
+// Merged FP64 diagonal + U ELL slots (SoA [U][n]), one per unique net
+// change; a missing neighbour stores col = row and value 0.
+template <typename V>
+struct HybF {
+    const double *__restrict__ diag;
+    const V *__restrict__ slot_vals;
+    const int *__restrict__ slot_cols;
+    const double *__restrict__ sink_out;  // dense outflow rates or nullptr
+    uint32_t n;
+    int U;
+};
+
+template <typename V, class X>
+__device__ inline double hyb_row(const HybF<V> &A, uint32_t i, const X &x, double &sink)
+{
+    if (i >= A.n) return 0.0;
+    const double xi = x(i);
+    double y = A.diag[i] * xi;
+    for (int u = 0; u < A.U; ++u) {
+        const size_t o = (size_t)u * A.n + i;
+        y += (double)A.slot_vals[o] * x(A.slot_cols[o]);
+    }
+    if (A.sink_out) sink += A.sink_out[i] * xi;
+    return y;
+}
+
+// -------------------------------------------------------------------- KRON
+// Matrix-free Kronecker-sum apply (the reference's fspmat_sdkron_mv formula,
+// src/fspmat_kron_kernels.cu:69-128): per reaction, the inflow from
+// i - delta_r times the source propensity minus x_i times the local
+// propensity.  Factor tables are staged in LDS once per workgroup (`tab`);
+// with use_table the propensities come from a dense R x n table instead.
+template <typename V>
+struct KronF {
+    const DevModel *__restrict__ dm;
+    const V *__restrict__ tab_g;          // compact factor table (global)
+    const double *__restrict__ prop_table; // R x n (use_table)
+    const double *__restrict__ coefs;      // device, R
+    uint32_t n;
+    int R;
+    int tab_size;
+    int with_sink;
+    int single;   // >= 0: only this reaction
+    double single_coef;
+};
+
+template <int D, typename V, class X>
+__device__ inline double kron_row(const KronF<V> &A, uint32_t i, const X &x, const V *tab,
+                                  double &sink)
+{
+    if (i >= A.n) return 0.0;
+    int xs[D];
+    decode<D>(i, A.dm, xs);
+    const double xh = x(i);
+    double y = 0.0, out = 0.0;
+    const int rbeg = A.single >= 0 ? A.single : 0;
+    const int rend = A.single >= 0 ? A.single + 1 : A.R;
+    for (int r = rbeg; r < rend; ++r) {
+        const RxnDev &rx = A.dm->rx[r];
+        if (rx.zero_nu) continue;
+        const double c = A.single >= 0 ? A.single_coef : A.coefs[r];
+        double alpha = 0.0, xs_src = 0.0, beta;
+        const bool in = source_in_box<D>(rx, xs, A.dm);
+        if (A.prop_table) {
+            beta = A.prop_table[(size_t)r * A.n + i];
+            if (in) {
+                const uint32_t src = (uint32_t)((int64_t)i - rx.delta);
+                alpha = A.prop_table[(size_t)r * A.n + src];
+                xs_src = x(src);
+            }
+        } else {
+            beta = prop_tab<D>(rx, xs, tab, 0);
+            if (in) {
+                alpha = prop_tab<D>(rx, xs, tab, 1);
+                xs_src = x((uint32_t)((int64_t)i - rx.delta));
+            }
+        }
+        y += c * (xs_src * alpha - xh * beta);
+        if (A.with_sink && target_leaves<D>(rx, xs, A.dm)) out += c * beta;
+    }
+    sink += out * xh;
+    return y;
+}
+
+// Transpose (adjoint) row: (A^T x)_j = sum_r c_r a_r(x_j) (x_{j+delta_r}
+// [target in box] - x_j).  No sink column (the sink is absorbing).
+template <int D, typename V, class X>
+__device__ inline double kron_row_t(const KronF<V> &A, uint32_t j, const X &x, const V *tab)
+{
+    if (j >= A.n) return 0.0;
+    int xs[D];
+    decode<D>(j, A.dm, xs);
+    const double xj = x(j);
+    double y = 0.0;
+    const int rbeg = A.single >= 0 ? A.single : 0;
+    const int rend = A.single >= 0 ? A.single + 1 : A.R;
+    for (int r = rbeg; r < rend; ++r) {
+        const RxnDev &rx = A.dm->rx[r];
+        if (rx.zero_nu) continue;
+        const double c = A.single >= 0 ? A.single_coef : A.coefs[r];
+        const double a = A.prop_table ? A.prop_table[(size_t)r * A.n + j]
+                                      : prop_tab<D>(rx, xs, tab, 0);
+        const bool leaves = target_leaves<D>(rx, xs, A.dm);
+        const double xt = leaves ? 0.0 : x((uint32_t)((int64_t)j + rx.delta));
+        y += c * a * (xt - xj);
+    }
+    return y;
+}
+
+// ------------------------------------------------------------ host views
+#define FSP_SWITCH_D(d, CALL)                                                  \
+    switch (d) {                                                               \
+    case 1: { constexpr int D_ = 1; CALL; } break;                             \
+    case 2: { constexpr int D_ = 2; CALL; } break;                             \
+    case 3: { constexpr int D_ = 3; CALL; } break;                             \
+    case 4: { constexpr int D_ = 4; CALL; } break;                             \
+    case 5: { constexpr int D_ = 5; CALL; } break;                             \
+    case 6: { constexpr int D_ = 6; CALL; } break;                             \
+    case 7: { constexpr int D_ = 7; CALL; } break;                             \
+    case 8: { constexpr int D_ = 8; CALL; } break;                             \
+    default: return ::fsp::fail(FSP_ERR_UNSUPPORTED, "n_species > 8");         \
+    }
+
+static constexpr size_t kLdsTableMax = 96 * 1024;
+
+template <typename V>
+inline CsrF<V> csr_view(const MatImpl &M)
+{
+    CsrF<V> A;
+    A.rp = M.rp;
+    A.cols = M.cols;
+    A.vals = (const V *)M.vals;
+    A.diag = M.prec == FSP_MIXED ? (const double *)M.diag : nullptr;
+    A.sink_off = M.with_sink ? M.sink_block_ptr : nullptr;
+    A.sink_cols = M.sink_cols;
+    A.sink_vals = (const double *)M.sink_vals;
+    A.n = (uint32_t)M.n;
+    int cap = kBlock * (M.max_row > 1 ? M.max_row : 1);
+    A.cap = cap > 4096 ? 4096 : cap;  // 32 KB of LDS products per chunk
+    return A;
+}
+
+template <typename V>
+inline HybF<V> hyb_view(const MatImpl &M)
+{
+    HybF<V> A;
+    A.diag = (const double *)M.diag;
+    A.slot_vals = (const V *)M.slot_vals;
+    A.slot_cols = M.slot_cols;
+    A.sink_out = M.with_sink ? (const double *)M.sink_out : nullptr;
+    A.n = (uint32_t)M.n;
+    A.U = M.hm.dm.U;
+    return A;
+}
+
+template <typename V>
+inline KronF<V> kron_view(const MatImpl &M, int single, double single_coef)
+{
+    KronF<V> A;
+    A.dm = M.d_model;
+    A.tab_g = std::is_same<V, float>::value ? (const V *)(const void *)M.d_compact_f
+                                            : (const V *)(const void *)M.d_compact;
+    A.prop_table = M.hm.prop_table_dev;
+    A.coefs = M.d_coefs;
+    A.n = (uint32_t)M.n;
+    A.R = M.hm.dm.R;
+    A.tab_size = M.hm.dm.compact_size;
+    A.with_sink = M.with_sink;
+    A.single = single;
+    A.single_coef = single_coef;
+    return A;
+}
+
+}  // namespace fsp

@@ -1,0 +1,1024 @@
+// fsp_krylov.hip -- device-resident Krylov expv (Expokit scheme) for gfx950.
+//
+// Restates KryExpvFSP::step (src/KryExpvFSP.cu:38-182) with the whole step on
+// the device: no host round trip inside a step (the reference syncs twice per
+// Arnoldi iteration and runs arma::expmat on the host).
+//
+// Per Arnoldi iteration j (operator held by an fsp_mat):
+//   k_arn_spmv   : V_j = w_j * sigma_j (the reference's V_{j+1} /= s, fused
+//                  into the read), w_{j+1} = A V_j, partial dots V_k . w_{j+1}
+//   k_arn_dots   : H(k, j) = sum of partials (fixed order), sink element
+//   k_arn_update : w_{j+1} -= V h (gemv N of :79-81), partial |w|^2
+//   k_arn_norm   : s = |w|, happy breakdown test, H(j+1, j) = s, sigma = 1/s
+// then k_expm_control (one workgroup: Pade expm of tau*H, error estimate,
+// reject loop, step-size update) and k_solution (v = beta V F0).
+// Reductions are deterministic: per-wave butterflies, per-block partials in
+// fixed order, one finalizing kernel.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "fsp_formats.h"
+
+namespace fsp {
+
+constexpr int kP10Lo = -300, kP10Hi = 300;
+constexpr long long kTauLogCap = 1 << 16;
+
+struct KState {
+    double beta, btol, t_now, t_new, t_final, tau, xm, err_loc, avnorm, fact;
+    double tol, anorm;
+    double tau_last;
+    double sigma[kMaxKrylov + 2];
+    double F0[kMaxKrylov + 2];
+    double H[(kMaxKrylov + 2) * (kMaxKrylov + 2)];
+    double p10[kP10Hi - kP10Lo + 1];   // correctly rounded 1e-300 .. 1e300
+    double *tau_log;                   // accepted step sizes, by step index
+    long long tau_cap;
+    int mb, k1, broke, active, mx, status, pad0, pad1;
+    long long i_step, n_reject, n_breakdown, n_matvec;
+};
+
+struct ExpvImpl {
+    MatImpl *A = nullptr;
+    fsp_matvec_cb cb = nullptr;
+    void *ctx = nullptr;
+    int64_t n = 0;       // vector length (n_rows of the operator)
+    int m = 30;
+    double tol = 1e-8;
+    int iop = 0, q_iop = 2;
+    double anorm = 1.0;
+    int max_reject = 1000;
+    int64_t max_nstep = 10000;
+    hipStream_t s = nullptr;
+    bool mixed = false;
+    KState *d_st = nullptr;
+    KState *h_st = nullptr;      // pinned mirror
+    double *V64 = nullptr;       // FP64 basis (all vectors, or V_0 when mixed)
+    float *V32 = nullptr;        // mixed: V_1..V_m
+    double *W[2] = {nullptr, nullptr};
+    double *partials = nullptr;
+    int G = 1;                   // blocks of the row kernels
+    int64_t tiles = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double *graph_v = nullptr;
+    bool use_graph = true;
+    hipStream_t cap = nullptr;   // private stream used only for graph capture
+    double *tau_log = nullptr;
+};
+
+namespace {
+
+// ------------------------------------------------------------- accessors
+template <typename B>
+struct KryX {
+    const double *__restrict__ in;
+    double sig;
+    __device__ double operator()(uint32_t c) const { return (double)(B)(in[c] * sig); }
+};
+
+template <typename B>
+__device__ inline double basis(const double *__restrict__ V64, const B *__restrict__ Vb,
+                               size_t nv, int k, uint32_t i)
+{
+    if (std::is_same<B, double>::value) return V64[(size_t)k * nv + i];
+    return k == 0 ? V64[i] : (double)Vb[(size_t)(k - 1) * nv + i];
+}
+
+template <typename B>
+__device__ inline void store_basis(double *__restrict__ V64, B *__restrict__ Vb, size_t nv,
+                                   int k, uint32_t i, double v)
+{
+    if (std::is_same<B, double>::value) {
+        V64[(size_t)k * nv + i] = v;
+    } else if (k == 0) {
+        V64[i] = v;
+    } else {
+        Vb[(size_t)(k - 1) * nv + i] = (B)v;
+    }
+}
+
+struct ArnArgs {
+    KState *st;
+    const double *in;     // w_j (or v for j = 0), unscaled
+    double *out;          // w_{j+1}
+    double *V64;
+    void *Vb;
+    double *partials;     // [(nk + 1) * G]
+    uint32_t n;           // box states (rows computed by the format)
+    uint32_t nv;          // vector length (n or n+1)
+    uint32_t tiles;
+    int j, istart, mode;  // mode 0: dots, 1: norm only (avnorm)
+    int G;
+};
+
+// Per-wave accumulation of up to kMaxKrylov+2 dot products in LDS.
+__device__ inline void wave_accum(double *acc, int k, double v)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x % kWave) == 0) acc[(threadIdx.x / kWave) * (kMaxKrylov + 2) + k] += v;
+}
+
+template <typename B>
+__device__ inline bool arn_skip(const ArnArgs &a)
+{
+    const KState *st = a.st;
+    if (!st->active || st->broke) return true;
+    if (a.mode == 1 && st->k1 == 0) return true;
+    return false;
+}
+
+template <typename B>
+__device__ inline void arn_row_epilogue(const ArnArgs &a, uint32_t i, double y, double xi,
+                                        double *acc)
+{
+    const bool live = i < a.n;
+    if (live) store_basis<B>(a.V64, (B *)a.Vb, a.nv, a.j, i, xi);
+    if (a.mode == 0) {
+        if (live) a.out[i] = y;
+        for (int k = a.istart; k <= a.j; ++k) {
+            const double vk = live ? (k == a.j ? xi : basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i)) : 0.0;
+            wave_accum(acc, k - a.istart, live ? y * vk : 0.0);
+        }
+    } else {
+        wave_accum(acc, 0, live ? y * y : 0.0);
+    }
+}
+
+template <typename B>
+__device__ inline void arn_block_end(const ArnArgs &a, double *acc, double sink, double *red)
+{
+    __syncthreads();
+    const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
+    for (int k = threadIdx.x; k < nk; k += kBlock) {
+        double s = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) s += acc[w * (kMaxKrylov + 2) + k];
+        a.partials[(size_t)k * a.G + blockIdx.x] = s;
+    }
+    const double t = block_sum(sink, red);
+    if (threadIdx.x == 0) a.partials[(size_t)nk * a.G + blockIdx.x] = t;
+}
+
+__device__ inline void acc_init(double *acc)
+{
+    for (int k = threadIdx.x; k < (kBlock / kWave) * (kMaxKrylov + 2); k += kBlock) acc[k] = 0.0;
+    __syncthreads();
+}
+
+template <typename V, typename B>
+__global__ void __launch_bounds__(kBlock) k_arn_csr(CsrF<V> A, ArnArgs a)
+{
+    if (arn_skip<B>(a)) return;
+    extern __shared__ double smem[];
+    double *acc = smem;
+    double *red = acc + (kBlock / kWave) * (kMaxKrylov + 2);
+    double *prod = red + kBlock / kWave;
+    acc_init(acc);
+    const KryX<B> X{a.in, a.st->sigma[a.j]};
+    double sink = 0.0;
+    for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
+        const double y = csr_tile<V>(A, t, X, prod, sink);
+        const uint32_t i = t * kBlock + threadIdx.x;
+        arn_row_epilogue<B>(a, i, y, i < a.n ? X(i) : 0.0, acc);
+    }
+    arn_block_end<B>(a, acc, sink, red);
+}
+
+template <typename V, typename B>
+__global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
+{
+    if (arn_skip<B>(a)) return;
+    extern __shared__ double smem[];
+    double *acc = smem;
+    double *red = acc + (kBlock / kWave) * (kMaxKrylov + 2);
+    acc_init(acc);
+    const KryX<B> X{a.in, a.st->sigma[a.j]};
+    double sink = 0.0;
+    for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        const double y = hyb_row<V>(A, i, X, sink);
+        arn_row_epilogue<B>(a, i, y, i < a.n ? X(i) : 0.0, acc);
+    }
+    arn_block_end<B>(a, acc, sink, red);
+}
+
+template <int D, typename V, typename B>
+__global__ void __launch_bounds__(kBlock) k_arn_kron(KronF<V> A, ArnArgs a, int use_lds)
+{
+    if (arn_skip<B>(a)) return;
+    extern __shared__ double smem[];
+    double *acc = smem;
+    double *red = acc + (kBlock / kWave) * (kMaxKrylov + 2);
+    V *tabl = (V *)(red + kBlock / kWave);
+    acc_init(acc);
+    const V *tab = A.tab_g;
+    if (use_lds && A.tab_g) {
+        stage_table(tabl, A.tab_g, A.tab_size);
+        tab = tabl;
+    }
+    const KryX<B> X{a.in, a.st->sigma[a.j]};
+    double sink = 0.0;
+    for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        const double y = kron_row<D, V>(A, i, X, tab, sink);
+        arn_row_epilogue<B>(a, i, y, i < a.n ? X(i) : 0.0, acc);
+    }
+    arn_block_end<B>(a, acc, sink, red);
+}
+
+// Finalize dots (mode 0) or the avnorm (mode 1).  Block k sums partials of
+// dot k in fixed order; the sink element (index n) is handled here.
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_arn_dots(ArnArgs a, int with_sink, int M2)
+{
+    if (arn_skip<B>(a)) return;
+    __shared__ double red[kBlock / kWave];
+    KState *st = a.st;
+    const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
+    double ys = 0.0, xs = 0.0;
+    if (with_sink) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < a.G; b += kBlock) s += a.partials[(size_t)nk * a.G + b];
+        ys = block_sum(s, red);
+        xs = (double)(B)(a.in[a.n] * st->sigma[a.j]);
+    }
+    const int k = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < a.G; b += kBlock) s += a.partials[(size_t)k * a.G + b];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) {
+        if (a.mode == 0) {
+            const int kk = a.istart + k;
+            if (with_sink) {
+                const double vk = (kk == a.j) ? xs : basis<B>(a.V64, (const B *)a.Vb, a.nv, kk, a.n);
+                s += ys * vk;
+            }
+            st->H[kk + a.j * M2] = s;
+            if (k == 0) {
+                st->n_matvec += 1;
+                if (with_sink) {
+                    a.out[a.n] = ys;
+                    store_basis<B>(a.V64, (B *)a.Vb, a.nv, a.j, a.n, xs);
+                }
+            }
+        } else {
+            st->avnorm = sqrt(s + ys * ys);
+            st->H[(a.j + 1) + a.j * M2] = 1.0;  // H(m+1, m) = 1 (src/KryExpvFSP.cu:115)
+            st->n_matvec += 1;
+            if (with_sink) store_basis<B>(a.V64, (B *)a.Vb, a.nv, a.j, a.n, xs);
+        }
+    }
+}
+
+// w_{j+1} -= V[istart..j] H(istart..j, j); partial |w|^2.
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2)
+{
+    if (arn_skip<B>(a)) return;
+    __shared__ double red[kBlock / kWave];
+    const KState *st = a.st;
+    double nrm = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += gridDim.x * kBlock) {
+        double acc = 0.0;
+        for (int k = a.istart; k <= a.j; ++k)
+            acc += basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) * st->H[k + a.j * M2];
+        const double w = -1.0 * acc + a.out[i];
+        a.out[i] = w;
+        nrm += w * w;
+    }
+    const double t = block_sum(nrm, red);
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kBlock) k_arn_norm(ArnArgs a, int M2, int G)
+{
+    KState *st = a.st;
+    if (!st->active || st->broke) return;
+    __shared__ double red[kBlock / kWave];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < G; b += kBlock) s += a.partials[b];
+    s = sqrt(block_sum(s, red));
+    if (threadIdx.x == 0) {
+        if (s < st->btol) {  // happy breakdown (src/KryExpvFSP.cu:90-98)
+            st->k1 = 0;
+            st->mb = a.j + 1;
+            st->tau = st->t_final - st->t_now;
+            st->broke = 1;
+            st->n_breakdown += 1;
+        } else {
+            st->H[(a.j + 1) + a.j * M2] = s;
+            st->sigma[a.j + 1] = 1.0 / s;
+        }
+    }
+}
+
+// ||v||^2 partials over the vector.
+__global__ void __launch_bounds__(kBlock) k_norm_partials(const double *__restrict__ v, uint32_t nv,
+                                                         double *partials, const KState *st)
+{
+    if (st && st->t_now >= st->t_final) return;
+    __shared__ double red[kBlock / kWave];
+    double s = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nv; i += gridDim.x * kBlock)
+        s += v[i] * v[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// Step prologue (src/KryExpvFSP.cu:41-65): beta = |v|, first step size,
+// tau = min(t_final - t_now, t_new).  Marks the step inactive when t_final
+// is reached, so steps launched ahead of the host check do nothing.
+__global__ void __launch_bounds__(kBlock) k_step_begin(KState *st, const double *partials, int G,
+                                                      int m, int M2)
+{
+    __shared__ double red[kBlock / kWave];
+    __shared__ int act;
+    if (threadIdx.x == 0) act = st->t_now < st->t_final;
+    __syncthreads();
+    if (!act) {
+        if (threadIdx.x == 0) st->active = 0;
+        return;
+    }
+    double s = 0.0;
+    for (int b = threadIdx.x; b < G; b += kBlock) s += partials[b];
+    s = block_sum(s, red);
+    for (int k = threadIdx.x; k < M2 * M2; k += kBlock) st->H[k] = 0.0;
+    if (threadIdx.x == 0) {
+        st->active = 1;
+        st->beta = sqrt(s);
+        if (st->i_step == 0) {
+            st->xm = 1.0 / (double)m;
+            st->t_new = (1.0 / st->anorm) *
+                        pow((st->fact * st->tol) / (4.0 * st->beta * st->anorm), st->xm);
+            st->btol = st->anorm * st->tol;
+        }
+        st->mb = m;
+        st->tau = fmin(st->t_final - st->t_now, st->t_new);
+        st->broke = 0;
+        st->sigma[0] = 1.0 / st->beta;
+    }
+}
+
+// --------------------------------------------------------------- dense expm
+// Golub & Van Loan, Matrix Computations, Alg. 11.3.1 (diagonal Pade q = 6,
+// scaling by 2^-j with j = max(0, 1 + floor(log2 |A|_inf)), Gaussian
+// elimination with partial pivoting, j squarings), column-major N x N in
+// LDS, one workgroup.  Same operation order as the oracle's orc_expm, so the
+// two agree bit for bit on equal input.  Result in Nm.
+__device__ void expm_block(int N, double *A, double *X, double *T, double *Nm, double *Dm,
+                           double *red)
+{
+    const int NN = N * N;
+    const int tid = threadIdx.x;
+    // infinity norm
+    double rs = 0.0;
+    if (tid < N)
+        for (int j = 0; j < N; ++j) rs += fabs(A[tid + j * N]);
+    __syncthreads();
+    if (tid < kBlock / kWave) red[tid] = 0.0;
+    __syncthreads();
+    // max over rows: every row sum goes through thread 0 in order
+    __shared__ double rows[64];
+    if (tid < N) rows[tid] = rs;
+    __syncthreads();
+    __shared__ int js_sh;
+    if (tid == 0) {
+        double nrm = 0.0;
+        for (int i = 0; i < N; ++i)
+            if (rows[i] > nrm) nrm = rows[i];
+        int js = 0;
+        if (nrm > 0.0) {
+            int e;
+            frexp(nrm, &e);
+            js = e > 0 ? e : 0;
+        }
+        js_sh = js;
+    }
+    __syncthreads();
+    const int js = js_sh;
+    const double scale = ldexp(1.0, -js);
+    for (int k = tid; k < NN; k += blockDim.x) {
+        A[k] = A[k] * scale;
+        const double id = ((k % N) == (k / N)) ? 1.0 : 0.0;
+        X[k] = id;
+        Nm[k] = id;
+        Dm[k] = id;
+    }
+    __syncthreads();
+    const int q = 6;
+    double c = 1.0;
+    for (int kk = 1; kk <= q; ++kk) {
+        c = c * (double)(q - kk + 1) / (double)((2 * q - kk + 1) * kk);
+        for (int e = tid; e < NN; e += blockDim.x) {
+            const int i = e % N, j = e / N;
+            double acc = 0.0;
+            for (int l = 0; l < N; ++l) acc += A[i + l * N] * X[l + j * N];
+            T[e] = acc;
+        }
+        __syncthreads();
+        const double sgn = (kk % 2) ? -1.0 : 1.0;
+        for (int e = tid; e < NN; e += blockDim.x) {
+            X[e] = T[e];
+            Nm[e] += c * T[e];
+            Dm[e] += sgn * c * T[e];
+        }
+        __syncthreads();
+    }
+    // solve Dm F = Nm
+    __shared__ int piv_sh;
+    for (int k = 0; k < N; ++k) {
+        if (tid == 0) {
+            int p = k;
+            double best = fabs(Dm[k + k * N]);
+            for (int i = k + 1; i < N; ++i)
+                if (fabs(Dm[i + k * N]) > best) {
+                    best = fabs(Dm[i + k * N]);
+                    p = i;
+                }
+            piv_sh = p;
+        }
+        __syncthreads();
+        const int p = piv_sh;
+        if (p != k) {
+            for (int j = tid; j < N; j += blockDim.x) {
+                double t = Dm[k + j * N];
+                Dm[k + j * N] = Dm[p + j * N];
+                Dm[p + j * N] = t;
+                t = Nm[k + j * N];
+                Nm[k + j * N] = Nm[p + j * N];
+                Nm[p + j * N] = t;
+            }
+        }
+        __syncthreads();
+        const double pv = Dm[k + k * N];
+        // multipliers into column k (rows below), then update
+        for (int i = k + 1 + tid; i < N; i += blockDim.x) T[i] = Dm[i + k * N] / pv;
+        __syncthreads();
+        for (int e = tid; e < N * N; e += blockDim.x) {
+            const int i = e % N, j = e / N;
+            if (i <= k) continue;
+            const double l = T[i];
+            if (j > k) Dm[i + j * N] -= l * Dm[k + j * N];
+            Nm[i + j * N] -= l * Nm[k + j * N];
+        }
+        __syncthreads();
+        for (int i = k + 1 + tid; i < N; i += blockDim.x) Dm[i + k * N] = T[i];
+        __syncthreads();
+    }
+    // back substitution, one right-hand side per thread
+    for (int j = tid; j < N; j += blockDim.x) {
+        for (int i = N - 1; i >= 0; --i) {
+            double acc = Nm[i + j * N];
+            for (int k = i + 1; k < N; ++k) acc -= Dm[i + k * N] * Nm[k + j * N];
+            Nm[i + j * N] = acc / Dm[i + i * N];
+        }
+    }
+    __syncthreads();
+    for (int sq = 0; sq < js; ++sq) {
+        for (int e = tid; e < NN; e += blockDim.x) {
+            const int i = e % N, j = e / N;
+            double acc = 0.0;
+            for (int l = 0; l < N; ++l) acc += Nm[i + l * N] * Nm[l + j * N];
+            T[e] = acc;
+        }
+        __syncthreads();
+        for (int e = tid; e < NN; e += blockDim.x) Nm[e] = T[e];
+        __syncthreads();
+    }
+}
+
+// src/KryExpvFSP.cu:148-150 and :172-174: s = 10^(floor(log10 t) - 1),
+// t = ceil(t / s) * s.  The decade is found by comparison with correctly
+// rounded decimal powers (built on the host with strtod) instead of
+// log10/pow, so the device and the host oracle round identically.
+__device__ inline double round_up_2sig(const KState *st, double t)
+{
+    if (!(t > 0.0) || isinf(t)) return t;
+    int lo = kP10Lo, hi = kP10Hi, e = kP10Lo;
+    while (lo <= hi) {
+        const int mid = (lo + hi) / 2;
+        if (st->p10[mid - kP10Lo] <= t) {
+            e = mid;
+            lo = mid + 1;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    if (e - 1 < kP10Lo) return t;
+    const double s = st->p10[e - 1 - kP10Lo];
+    return ceil(t / s) * s;
+}
+
+// Error estimate, reject loop and step update (src/KryExpvFSP.cu:122-174).
+__global__ void __launch_bounds__(kBlock) k_expm_control(KState *st, int m, int max_reject)
+{
+    if (!st->active) return;
+    extern __shared__ double smem[];
+    const int M2 = m + 2;
+    const int NN = M2 * M2;
+    double *A = smem, *X = A + NN, *T = X + NN, *Nm = T + NN, *Dm = Nm + NN;
+    __shared__ double red[kBlock / kWave];
+    const int mb = st->mb, k1 = st->k1;
+    const double beta = st->beta, avnorm = st->avnorm, btol = st->btol, tol = st->tol;
+    double tau = st->tau, xm = st->xm, err = 0.0;
+    int ireject = 0;
+    while (ireject < max_reject) {
+        const int mx = mb + k1;
+        for (int e = threadIdx.x; e < NN; e += blockDim.x) A[e] = tau * st->H[e];
+        __syncthreads();
+        expm_block(M2, A, X, T, Nm, Dm, red);
+        if (k1 == 0) {
+            err = btol;
+            break;
+        }
+        const double phi1 = fabs(beta * Nm[(mx - 2) + 0 * M2]);
+        const double phi2 = fabs(beta * Nm[(mx - 1) + 0 * M2] * avnorm);
+        if (phi1 > phi2 * 10.0) {
+            err = phi2;
+            xm = 1.0 / (double)mx;
+        } else if (phi1 > phi2) {
+            err = (phi1 * phi2) / (phi1 - phi2);
+            xm = 1.0 / (double)mx;
+        } else {
+            err = phi1;
+            xm = 1.0 / (double)(mx - 1);
+        }
+        if (err <= 1.2 * tau * tol) break;
+        tau = 0.9 * tau * pow(tau * tol / err, xm);
+        tau = round_up_2sig(st, tau);
+        ++ireject;
+        __syncthreads();
+    }
+    const int mx = mb + max(0, k1 - 1);
+    for (int k = threadIdx.x; k < M2; k += blockDim.x) st->F0[k] = (k < mx) ? Nm[k] : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->mx = mx;
+        st->n_reject += ireject;
+        if (ireject >= max_reject) st->status = FSP_ERR_NOT_CONVERGED;
+        st->err_loc = err;
+        st->xm = xm;
+        st->tau_last = tau;
+        if (st->tau_log && st->i_step < st->tau_cap) st->tau_log[st->i_step] = tau;
+        st->t_now = st->t_now + tau;
+        double tn = 0.9 * tau * pow(tau * tol / err, xm);
+        st->t_new = round_up_2sig(st, tn);
+        st->i_step += 1;
+    }
+}
+
+// v = beta * V[:, 0:mx] F0 (src/KryExpvFSP.cu:160-166).
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_solution(const KState *st, const double *__restrict__ V64,
+                                                    const B *__restrict__ Vb, uint32_t nv,
+                                                    double *__restrict__ v)
+{
+    if (!st->active) return;
+    const int mx = st->mx;
+    const double beta = st->beta;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nv; i += gridDim.x * kBlock) {
+        double acc = 0.0;
+        for (int k = 0; k < mx; ++k) acc += basis<B>(V64, Vb, nv, k, i) * st->F0[k];
+        v[i] = beta * acc;
+    }
+}
+
+// ---- callback operator path: explicit scale / zero / dots kernels ----
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_cb_scale(ArnArgs a)
+{
+    if (!a.st->active || a.st->broke) return;
+    const double sig = a.st->sigma[a.j];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += gridDim.x * kBlock) {
+        store_basis<B>(a.V64, (B *)a.Vb, a.nv, a.j, i, (double)(B)(a.in[i] * sig));
+        a.out[i] = 0.0;
+    }
+}
+
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_cb_dots(ArnArgs a)
+{
+    if (arn_skip<B>(a)) return;
+    __shared__ double acc[(kBlock / kWave) * (kMaxKrylov + 2)];
+    __shared__ double red[kBlock / kWave];
+    acc_init(acc);
+    for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        const bool live = i < a.nv;
+        const double y = live ? a.out[i] : 0.0;
+        if (a.mode == 0) {
+            for (int k = a.istart; k <= a.j; ++k)
+                wave_accum(acc, k - a.istart,
+                           live ? y * basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) : 0.0);
+        } else {
+            wave_accum(acc, 0, y * y);
+        }
+    }
+    arn_block_end<B>(a, acc, 0.0, red);
+}
+
+}  // namespace
+
+// ======================================================================= host
+static fsp_status expv_alloc(ExpvImpl &E)
+{
+    const size_t nv = (size_t)E.n;
+    FSP_HIP(hipMalloc(&E.d_st, sizeof(KState)));
+    FSP_HIP(hipHostMalloc(&E.h_st, sizeof(KState), hipHostMallocDefault));
+    std::memset(E.h_st, 0, sizeof(KState));
+    if (E.mixed) {
+        FSP_HIP(hipMalloc(&E.V64, sizeof(double) * nv));
+        FSP_HIP(hipMalloc(&E.V32, sizeof(float) * nv * (size_t)(E.m + 1)));
+    } else {
+        FSP_HIP(hipMalloc(&E.V64, sizeof(double) * nv * (size_t)(E.m + 2)));
+    }
+    FSP_HIP(hipMalloc(&E.W[0], sizeof(double) * nv));
+    FSP_HIP(hipMalloc(&E.W[1], sizeof(double) * nv));
+    E.tiles = ((int64_t)nv + kBlock - 1) / kBlock;
+    E.G = grid_for(E.tiles, 2048);
+    FSP_HIP(hipMalloc(&E.partials, sizeof(double) * (size_t)E.G * (kMaxKrylov + 4)));
+    KState &h = *E.h_st;
+    h.t_final = 0.0;
+    h.t_now = 0.0;
+    h.k1 = 2;
+    h.mb = E.m;
+    h.tol = E.tol;
+    h.anorm = E.anorm;
+    // fact = ((m+1)/e)^(m+1) sqrt(2 pi (m+1)) with pi written 3.1416 as in
+    // src/KryExpvFSP.cu:48; computed once on the host.
+    h.fact = std::pow((E.m + 1) / std::exp(1.0), E.m + 1) * std::sqrt(2 * (3.1416) * (E.m + 1));
+    for (int k = kP10Lo; k <= kP10Hi; ++k) {
+        char buf[16];
+        std::snprintf(buf, sizeof buf, "1e%d", k);
+        h.p10[k - kP10Lo] = std::strtod(buf, nullptr);
+    }
+    FSP_HIP(hipMalloc(&E.tau_log, sizeof(double) * (size_t)kTauLogCap));
+    h.tau_log = E.tau_log;
+    h.tau_cap = kTauLogCap;
+    FSP_HIP(hipMemcpy(E.d_st, &h, sizeof(KState), hipMemcpyHostToDevice));
+    return FSP_OK;
+}
+
+static void expv_free(ExpvImpl &E)
+{
+    if (E.gexec) hipGraphExecDestroy(E.gexec);
+    if (E.graph) hipGraphDestroy(E.graph);
+    if (E.cap) (void)hipStreamDestroy(E.cap);
+    void *ptrs[] = {E.d_st, E.V64, E.V32, E.W[0], E.W[1], E.partials, E.tau_log};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (E.h_st) hipHostFree(E.h_st);
+}
+
+template <typename B>
+static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
+{
+    MatImpl &M = *E.A;
+    const size_t acc_b = sizeof(double) * ((kBlock / kWave) * (kMaxKrylov + 2) + kBlock / kWave);
+    const bool mixed = M.prec == FSP_MIXED;
+    if (M.format == FSP_CSR) {
+        if (mixed) {
+            CsrF<float> A = csr_view<float>(M);
+            k_arn_csr<float, B><<<E.G, kBlock, acc_b + sizeof(double) * A.cap, E.s>>>(A, a);
+        } else {
+            CsrF<double> A = csr_view<double>(M);
+            k_arn_csr<double, B><<<E.G, kBlock, acc_b + sizeof(double) * A.cap, E.s>>>(A, a);
+        }
+    } else if (M.format == FSP_HYB) {
+        if (mixed)
+            k_arn_hyb<float, B><<<E.G, kBlock, acc_b, E.s>>>(hyb_view<float>(M), a);
+        else
+            k_arn_hyb<double, B><<<E.G, kBlock, acc_b, E.s>>>(hyb_view<double>(M), a);
+    } else {
+        if (mixed && M.hm.has_factors) {
+            KronF<float> A = kron_view<float>(M, -1, 0.0);
+            const size_t tb = sizeof(float) * (size_t)A.tab_size;
+            const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
+            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, float, B><<<E.G, kBlock, acc_b + (use_lds ? tb : 0), E.s>>>(A, a, use_lds)));
+        } else {
+            KronF<double> A = kron_view<double>(M, -1, 0.0);
+            const size_t tb = sizeof(double) * (size_t)A.tab_size;
+            const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
+            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, double, B><<<E.G, kBlock, acc_b + (use_lds ? tb : 0), E.s>>>(A, a, use_lds)));
+        }
+    }
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
+}
+
+// Enqueue one full step (no host synchronisation).
+template <typename B>
+static fsp_status enqueue_step(ExpvImpl &E, double *d_v)
+{
+    const int m = E.m, M2 = m + 2;
+    const uint32_t nv = (uint32_t)E.n;
+    const uint32_t nbox = E.A ? (uint32_t)E.A->n : nv;
+    const int with_sink = (E.A && E.A->with_sink) ? 1 : 0;
+    k_norm_partials<<<E.G, kBlock, 0, E.s>>>(d_v, nv, E.partials, E.d_st);
+    k_step_begin<<<1, kBlock, 0, E.s>>>(E.d_st, E.partials, E.G, m, M2);
+    ArnArgs a;
+    a.st = E.d_st;
+    a.V64 = E.V64;
+    a.Vb = E.mixed ? (void *)E.V32 : (void *)E.V64;
+    a.partials = E.partials;
+    a.n = nbox;
+    a.nv = nv;
+    a.tiles = (uint32_t)E.tiles;
+    a.G = E.G;
+    int istart = 0;
+    for (int j = 0; j <= m; ++j) {
+        const bool last = (j == m);
+        if (!last && E.iop) istart = (j >= E.q_iop - 1) ? j - E.q_iop + 1 : 0;
+        a.in = (j == 0) ? d_v : E.W[j & 1];
+        a.out = E.W[(j + 1) & 1];
+        a.j = j;
+        a.istart = last ? 0 : istart;
+        a.mode = last ? 1 : 0;
+        const int nk = last ? 1 : j - istart + 1;
+        if (E.A) {
+            FSP_TRY(launch_arn_spmv<B>(E, a));
+            k_arn_dots<B><<<nk, kBlock, 0, E.s>>>(a, with_sink, M2);
+        } else {
+            k_cb_scale<B><<<E.G, kBlock, 0, E.s>>>(a);
+            FSP_HIP(hipGetLastError());
+            E.cb(E.ctx, a.V64 + (size_t)j * nv, a.out);
+            k_cb_dots<B><<<E.G, kBlock, 0, E.s>>>(a);
+            k_arn_dots<B><<<nk, kBlock, 0, E.s>>>(a, 0, M2);
+        }
+        FSP_HIP(hipGetLastError());
+        if (last) break;
+        k_arn_update<B><<<E.G, kBlock, 0, E.s>>>(a, M2);
+        k_arn_norm<<<1, kBlock, 0, E.s>>>(a, M2, E.G);
+        FSP_HIP(hipGetLastError());
+    }
+    const size_t lds = sizeof(double) * 5 * (size_t)M2 * M2;
+    k_expm_control<<<1, kBlock, lds, E.s>>>(E.d_st, m, E.max_reject);
+    k_solution<B><<<E.G, kBlock, 0, E.s>>>(E.d_st, E.V64, (const B *)(E.mixed ? (void *)E.V32 : (void *)E.V64), nv, d_v);
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
+}
+
+static fsp_status enqueue_any(ExpvImpl &E, double *d_v)
+{
+    return E.mixed ? enqueue_step<float>(E, d_v) : enqueue_step<double>(E, d_v);
+}
+
+static fsp_status launch_step(ExpvImpl &E, double *d_v)
+{
+    if (!E.A || !E.use_graph) return enqueue_any(E, d_v);
+    if (!E.gexec || E.graph_v != d_v) {
+        if (E.gexec) hipGraphExecDestroy(E.gexec);
+        if (E.graph) hipGraphDestroy(E.graph);
+        E.gexec = nullptr;
+        E.graph = nullptr;
+        if (!E.cap) FSP_HIP(hipStreamCreateWithFlags(&E.cap, hipStreamNonBlocking));
+        // the step's kernels are recorded on the private stream, then the
+        // graph is launched on the caller's stream
+        FSP_HIP(hipStreamSynchronize(E.s));
+        hipStream_t user = E.s;
+        E.s = E.cap;
+        FSP_HIP(hipStreamBeginCapture(E.cap, hipStreamCaptureModeThreadLocal));
+        fsp_status st = enqueue_any(E, d_v);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(E.cap, &g);
+        E.s = user;
+        FSP_TRY(st);
+        if (ce != hipSuccess) return fail(FSP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+        E.graph = g;
+        FSP_HIP(hipGraphInstantiate(&E.gexec, g, nullptr, nullptr, 0));
+        E.graph_v = d_v;
+    }
+    FSP_HIP(hipGraphLaunch(E.gexec, E.s));
+    return FSP_OK;
+}
+
+static fsp_status pull_state(ExpvImpl &E)
+{
+    FSP_HIP(hipMemcpyAsync(E.h_st, E.d_st, sizeof(KState), hipMemcpyDeviceToHost, E.s));
+    FSP_HIP(hipStreamSynchronize(E.s));
+    if (E.h_st->status != FSP_OK)
+        return fail((fsp_status)E.h_st->status, "Krylov reject loop exhausted max_reject");
+    return FSP_OK;
+}
+
+static fsp_status push_state(ExpvImpl &E)
+{
+    FSP_HIP(hipMemcpyAsync(E.d_st, E.h_st, sizeof(KState), hipMemcpyHostToDevice, E.s));
+    FSP_HIP(hipStreamSynchronize(E.s));
+    return FSP_OK;
+}
+
+}  // namespace fsp
+
+// ===================================================================== C ABI
+using fsp::ExpvImpl;
+
+struct fsp_expv_s {
+    ExpvImpl impl;
+};
+
+extern "C" fsp_status fsp_expv_create(fsp_mat A, fsp_matvec_cb cb, void *cb_ctx, int64_t n,
+                                      double t_final, int m, double tol, int iop, int q_iop,
+                                      double anorm, void *stream, fsp_expv *out);
+extern "C" fsp_status fsp_mat_internal(fsp_mat A, fsp::MatImpl **impl);
+
+extern "C" fsp_status fsp_expv_create(fsp_mat A, fsp_matvec_cb cb, void *cb_ctx, int64_t n,
+                                      double t_final, int m, double tol, int iop, int q_iop,
+                                      double anorm, void *stream, fsp_expv *out)
+{
+    using namespace fsp;
+    if (!out) return fail(FSP_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (m < 1 || m > kMaxKrylov) return fail(FSP_ERR_UNSUPPORTED, "m must be in 1..50");
+    if (!A && !cb) return fail(FSP_ERR_INVALID, "need an fsp_mat or a matvec callback");
+    if (n < 1) return fail(FSP_ERR_INVALID, "n < 1");
+    if (n >= 2147483647LL) return fail(FSP_ERR_OVERFLOW, "vector too long");
+    if (!(tol > 0.0) || !(anorm > 0.0)) return fail(FSP_ERR_INVALID, "tol and anorm must be > 0");
+    if (iop && q_iop < 1) return fail(FSP_ERR_INVALID, "q_iop < 1");
+    auto *h = new fsp_expv_s();
+    ExpvImpl &E = h->impl;
+    if (A) {
+        FSP_TRY(fsp_mat_internal(A, &E.A));
+        if (n != E.A->n_rows) {
+            delete h;
+            return fail(FSP_ERR_INVALID, "n differs from the matrix rows");
+        }
+        E.mixed = E.A->prec == FSP_MIXED;
+    }
+    E.cb = cb;
+    E.ctx = cb_ctx;
+    E.n = n;
+    E.m = m;
+    E.tol = tol;
+    E.iop = iop;
+    E.q_iop = q_iop;
+    E.anorm = anorm;
+    E.s = (hipStream_t)stream;
+    fsp_status st = expv_alloc(E);
+    if (st != FSP_OK) {
+        expv_free(E);
+        delete h;
+        return st;
+    }
+    E.h_st->t_final = t_final;
+    st = push_state(E);
+    if (st != FSP_OK) {
+        expv_free(E);
+        delete h;
+        return st;
+    }
+    *out = h;
+    return FSP_OK;
+}
+
+extern "C" fsp_status fsp_expv_destroy(fsp_expv e)
+{
+    if (!e) return FSP_OK;
+    fsp::expv_free(e->impl);
+    delete e;
+    return FSP_OK;
+}
+
+extern "C" fsp_status fsp_expv_step(fsp_expv e, double *d_v)
+{
+    using namespace fsp;
+    if (!e || !d_v) return fail(FSP_ERR_INVALID, "NULL argument");
+    ExpvImpl &E = e->impl;
+    FSP_TRY(launch_step(E, d_v));
+    return pull_state(E);
+}
+
+extern "C" fsp_status fsp_expv_solve(fsp_expv e, double *d_v)
+{
+    using namespace fsp;
+    if (!e || !d_v) return fail(FSP_ERR_INVALID, "NULL argument");
+    ExpvImpl &E = e->impl;
+    FSP_TRY(pull_state(E));
+    // Steps are launched ahead of the host check (inactive steps do nothing).
+    const int ahead = E.A ? 4 : 1;
+    while (E.h_st->t_now < E.h_st->t_final) {
+        for (int k = 0; k < ahead; ++k) FSP_TRY(launch_step(E, d_v));
+        FSP_TRY(pull_state(E));
+        if (E.h_st->i_step > E.max_nstep)
+            return fail(FSP_ERR_NOT_CONVERGED, "Krylov step count exceeded max_nstep");
+    }
+    return FSP_OK;
+}
+
+extern "C" fsp_status fsp_expv_reset_time(fsp_expv e)
+{
+    using namespace fsp;
+    if (!e) return fail(FSP_ERR_INVALID, "NULL argument");
+    FSP_TRY(pull_state(e->impl));
+    e->impl.h_st->t_now = 0.0;
+    return push_state(e->impl);
+}
+
+extern "C" fsp_status fsp_expv_restart(fsp_expv e, double t_final)
+{
+    using namespace fsp;
+    if (!e) return fail(FSP_ERR_INVALID, "NULL argument");
+    FSP_TRY(pull_state(e->impl));
+    KState &h = *e->impl.h_st;
+    h.t_now = 0.0;
+    h.t_final = t_final;
+    h.t_new = 0.0;
+    h.i_step = 0;
+    h.k1 = 2;
+    h.status = 0;
+    h.n_reject = h.n_breakdown = h.n_matvec = 0;
+    return push_state(e->impl);
+}
+
+extern "C" fsp_status fsp_expv_set_final_time(fsp_expv e, double t_final)
+{
+    using namespace fsp;
+    if (!e) return fail(FSP_ERR_INVALID, "NULL argument");
+    FSP_TRY(pull_state(e->impl));
+    e->impl.h_st->t_final = t_final;
+    e->impl.h_st->k1 = 2;
+    return push_state(e->impl);
+}
+
+extern "C" fsp_status fsp_expv_final_time_reached(fsp_expv e, int *reached)
+{
+    using namespace fsp;
+    if (!e || !reached) return fail(FSP_ERR_INVALID, "NULL argument");
+    FSP_TRY(pull_state(e->impl));
+    *reached = e->impl.h_st->t_now >= e->impl.h_st->t_final;
+    return FSP_OK;
+}
+
+extern "C" fsp_status fsp_expv_get_stats(fsp_expv e, fsp_expv_stats *s)
+{
+    using namespace fsp;
+    if (!e || !s) return fail(FSP_ERR_INVALID, "NULL argument");
+    FSP_TRY(pull_state(e->impl));
+    const KState &h = *e->impl.h_st;
+    s->t_now = h.t_now;
+    s->t_new = h.t_new;
+    s->tau_last = h.tau_last;
+    s->err_last = h.err_loc;
+    s->beta_last = h.beta;
+    s->i_step = h.i_step;
+    s->n_matvec = h.n_matvec;
+    s->n_reject = h.n_reject;
+    s->n_breakdown = h.n_breakdown;
+    return FSP_OK;
+}
+
+extern "C" fsp_status fsp_expv_get_tau_history(fsp_expv e, double *h_taus, int64_t cap,
+                                               int64_t *count)
+{
+    using namespace fsp;
+    if (!e || !count) return fail(FSP_ERR_INVALID, "NULL argument");
+    FSP_TRY(pull_state(e->impl));
+    const int64_t n = std::min<int64_t>(e->impl.h_st->i_step, kTauLogCap);
+    *count = n;
+    if (h_taus && cap > 0 && n > 0)
+        FSP_HIP(hipMemcpy(h_taus, e->impl.tau_log, sizeof(double) * (size_t)std::min(n, cap),
+                          hipMemcpyDeviceToHost));
+    return FSP_OK;
+}
+
+extern "C" fsp_status fsp_expv_set_limits(fsp_expv e, int max_reject, int64_t max_nstep)
+{
+    using namespace fsp;
+    if (!e || max_reject < 1 || max_nstep < 1) return fail(FSP_ERR_INVALID, "bad limits");
+    ExpvImpl &E = e->impl;
+    E.max_reject = max_reject;
+    E.max_nstep = max_nstep;
+    if (E.gexec) hipGraphExecDestroy(E.gexec);
+    if (E.graph) hipGraphDestroy(E.graph);
+    E.gexec = nullptr;
+    E.graph = nullptr;
+    return FSP_OK;
+}
+
+namespace {
+__global__ void __launch_bounds__(fsp::kBlock) k_expm_dense(int N, const double *Ag, double *Fg)
+{
+    extern __shared__ double smem[];
+    const int NN = N * N;
+    double *A = smem, *X = A + NN, *T = X + NN, *Nm = T + NN, *Dm = Nm + NN;
+    __shared__ double red[fsp::kBlock / fsp::kWave];
+    for (int e = threadIdx.x; e < NN; e += blockDim.x) A[e] = Ag[e];
+    __syncthreads();
+    fsp::expm_block(N, A, X, T, Nm, Dm, red);
+    for (int e = threadIdx.x; e < NN; e += blockDim.x) Fg[e] = Nm[e];
+}
+}  // namespace
+
+extern "C" fsp_status fsp_expm_dense(int N, const double *d_A, double *d_F, void *stream)
+{
+    using namespace fsp;
+    if (N < 1 || N > kMaxKrylov + 2) return fail(FSP_ERR_UNSUPPORTED, "N must be in 1..52");
+    const size_t lds = sizeof(double) * 5 * (size_t)N * N;
+    k_expm_dense<<<1, kBlock, lds, (hipStream_t)stream>>>(N, d_A, d_F);
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
+}
+

@@ -1,0 +1,829 @@
+// fsp_mat.hip -- generator assembly and apply (CSR, HYB, KRON) for gfx950.
+//
+// Replaces generate_fsp_mats_cuda_csr / fspmat_csr_* (src/fspmat_csr_kernels.cu),
+// generate_fsp_mats_hyb / fspmat_hyb_* (src/fspmat_hyb_kernels.cu) and
+// generate_fsp_mats_sdkron / fspmat_sdkron_mv (src/fspmat_kron_kernels.cu).
+// All assembly runs on the device: one fused count kernel, an own int scan,
+// and one fill kernel per format, instead of the reference's per-reaction
+// launch + Thrust scan + host copy loop.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <type_traits>
+
+#include "fsp_formats.h"
+
+namespace fsp {
+
+// ===================================================================== model
+fsp_status build_host_model(const fsp_model *m, HostModel &hm)
+{
+    if (!m) return fail(FSP_ERR_INVALID, "model is NULL");
+    const int d = m->n_species, R = m->n_reactions;
+    if (d < 1 || d > kMaxSpecies)
+        return fail(FSP_ERR_UNSUPPORTED, "n_species must be in 1.." + std::to_string(kMaxSpecies));
+    if (R < 1 || R > kMaxReactions)
+        return fail(FSP_ERR_UNSUPPORTED, "n_reactions must be in 1.." + std::to_string(kMaxReactions));
+    if (!m->bounds || !m->stoich_vals || !m->stoich_cols || !m->stoich_rowptrs)
+        return fail(FSP_ERR_INVALID, "bounds and stoichiometry are required");
+    if ((m->factors == nullptr) == (m->prop_table_dev == nullptr))
+        return fail(FSP_ERR_INVALID, "give exactly one of factors / prop_table_dev");
+    DevModel &dm = hm.dm;
+    std::memset(&dm, 0, sizeof(dm));
+    dm.d = d;
+    dm.R = R;
+    int64_t n = 1;
+    for (int s = 0; s < d; ++s) {
+        if (m->bounds[s] < 0) return fail(FSP_ERR_INVALID, "negative bound");
+        dm.radix[s] = m->bounds[s] + 1;
+        dm.stride[s] = n;
+        dm.fd[s] = make_fastdiv((uint32_t)dm.radix[s]);
+        n *= dm.radix[s];
+        if (n > 2147483647LL)
+            return fail(FSP_ERR_OVERFLOW, "more than 2^31-1 states in the FSP box");
+    }
+    dm.n = n;
+    dm.use_table = m->prop_table_dev != nullptr;
+    hm.prop_table_dev = m->prop_table_dev;
+    hm.has_factors = m->factors != nullptr;
+    hm.nu_dense.assign((size_t)R * d, 0);
+    for (int r = 0; r < R; ++r) {
+        if (m->stoich_rowptrs[r + 1] < m->stoich_rowptrs[r])
+            return fail(FSP_ERR_INVALID, "stoich row pointers decrease");
+        for (int k = m->stoich_rowptrs[r]; k < m->stoich_rowptrs[r + 1]; ++k) {
+            const int c = m->stoich_cols[k];
+            if (c < 0 || c >= d) return fail(FSP_ERR_INVALID, "stoich column out of range");
+            hm.nu_dense[(size_t)r * d + c] += m->stoich_vals[k];
+        }
+    }
+    int foff_stride = 0;
+    for (int s = 0; s < d; ++s) foff_stride += dm.radix[s];
+    hm.compact.clear();
+    for (int r = 0; r < R; ++r) {
+        RxnDev &rx = dm.rx[r];
+        rx.delta = 0;
+        rx.zero_nu = 1;
+        rx.nsp = 0;
+        int off = 0;
+        for (int s = 0; s < d; ++s) {
+            const int nu = hm.nu_dense[(size_t)r * d + s];
+            rx.delta += (int64_t)nu * dm.stride[s];
+            if (nu != 0) rx.zero_nu = 0;
+            bool trivial = true;
+            const double *f = nullptr;
+            if (m->factors) {
+                f = m->factors + (size_t)r * foff_stride + off;
+                for (int i = 0; i < dm.radix[s]; ++i) trivial = trivial && (f[i] == 1.0);
+            }
+            if (nu != 0 || !trivial) {
+                const int k = rx.nsp++;
+                rx.sp[k] = s;
+                rx.nu[k] = nu;
+                rx.foff[k] = -1;
+                if (!trivial) {
+                    rx.foff[k] = (int)hm.compact.size();
+                    hm.compact.insert(hm.compact.end(), f, f + dm.radix[s]);
+                }
+            }
+            off += dm.radix[s];
+        }
+    }
+    dm.compact_size = (int)hm.compact.size();
+    // unique net-change slots in order of first appearance
+    int U = 0;
+    std::vector<int> first;
+    for (int r = 0; r < R; ++r) {
+        dm.rx[r].slot = -1;
+        if (dm.rx[r].zero_nu) continue;
+        int slot = -1;
+        for (int u = 0; u < U && slot < 0; ++u) {
+            bool same = true;
+            for (int s = 0; s < d; ++s)
+                same = same && hm.nu_dense[(size_t)first[u] * d + s] == hm.nu_dense[(size_t)r * d + s];
+            if (same) slot = u;
+        }
+        if (slot < 0) {
+            first.push_back(r);
+            slot = U++;
+        }
+        dm.rx[r].slot = slot;
+    }
+    dm.U = U;
+    int pos = 0;
+    for (int u = 0; u < U; ++u) {
+        dm.slot_ptr[u] = pos;
+        dm.slot_delta[u] = dm.rx[first[u]].delta;
+        for (int r = 0; r < R; ++r)
+            if (dm.rx[r].slot == u) dm.slot_rxn[pos++] = r;
+    }
+    dm.slot_ptr[U] = pos;
+    // column order of a merged row: column i - delta ascending = delta descending
+    std::vector<std::pair<int64_t, int>> ent;
+    ent.push_back({0, -1});
+    for (int u = 0; u < U; ++u) ent.push_back({dm.slot_delta[u], u});
+    std::sort(ent.begin(), ent.end(), [](auto &a, auto &b) { return a.first > b.first; });
+    for (int k = 0; k <= U; ++k) dm.csr_order[k] = ent[k].second;
+    return FSP_OK;
+}
+
+// ====================================================================== scan
+namespace {
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__global__ void k_scan_tiles(const int *__restrict__ cnt, int64_t n, long long *tile_sums)
+{
+    __shared__ double red[kBlock / kWave];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    long long s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+        if (i < n) s += cnt[i];
+    }
+    // exact in double for counts < 2^53
+    const double t = block_sum((double)s, red);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = (long long)t;
+}
+
+__global__ void k_scan_top(long long *tile_sums, int64_t tiles, long long *total)
+{
+    // single workgroup: exclusive scan in place, chunks of kBlock
+    __shared__ long long buf[kBlock];
+    long long carry = 0;
+    for (int64_t c0 = 0; c0 < tiles; c0 += kBlock) {
+        const int64_t i = c0 + threadIdx.x;
+        long long v = (i < tiles) ? tile_sums[i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < kBlock; off <<= 1) {
+            long long a = (threadIdx.x >= (unsigned)off) ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += a;
+            __syncthreads();
+        }
+        if (i < tiles) tile_sums[i] = carry + buf[threadIdx.x] - v;
+        carry += buf[kBlock - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_scan_apply(const int *__restrict__ cnt, int64_t n,
+                             const long long *__restrict__ tile_off, int *out)
+{
+    __shared__ long long buf[kBlock];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    long long loc[kScanItems];
+    long long s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k;
+        s += (i < n) ? cnt[i] : 0;
+        loc[k] = s;
+    }
+    buf[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        long long a = (threadIdx.x >= (unsigned)off) ? buf[threadIdx.x - off] : 0;
+        __syncthreads();
+        buf[threadIdx.x] += a;
+        __syncthreads();
+    }
+    const long long pre = tile_off[blockIdx.x] + buf[threadIdx.x] - s;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k;
+        if (i < n) out[i + 1] = (int)(pre + loc[k]);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+}
+}  // namespace
+
+fsp_status scan_counts(const int *d_counts, int *d_out, int64_t n, int64_t *h_total,
+                       hipStream_t s)
+{
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    long long *tmp = nullptr;
+    FSP_HIP(hipMalloc(&tmp, sizeof(long long) * (size_t)(tiles + 1)));
+    if (tiles > 0) {
+        k_scan_tiles<<<(unsigned)tiles, kBlock, 0, s>>>(d_counts, n, tmp);
+        k_scan_top<<<1, kBlock, 0, s>>>(tmp, tiles, tmp + tiles);
+        k_scan_apply<<<(unsigned)tiles, kBlock, 0, s>>>(d_counts, n, tmp, d_out);
+    } else {
+        FSP_HIP(hipMemsetAsync(d_out, 0, sizeof(int), s));
+        FSP_HIP(hipMemsetAsync(tmp, 0, sizeof(long long), s));
+    }
+    long long total = 0;
+    FSP_HIP(hipMemcpyAsync(&total, tmp + tiles, sizeof(long long), hipMemcpyDeviceToHost, s));
+    FSP_HIP(hipStreamSynchronize(s));
+    FSP_HIP(hipGetLastError());
+    hipFree(tmp);
+    *h_total = total;
+    if (total > 2147483647LL) return fail(FSP_ERR_OVERFLOW, "more than 2^31-1 nonzeros");
+    return FSP_OK;
+}
+
+// ================================================================ assembly
+namespace {
+
+template <int D>
+__device__ inline double prop_at(const DevModel *__restrict__ dm, const double *tab,
+                                 const double *__restrict__ table, int r, uint32_t i,
+                                 const int (&xs)[D], int shift)
+{
+    if (table) {
+        const uint32_t idx = shift ? (uint32_t)((int64_t)i - dm->rx[r].delta) : i;
+        return table[(size_t)r * dm->n + idx];
+    }
+    return prop_tab<D>(dm->rx[r], xs, tab, shift);
+}
+
+// Stage the compact FP64 factor table in LDS when it fits, else read global.
+__device__ inline const double *table_ptr(const DevModel *__restrict__ dm,
+                                          const double *__restrict__ g, double *lds, int use_lds)
+{
+    if (!g) return nullptr;
+    if (use_lds) {
+        stage_table(lds, g, dm->compact_size);
+        return lds;
+    }
+    return g;
+}
+
+// Merged CSR: per-row entry count and sink flag.
+template <int D>
+__global__ void k_count_merged(const DevModel *__restrict__ dm, uint32_t n, int diag_in_row,
+                               int *__restrict__ counts, int *__restrict__ sink_cnt)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    int xs[D];
+    decode<D>(i, dm, xs);
+    int c = diag_in_row;
+    for (int u = 0; u < dm->U; ++u)
+        c += source_in_box<D>(dm->rx[dm->slot_rxn[dm->slot_ptr[u]]], xs, dm) ? 1 : 0;
+    counts[i] = c;
+    if (sink_cnt) {
+        int has = 0;
+        for (int r = 0; r < dm->R; ++r) has |= target_leaves<D>(dm->rx[r], xs, dm) ? 1 : 0;
+        sink_cnt[i] = has;
+    }
+}
+
+// Row values shared by CSR and HYB fills.  F64 (mixed = 0): slot value =
+// sum_{r in u} c_r a_r(x - nu) (first term assigned, reaction order),
+// diag = -(sum_r c_r a_r(x)), out = sum_{r leaves} c_r a_r(x).  Mixed: the
+// per-slot outflow rates w_u(x) = sum_{r in u} c_r a_r(x) are rounded to
+// FP32 once, and diag / out are exact FP64 sums of those rounded rates, so
+// each column of the stored generator sums to exactly zero.
+template <int D>
+__device__ inline double slot_value(const DevModel *__restrict__ dm, const double *tab,
+                                    const double *__restrict__ table,
+                                    const double *__restrict__ coefs, int u, uint32_t i,
+                                    const int (&xs)[D], int shift)
+{
+    double v = 0.0;
+    for (int k = dm->slot_ptr[u]; k < dm->slot_ptr[u + 1]; ++k) {
+        const int r = dm->slot_rxn[k];
+        const double t = coefs[r] * prop_at<D>(dm, tab, table, r, i, xs, shift);
+        v = (k == dm->slot_ptr[u]) ? t : v + t;
+    }
+    return v;
+}
+
+template <int D>
+__device__ inline void diag_and_out(const DevModel *__restrict__ dm, const double *tab,
+                                    const double *__restrict__ table,
+                                    const double *__restrict__ coefs, int mixed, uint32_t i,
+                                    const int (&xs)[D], double &diag, double &out, int &has)
+{
+    double acc = 0.0, o = 0.0;
+    has = 0;
+    if (!mixed) {
+        for (int r = 0; r < dm->R; ++r) {
+            if (dm->rx[r].zero_nu) continue;
+            const double t = coefs[r] * prop_at<D>(dm, tab, table, r, i, xs, 0);
+            acc += t;
+            if (target_leaves<D>(dm->rx[r], xs, dm)) {
+                o += t;
+                has = 1;
+            }
+        }
+    } else {
+        for (int u = 0; u < dm->U; ++u) {
+            const double w = (double)(float)slot_value<D>(dm, tab, table, coefs, u, i, xs, 0);
+            acc += w;
+            if (target_leaves<D>(dm->rx[dm->slot_rxn[dm->slot_ptr[u]]], xs, dm)) {
+                o += w;
+                has = 1;
+            }
+        }
+    }
+    diag = -acc;
+    out = o;
+}
+
+template <int D, typename V>
+__global__ void k_fill_merged(const DevModel *__restrict__ dm, const double *__restrict__ tab_g,
+                              int use_lds, const double *__restrict__ table,
+                              const double *__restrict__ coefs, uint32_t n, int mixed,
+                              const int *__restrict__ rp, int *__restrict__ cols,
+                              V *__restrict__ vals, double *__restrict__ diag_out,
+                              const int *__restrict__ sink_off, int *__restrict__ sink_cols,
+                              double *__restrict__ sink_vals)
+{
+    extern __shared__ double smem[];
+    const double *tab = table_ptr(dm, tab_g, smem, use_lds);
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    int xs[D];
+    decode<D>(i, dm, xs);
+    double dg, out;
+    int has;
+    diag_and_out<D>(dm, tab, table, coefs, mixed, i, xs, dg, out, has);
+    int pos = rp[i];
+    for (int k = 0; k <= dm->U; ++k) {
+        const int u = dm->csr_order[k];
+        if (u < 0) {
+            if (!mixed) {
+                cols[pos] = (int)i;
+                vals[pos] = (V)dg;
+                ++pos;
+            } else {
+                diag_out[i] = dg;
+            }
+            continue;
+        }
+        if (!source_in_box<D>(dm->rx[dm->slot_rxn[dm->slot_ptr[u]]], xs, dm)) continue;
+        const double v = slot_value<D>(dm, tab, table, coefs, u, i, xs, 1);
+        cols[pos] = (int)((int64_t)i - dm->slot_delta[u]);
+        vals[pos] = (V)v;
+        ++pos;
+    }
+    if (sink_off && has) {
+        const int p = sink_off[i];
+        sink_cols[p] = (int)i;
+        sink_vals[p] = out;
+    }
+}
+
+template <int D, typename V>
+__global__ void k_fill_hyb(const DevModel *__restrict__ dm, const double *__restrict__ tab_g,
+                           int use_lds, const double *__restrict__ table,
+                           const double *__restrict__ coefs, uint32_t n, int mixed,
+                           double *__restrict__ diag, V *__restrict__ slot_vals,
+                           int *__restrict__ slot_cols, double *__restrict__ sink_out)
+{
+    extern __shared__ double smem[];
+    const double *tab = table_ptr(dm, tab_g, smem, use_lds);
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    int xs[D];
+    decode<D>(i, dm, xs);
+    double dg, out;
+    int has;
+    diag_and_out<D>(dm, tab, table, coefs, mixed, i, xs, dg, out, has);
+    diag[i] = dg;
+    if (sink_out) sink_out[i] = out;
+    for (int u = 0; u < dm->U; ++u) {
+        const size_t o = (size_t)u * n + i;
+        if (source_in_box<D>(dm->rx[dm->slot_rxn[dm->slot_ptr[u]]], xs, dm)) {
+            slot_vals[o] = (V)slot_value<D>(dm, tab, table, coefs, u, i, xs, 1);
+            slot_cols[o] = (int)((int64_t)i - dm->slot_delta[u]);
+        } else {
+            slot_vals[o] = (V)0.0;
+            slot_cols[o] = (int)i;
+        }
+    }
+}
+
+// Reference-layout per-reaction CSR (src/fspmat_csr_kernels.cu:7-94).
+template <int D>
+__global__ void k_term_count(const DevModel *__restrict__ dm, int r, uint32_t n, int *counts)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    int xs[D];
+    decode<D>(i, dm, xs);
+    counts[i] = 1 + (source_in_box<D>(dm->rx[r], xs, dm) ? 1 : 0);
+}
+
+template <int D>
+__global__ void k_term_fill(const DevModel *__restrict__ dm, const double *__restrict__ tab_g,
+                            int use_lds, const double *__restrict__ table, int r, uint32_t n,
+                            const int *__restrict__ rp, int *__restrict__ cols,
+                            double *__restrict__ vals)
+{
+    extern __shared__ double smem[];
+    const double *tab = table_ptr(dm, tab_g, smem, use_lds);
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    int xs[D];
+    decode<D>(i, dm, xs);
+    const int p = rp[i];
+    const double a_here = prop_at<D>(dm, tab, table, r, i, xs, 0);
+    if (source_in_box<D>(dm->rx[r], xs, dm)) {
+        const int64_t j = (int64_t)i - dm->rx[r].delta;
+        const int idg = (j > (int64_t)i) ? p : p + 1;
+        const int iof = (j > (int64_t)i) ? p + 1 : p;
+        cols[idg] = (int)i;
+        cols[iof] = (int)j;
+        if (vals) {
+            vals[idg] = -1.0 * a_here;
+            vals[iof] = prop_at<D>(dm, tab, table, r, i, xs, 1);
+        }
+    } else {
+        cols[p] = (int)i;
+        if (vals) vals[p] = -1.0 * a_here;
+    }
+}
+
+}  // namespace
+
+static fsp_status upload_model(MatImpl &M)
+{
+    FSP_HIP(hipMalloc(&M.d_model, sizeof(DevModel)));
+    FSP_HIP(hipMemcpy(M.d_model, &M.hm.dm, sizeof(DevModel), hipMemcpyHostToDevice));
+    if (!M.hm.compact.empty()) {
+        FSP_HIP(hipMalloc(&M.d_compact, sizeof(double) * M.hm.compact.size()));
+        FSP_HIP(hipMemcpy(M.d_compact, M.hm.compact.data(), sizeof(double) * M.hm.compact.size(),
+                          hipMemcpyHostToDevice));
+    }
+    FSP_HIP(hipMalloc(&M.d_coefs, sizeof(double) * (size_t)M.hm.dm.R));
+    FSP_HIP(hipMemcpy(M.d_coefs, M.coefs.data(), sizeof(double) * (size_t)M.hm.dm.R,
+                      hipMemcpyHostToDevice));
+    return FSP_OK;
+}
+
+static size_t table_lds_bytes(const HostModel &hm, int &use_lds)
+{
+    const size_t b = sizeof(double) * hm.compact.size();
+    use_lds = (!hm.compact.empty() && b <= kLdsTableMax) ? 1 : 0;
+    return use_lds ? b : 0;
+}
+
+static fsp_status fill_values(MatImpl &M, hipStream_t s)
+{
+    const DevModel &dm = M.hm.dm;
+    const uint32_t n = (uint32_t)M.n;
+    const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    int use_lds = 0;
+    const size_t lds = table_lds_bytes(M.hm, use_lds);
+    const double *tab = M.d_compact;
+    const double *table = M.hm.prop_table_dev;
+    const int mixed = M.prec == FSP_MIXED;
+    FSP_HIP(hipMemcpyAsync(M.d_coefs, M.coefs.data(), sizeof(double) * (size_t)dm.R,
+                           hipMemcpyHostToDevice, s));
+    if (M.format == FSP_CSR) {
+        if (mixed) {
+            FSP_SWITCH_D(dm.d, (k_fill_merged<D_, float><<<blocks, kBlock, lds, s>>>(
+                                   M.d_model, tab, use_lds, table, M.d_coefs, n, 1, M.rp,
+                                   M.cols, (float *)M.vals, (double *)M.diag,
+                                   M.with_sink ? M.sink_block_ptr : nullptr, M.sink_cols,
+                                   (double *)M.sink_vals)));
+        } else {
+            FSP_SWITCH_D(dm.d, (k_fill_merged<D_, double><<<blocks, kBlock, lds, s>>>(
+                                   M.d_model, tab, use_lds, table, M.d_coefs, n, 0, M.rp,
+                                   M.cols, (double *)M.vals, nullptr,
+                                   M.with_sink ? M.sink_block_ptr : nullptr, M.sink_cols,
+                                   (double *)M.sink_vals)));
+        }
+    } else if (M.format == FSP_HYB) {
+        if (mixed) {
+            FSP_SWITCH_D(dm.d, (k_fill_hyb<D_, float><<<blocks, kBlock, lds, s>>>(
+                                   M.d_model, tab, use_lds, table, M.d_coefs, n, 1,
+                                   (double *)M.diag, (float *)M.slot_vals, M.slot_cols,
+                                   (double *)M.sink_out)));
+        } else {
+            FSP_SWITCH_D(dm.d, (k_fill_hyb<D_, double><<<blocks, kBlock, lds, s>>>(
+                                   M.d_model, tab, use_lds, table, M.d_coefs, n, 0,
+                                   (double *)M.diag, (double *)M.slot_vals, M.slot_cols,
+                                   (double *)M.sink_out)));
+        }
+    }
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
+}
+
+fsp_status mat_assemble(MatImpl &M, hipStream_t s)
+{
+    const DevModel &dm = M.hm.dm;
+    M.n = dm.n;
+    M.n_rows = dm.n + (M.with_sink ? 1 : 0);
+    const uint32_t n = (uint32_t)M.n;
+    const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    const int mixed = M.prec == FSP_MIXED;
+    FSP_TRY(upload_model(M));
+    const int64_t tiles = (M.n + kBlock - 1) / kBlock;
+    M.n_partials = std::max<int64_t>(tiles, 1) * 64;
+    FSP_HIP(hipMalloc(&M.partials, sizeof(double) * (size_t)M.n_partials));
+    FSP_HIP(hipMalloc(&M.ticket, sizeof(unsigned int) * 4));
+    FSP_HIP(hipMemset(M.ticket, 0, sizeof(unsigned int) * 4));
+    if (M.format == FSP_CSR) {
+        int *cnt = nullptr, *scnt = nullptr;
+        FSP_HIP(hipMalloc(&cnt, sizeof(int) * (size_t)(M.n + 1)));
+        FSP_HIP(hipMalloc(&M.rp, sizeof(int) * (size_t)(M.n + 1)));
+        if (M.with_sink) {
+            FSP_HIP(hipMalloc(&scnt, sizeof(int) * (size_t)(M.n + 1)));
+            FSP_HIP(hipMalloc(&M.sink_block_ptr, sizeof(int) * (size_t)(M.n + 1)));
+        }
+        FSP_SWITCH_D(dm.d, (k_count_merged<D_><<<blocks, kBlock, 0, s>>>(M.d_model, n, mixed ? 0 : 1,
+                                                                         cnt, scnt)));
+        FSP_HIP(hipGetLastError());
+        int64_t total = 0;
+        FSP_TRY(scan_counts(cnt, M.rp, M.n, &total, s));
+        M.nnz_rows = total;
+        if (M.with_sink) {
+            int64_t st = 0;
+            FSP_TRY(scan_counts(scnt, M.sink_block_ptr, M.n, &st, s));
+            M.nnz_sink = st;
+            FSP_HIP(hipMalloc(&M.sink_cols, sizeof(int) * (size_t)std::max<int64_t>(st, 1)));
+            FSP_HIP(hipMalloc(&M.sink_vals, sizeof(double) * (size_t)std::max<int64_t>(st, 1)));
+        }
+        // longest row (for the LDS product buffer); counts are still in cnt
+        {
+            std::vector<int> h((size_t)M.n);
+            FSP_HIP(hipMemcpy(h.data(), cnt, sizeof(int) * (size_t)M.n, hipMemcpyDeviceToHost));
+            M.max_row = M.n ? *std::max_element(h.begin(), h.end()) : 0;
+        }
+        hipFree(cnt);
+        if (scnt) hipFree(scnt);
+        const size_t vb = mixed ? sizeof(float) : sizeof(double);
+        FSP_HIP(hipMalloc(&M.cols, sizeof(int) * (size_t)std::max<int64_t>(M.nnz_rows, 1)));
+        FSP_HIP(hipMalloc(&M.vals, vb * (size_t)std::max<int64_t>(M.nnz_rows, 1)));
+        if (mixed) FSP_HIP(hipMalloc(&M.diag, sizeof(double) * (size_t)M.n));
+    } else if (M.format == FSP_HYB) {
+        const size_t vb = mixed ? sizeof(float) : sizeof(double);
+        const size_t slots = (size_t)std::max(dm.U, 1) * (size_t)M.n;
+        FSP_HIP(hipMalloc(&M.diag, sizeof(double) * (size_t)M.n));
+        FSP_HIP(hipMalloc(&M.slot_vals, vb * slots));
+        FSP_HIP(hipMalloc(&M.slot_cols, sizeof(int) * slots));
+        if (M.with_sink) FSP_HIP(hipMalloc(&M.sink_out, sizeof(double) * (size_t)M.n));
+        M.nnz_rows = (int64_t)M.n * (dm.U + 1);
+    } else {
+        if (!M.hm.has_factors && !M.hm.prop_table_dev)
+            return fail(FSP_ERR_INVALID, "KRON needs factor tables or a propensity table");
+        if (mixed && !M.hm.compact.empty()) {
+            std::vector<float> f(M.hm.compact.begin(), M.hm.compact.end());
+            FSP_HIP(hipMalloc(&M.d_compact_f, sizeof(float) * f.size()));
+            FSP_HIP(hipMemcpy(M.d_compact_f, f.data(), sizeof(float) * f.size(),
+                              hipMemcpyHostToDevice));
+        }
+        M.nnz_rows = 0;
+    }
+    FSP_TRY(fill_values(M, s));
+    FSP_HIP(hipStreamSynchronize(s));
+    return FSP_OK;
+}
+
+fsp_status mat_revalue(MatImpl &M, const double *coefs, hipStream_t s)
+{
+    bool same = true;
+    for (int r = 0; r < M.hm.dm.R; ++r) same = same && (M.coefs[r] == coefs[r]);
+    if (same) return FSP_OK;
+    M.coefs.assign(coefs, coefs + M.hm.dm.R);
+    if (M.format == FSP_KRON) {
+        FSP_HIP(hipMemcpyAsync(M.d_coefs, M.coefs.data(), sizeof(double) * (size_t)M.hm.dm.R,
+                               hipMemcpyHostToDevice, s));
+        // the host copy must stay valid until the copy ran
+        FSP_HIP(hipStreamSynchronize(s));
+        return FSP_OK;
+    }
+    FSP_TRY(fill_values(M, s));
+    FSP_HIP(hipStreamSynchronize(s));
+    return FSP_OK;
+}
+
+// ==================================================================== apply
+namespace {
+
+struct PlainX {
+    const double *__restrict__ x;
+    __device__ double operator()(uint32_t c) const { return x[c]; }
+};
+
+__global__ void k_finalize_sink(const double *__restrict__ partials, int count, double *y,
+                                int accumulate)
+{
+    __shared__ double red[kBlock / kWave];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < count; k += kBlock) s += partials[k];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) *y = accumulate ? *y + s : s;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_apply_csr(CsrF<V> A, PlainX X, double *__restrict__ y,
+                                                      int accumulate, uint32_t tiles,
+                                                      double *__restrict__ partials)
+{
+    extern __shared__ double smem[];
+    double *prod = smem;
+    double *red = smem + A.cap;
+    double sink = 0.0;
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const double v = csr_tile<V>(A, t, X, prod, sink);
+        const uint32_t i = t * kBlock + threadIdx.x;
+        if (i < A.n) y[i] = accumulate ? y[i] + v : v;
+    }
+    if (A.sink_off) {
+        const double s = block_sum(sink, red);
+        if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_apply_hyb(HybF<V> A, PlainX X, double *__restrict__ y,
+                                                      int accumulate, uint32_t tiles,
+                                                      double *__restrict__ partials)
+{
+    __shared__ double red[kBlock / kWave];
+    double sink = 0.0;
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        const double v = hyb_row<V>(A, i, X, sink);
+        if (i < A.n) y[i] = accumulate ? y[i] + v : v;
+    }
+    if (A.sink_out) {
+        const double s = block_sum(sink, red);
+        if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    }
+}
+
+template <int D, typename V>
+__global__ void __launch_bounds__(kBlock) k_apply_kron(KronF<V> A, PlainX X, double *__restrict__ y,
+                                                       int accumulate, uint32_t tiles,
+                                                       double *__restrict__ partials, int use_lds,
+                                                       int transpose)
+{
+    extern __shared__ double smem[];
+    __shared__ double red[kBlock / kWave];
+    const V *tab = A.tab_g;
+    if (use_lds && A.tab_g) {
+        stage_table((V *)smem, A.tab_g, A.tab_size);
+        tab = (const V *)smem;
+    }
+    double sink = 0.0;
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        const double v = transpose ? kron_row_t<D, V>(A, i, X, tab)
+                                   : kron_row<D, V>(A, i, X, tab, sink);
+        if (i < A.n) y[i] = accumulate ? y[i] + v : v;
+    }
+    if (A.with_sink && !transpose) {
+        const double s = block_sum(sink, red);
+        if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    }
+}
+
+}  // namespace
+
+static int apply_grid_cap() { return 4096; }
+
+template <typename V>
+static fsp_status launch_kron(const MatImpl &M, const double *x, double *y, int accumulate,
+                              hipStream_t s, int transpose, int single, double single_coef)
+{
+    KronF<V> A = kron_view<V>(M, single, single_coef);
+    const uint32_t tiles = (uint32_t)((M.n + kBlock - 1) / kBlock);
+    const int grid = grid_for(tiles, apply_grid_cap());
+    const size_t tb = sizeof(V) * (size_t)A.tab_size;
+    const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
+    const int with_sink = M.with_sink && !transpose && single < 0;
+    A.with_sink = with_sink;
+    FSP_SWITCH_D(M.hm.dm.d, (k_apply_kron<D_, V><<<grid, kBlock, use_lds ? tb : 0, s>>>(
+                                A, PlainX{x}, y, accumulate, tiles, M.partials, use_lds,
+                                transpose)));
+    FSP_HIP(hipGetLastError());
+    if (with_sink) {
+        k_finalize_sink<<<1, kBlock, 0, s>>>(M.partials, grid, y + M.n, accumulate);
+    } else if (M.with_sink && !accumulate) {
+        FSP_HIP(hipMemsetAsync(y + M.n, 0, sizeof(double), s));
+    }
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
+}
+
+fsp_status mat_apply(MatImpl &M, const double *d_x, double *d_y, int accumulate, hipStream_t s,
+                     const int *)
+{
+    const uint32_t tiles = (uint32_t)((M.n + kBlock - 1) / kBlock);
+    const int grid = grid_for(tiles, apply_grid_cap());
+    const bool mixed = M.prec == FSP_MIXED;
+    if (M.format == FSP_CSR) {
+        if (mixed) {
+            CsrF<float> A = csr_view<float>(M);
+            const size_t lds = sizeof(double) * ((size_t)A.cap + kBlock / kWave);
+            k_apply_csr<float><<<grid, kBlock, lds, s>>>(A, PlainX{d_x}, d_y, accumulate, tiles,
+                                                         M.partials);
+        } else {
+            CsrF<double> A = csr_view<double>(M);
+            const size_t lds = sizeof(double) * ((size_t)A.cap + kBlock / kWave);
+            k_apply_csr<double><<<grid, kBlock, lds, s>>>(A, PlainX{d_x}, d_y, accumulate, tiles,
+                                                          M.partials);
+        }
+    } else if (M.format == FSP_HYB) {
+        if (mixed)
+            k_apply_hyb<float><<<grid, kBlock, 0, s>>>(hyb_view<float>(M), PlainX{d_x}, d_y,
+                                                      accumulate, tiles, M.partials);
+        else
+            k_apply_hyb<double><<<grid, kBlock, 0, s>>>(hyb_view<double>(M), PlainX{d_x}, d_y,
+                                                       accumulate, tiles, M.partials);
+    } else {
+        return (mixed && M.hm.has_factors)
+                   ? launch_kron<float>(M, d_x, d_y, accumulate, s, 0, -1, 0.0)
+                     : launch_kron<double>(M, d_x, d_y, accumulate, s, 0, -1, 0.0);
+    }
+    FSP_HIP(hipGetLastError());
+    if (M.with_sink) k_finalize_sink<<<1, kBlock, 0, s>>>(M.partials, grid, d_y + M.n, accumulate);
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
+}
+
+fsp_status mat_apply_transpose(MatImpl &M, const double *d_x, double *d_y, int accumulate,
+                               hipStream_t s)
+{
+    if (M.with_sink) return fail(FSP_ERR_UNSUPPORTED, "transpose apply with a sink state");
+    // The transpose is matrix-free from the model for every format: the
+    // generator entries are the model's propensities, so no transposed copy
+    // of the CSR / HYB arrays is kept.
+    return launch_kron<double>(M, d_x, d_y, accumulate, s, 1, -1, 0.0);
+}
+
+fsp_status mat_apply_term(MatImpl &M, int r, double c, const double *d_x, double *d_y,
+                          int accumulate, hipStream_t s)
+{
+    if (r < 0 || r >= M.hm.dm.R) return fail(FSP_ERR_INVALID, "reaction index out of range");
+    if (M.with_sink) return fail(FSP_ERR_UNSUPPORTED, "term apply with a sink state");
+    return launch_kron<double>(M, d_x, d_y, accumulate, s, 0, r, c);
+}
+
+void mat_free(MatImpl &M)
+{
+    void *ptrs[] = {M.d_model, M.d_compact, M.d_compact_f, M.d_coefs, M.rp, M.cols, M.vals,
+                    M.sink_cols, M.sink_vals, M.sink_block_ptr, M.diag, M.slot_vals,
+                    M.slot_cols, M.sink_out, M.partials, M.ticket};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+}
+
+double mat_bytes_per_apply(const MatImpl &M)
+{
+    // Algorithmic bytes of one y = A x (x read once, y written once).
+    const double n = (double)M.n;
+    const double vb = M.prec == FSP_MIXED ? 4.0 : 8.0;
+    double b = 0.0;
+    if (M.format == FSP_CSR) {
+        b = 4.0 * (n + 1) + (vb + 4.0) * (double)M.nnz_rows + 16.0 * n;
+        if (M.prec == FSP_MIXED) b += 8.0 * n;  // separate FP64 diagonal
+        if (M.with_sink) b += 12.0 * (double)M.nnz_sink;
+    } else if (M.format == FSP_HYB) {
+        b = 8.0 * n + (vb + 4.0) * (double)M.hm.dm.U * n + 16.0 * n;
+        if (M.with_sink) b += 8.0 * n;
+    } else {
+        b = 16.0 * n;
+    }
+    return b;
+}
+
+// ============================================================= term CSR
+fsp_status term_csr(const HostModel &hm, int r, int *d_rp, int *d_cols, double *d_vals,
+                    int64_t *h_nnz, hipStream_t s)
+{
+    const DevModel &dm = hm.dm;
+    if (r < 0 || r >= dm.R) return fail(FSP_ERR_INVALID, "reaction index out of range");
+    const uint32_t n = (uint32_t)dm.n;
+    const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    DevModel *d_model = nullptr;
+    double *d_tab = nullptr;
+    int *cnt = nullptr, *rp = d_rp;
+    FSP_HIP(hipMalloc(&d_model, sizeof(DevModel)));
+    FSP_HIP(hipMemcpy(d_model, &dm, sizeof(DevModel), hipMemcpyHostToDevice));
+    if (!hm.compact.empty()) {
+        FSP_HIP(hipMalloc(&d_tab, sizeof(double) * hm.compact.size()));
+        FSP_HIP(hipMemcpy(d_tab, hm.compact.data(), sizeof(double) * hm.compact.size(),
+                          hipMemcpyHostToDevice));
+    }
+    FSP_HIP(hipMalloc(&cnt, sizeof(int) * (size_t)(n + 1)));
+    if (!rp) FSP_HIP(hipMalloc(&rp, sizeof(int) * (size_t)(n + 1)));
+    FSP_SWITCH_D(dm.d, (k_term_count<D_><<<blocks, kBlock, 0, s>>>(d_model, r, n, cnt)));
+    FSP_HIP(hipGetLastError());
+    int64_t total = 0;
+    fsp_status st = scan_counts(cnt, rp, n, &total, s);
+    if (st == FSP_OK && d_cols) {
+        int use_lds = 0;
+        const size_t lds = table_lds_bytes(hm, use_lds);
+        FSP_SWITCH_D(dm.d, (k_term_fill<D_><<<blocks, kBlock, lds, s>>>(
+                               d_model, d_tab, use_lds, hm.prop_table_dev, r, n, rp, d_cols,
+                               d_vals)));
+        FSP_HIP(hipGetLastError());
+        FSP_HIP(hipStreamSynchronize(s));
+    }
+    *h_nnz = total;
+    hipFree(cnt);
+    if (rp != d_rp) hipFree(rp);
+    hipFree(d_model);
+    if (d_tab) hipFree(d_tab);
+    return st;
+}
+
+}  // namespace fsp

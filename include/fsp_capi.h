@@ -1,0 +1,207 @@
+/*
+ * fsp_capi.h -- C ABI of the MI355X FSP engine (HIP kernels for gfx950).
+ *
+ * Plain pointers and sizes only.  "d_" pointers are device (HBM) pointers,
+ * "h_" pointers host pointers.  `stream` is a hipStream_t passed as void*
+ * (NULL = the null stream).  Every call returns an fsp_status; on failure
+ * fsp_last_error() returns a message (thread-local).  No call exits the
+ * process: the reference's exit(EXIT_FAILURE) on CUDA errors
+ * (src/cme_util.h:9-22) becomes a status.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the voduchuy/CUDA_FSP source tree).
+ */
+#ifndef FSP_CAPI_H
+#define FSP_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum fsp_status {
+    FSP_OK = 0,
+    FSP_ERR_INVALID = 1,       /* bad argument                                 */
+    FSP_ERR_UNSUPPORTED = 2,   /* format / precision / size not supported      */
+    FSP_ERR_OVERFLOW = 3,      /* state or nonzero count beyond int32          */
+    FSP_ERR_HIP = 4,           /* HIP runtime error                            */
+    FSP_ERR_NOMEM = 5,         /* device allocation failed                     */
+    FSP_ERR_NOT_CONVERGED = 6, /* reject / step cap exhausted                  */
+    FSP_ERR_NO_DEVICE = 7      /* no GPU visible                               */
+} fsp_status;
+
+/* src/FSPMat.h:17-19 (enum MatrixFormat {CUDA_CSR, HYB, KRONECKER}) */
+typedef enum fsp_format { FSP_CSR = 0, FSP_HYB = 1, FSP_KRON = 2 } fsp_format;
+
+typedef enum fsp_precision {
+    FSP_F64 = 0,   /* values, vectors and sums in FP64                           */
+    FSP_MIXED = 1  /* FP32 values and Krylov basis, FP64 row sums / reductions  */
+} fsp_precision;
+
+/* Model description: the arguments of the FSPMat constructors
+ * (src/FSPMat.h:43-52, src/FSPMat.cu:10-76).
+ *   bounds[d]            : fsp_dim, box 0..bounds[s]
+ *   stoich_*             : CSRMatInt stoich, reactions x species (host)
+ *   factors              : host, R * sum_s(bounds[s]+1) separable propensity
+ *                          tables in the layout of
+ *                          fspmat_sdkron_fill_host_data
+ *                          (src/fspmat_kron_kernels.cu:170-190), or NULL
+ *   prop_table_dev       : device, R * n values a_r(x_i) (what a PropFun
+ *                          evaluates, src/cme_util.h:25), or NULL
+ * Exactly one of factors / prop_table_dev must be given.  The KRON format
+ * needs factors. */
+typedef struct fsp_model {
+    int n_species;
+    int n_reactions;
+    const int *bounds;
+    const int *stoich_vals;
+    const int *stoich_cols;
+    const int *stoich_rowptrs;
+    const double *factors;
+    const double *prop_table_dev;
+} fsp_model;
+
+const char *fsp_last_error(void);
+int fsp_version(void);
+
+/* ---- device memory helpers (so C/C++ callers need no HIP headers) ---- */
+fsp_status fsp_device_count(int *count);
+fsp_status fsp_malloc(void **d_ptr, size_t bytes);
+fsp_status fsp_free(void *d_ptr);
+fsp_status fsp_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
+fsp_status fsp_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
+fsp_status fsp_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream);
+fsp_status fsp_memset_zero(void *d_ptr, size_t bytes, void *stream);
+fsp_status fsp_stream_sync(void *stream);
+/* Sum / L1 norm of a device FP64 vector (deterministic two-stage reduction).
+ * Replaces thrust::reduce in the reference tests (tests/test_fsp_mat.cu:154). */
+fsp_status fsp_vec_sum(const double *d_x, int64_t n, double *h_sum, void *stream);
+fsp_status fsp_vec_axpy(double a, const double *d_x, double *d_y, int64_t n, void *stream);
+fsp_status fsp_vec_fill(double *d_x, double value, int64_t n, void *stream);
+
+/* ---- state space: src/cme_util.cu ---- */
+/* rect_fsp_num_states (:55-62); FSP_ERR_OVERFLOW past 2^31-1 states. */
+fsp_status fsp_num_states(int n_species, const int *bounds, int64_t *n_states);
+/* fsp_get_states kernel (:24-41): d_states is AoS int[n][d]. */
+fsp_status fsp_states_enumerate(int n_species, const int *bounds, int *d_states,
+                                void *stream);
+/* state2indx (:13-22) over a batch of AoS states; states outside the box map
+ * to -1. */
+fsp_status fsp_states_to_index(int n_species, const int *bounds, const int *d_states,
+                               int64_t count, int *d_index, void *stream);
+
+/* ---- generator: src/FSPMat.cu, src/fspmat_{csr,hyb,kron}_kernels.cu ---- */
+typedef struct fsp_mat_s *fsp_mat;
+
+typedef struct fsp_mat_info {
+    int64_t n_states;   /* box states n                                      */
+    int64_t n_rows;     /* n, or n+1 with the sink row                        */
+    int64_t nnz;        /* stored entries (CSR incl. sink row; HYB n*(U+1))   */
+    int n_slots;        /* HYB: unique net-change slots U                     */
+    int format;
+    int precision;
+    int with_sink;
+    int max_row;        /* longest CSR row (sink row excluded)               */
+    double bytes_per_apply; /* algorithmic HBM bytes of one apply            */
+} fsp_mat_info;
+
+/* FSPMat constructors (src/FSPMat.cu:10-76) with generate_fsp_mats_cuda_csr
+ * (src/fspmat_csr_kernels.cu:138-248), generate_fsp_mats_hyb
+ * (src/fspmat_hyb_kernels.cu:108-184) and generate_fsp_mats_sdkron
+ * (src/fspmat_kron_kernels.cu:131-168).  All assembly runs on the GPU.
+ * `coefs` (host, R) are the coefficients the CSR/HYB values are built with;
+ * apply() re-values them when it is given different coefficients.
+ * with_sink appends state n that collects the outflow leaving the box. */
+fsp_status fsp_mat_create(const fsp_model *model, fsp_format format,
+                          fsp_precision precision, int with_sink,
+                          const double *coefs, void *stream, fsp_mat *out);
+/* FSPMat::~FSPMat / destroy (src/FSPMat.cu:79-95) */
+fsp_status fsp_mat_destroy(fsp_mat A);
+fsp_status fsp_mat_get_info(fsp_mat A, fsp_mat_info *info);
+/* Rebuild CSR/HYB values for new coefficients (no-op for KRON). */
+fsp_status fsp_mat_set_coefs(fsp_mat A, const double *h_coefs, void *stream);
+/* FSPMat::action (src/FSPMat.cu:102-105): y (+)= A(coefs) x.
+ * h_coefs NULL = the coefficients last set.  accumulate != 0 gives the
+ * reference's y += A x; 0 overwrites y.  Vectors have n_rows entries.
+ * FP64 vectors for both precisions (a mixed matrix stores FP32 values and
+ * sums rows in FP64). */
+fsp_status fsp_mat_apply(fsp_mat A, const double *h_coefs, const double *d_x,
+                         double *d_y, int accumulate, void *stream);
+/* y (+)= A^T x (adjoint operator; no reference counterpart). */
+fsp_status fsp_mat_apply_transpose(fsp_mat A, const double *h_coefs,
+                                   const double *d_x, double *d_y, int accumulate,
+                                   void *stream);
+/* Single-term apply y (+)= c * A_r x (per-reaction generator; the building
+ * block of rate gradients; no reference counterpart). */
+fsp_status fsp_mat_apply_term(fsp_mat A, int reaction, double c, const double *d_x,
+                              double *d_y, int accumulate, void *stream);
+/* Copy out the merged CSR (FSP_CSR only): arrays sized from info. */
+fsp_status fsp_mat_export_csr(fsp_mat A, int *d_row_ptrs, int *d_cols, double *d_vals,
+                              void *stream);
+/* Copy out the HYB slots (FSP_HYB only): diag[n], slot_vals[U*n],
+ * slot_cols[U*n], sink_out[n] (may be NULL). */
+fsp_status fsp_mat_export_hyb(fsp_mat A, double *d_diag, double *d_slot_vals,
+                              int *d_slot_cols, double *d_sink_out, void *stream);
+
+/* Per-reaction CSR in the reference layout (fspmat_csr_get_nnz_per_row,
+ * the row-pointer scan and fspmat_csr_fill_data,
+ * src/fspmat_csr_kernels.cu:7-94,206-213), assembled on the GPU.
+ * Call with d_cols == NULL to get *h_nnz (row pointers are written if
+ * d_row_ptrs != NULL), then again with arrays of h_nnz entries. */
+fsp_status fsp_term_csr(const fsp_model *model, int reaction, int *d_row_ptrs,
+                        int *d_cols, double *d_vals, int64_t *h_nnz, void *stream);
+
+/* ---- Krylov expv: src/KryExpvFSP.{h,cu} ---- */
+typedef struct fsp_expv_s *fsp_expv;
+
+typedef struct fsp_expv_stats {
+    double t_now, t_new, tau_last, err_last, beta_last;
+    int64_t i_step, n_matvec, n_reject, n_breakdown;
+} fsp_expv_stats;
+
+/* Host callback operator: y += A x on device vectors (the reference's MVFun,
+ * src/KryExpvFSP.h:14).  Called from the host thread driving the solve. */
+typedef void (*fsp_matvec_cb)(void *ctx, const double *d_x, double *d_y);
+
+/* KryExpvFSP(t_final, matvec, v, m, tol, iop, q_iop, anorm)
+ * (src/KryExpvFSP.cu:6-36).  Operator: an fsp_mat (A != NULL; fused device
+ * path, hipGraph-captured steps) or a callback (A == NULL).  n = vector
+ * length.  m <= 50.  For a FSP_MIXED matrix the basis is FP32. */
+fsp_status fsp_expv_create(fsp_mat A, fsp_matvec_cb cb, void *cb_ctx, int64_t n,
+                           double t_final, int m, double tol, int iop, int q_iop,
+                           double anorm, void *stream, fsp_expv *out);
+fsp_status fsp_expv_destroy(fsp_expv e);
+/* KryExpvFSP::step (src/KryExpvFSP.cu:38-182): one adaptive step, v in place */
+fsp_status fsp_expv_step(fsp_expv e, double *d_v);
+/* KryExpvFSP::solve (src/KryExpvFSP.cu:184-188) */
+fsp_status fsp_expv_solve(fsp_expv e, double *d_v);
+/* KryExpvFSP::reset_time (src/KryExpvFSP.h:82-84) */
+fsp_status fsp_expv_reset_time(fsp_expv e);
+/* New final time (continues from t_now; clears the breakdown latch).
+ * No reference counterpart (the reference's FSPSolver calls a missing
+ * update_final_time, src/FSPSolver.cpp:12). */
+fsp_status fsp_expv_set_final_time(fsp_expv e, double t_final);
+/* Fresh solve with the same buffers: t_now = 0, step counter and statistics
+ * cleared (the first step size is recomputed from |v|).  No reference
+ * counterpart (a new KryExpvFSP object there). */
+fsp_status fsp_expv_restart(fsp_expv e, double t_final);
+/* KryExpvFSP::final_time_reached (src/KryExpvFSP.h:100-102) */
+fsp_status fsp_expv_final_time_reached(fsp_expv e, int *reached);
+fsp_status fsp_expv_get_stats(fsp_expv e, fsp_expv_stats *stats);
+/* Accepted step sizes so far (the first min(count, cap) are copied).  No
+ * reference counterpart: lets a CPU run replay the device's step sequence. */
+fsp_status fsp_expv_get_tau_history(fsp_expv e, double *h_taus, int64_t cap,
+                                    int64_t *count);
+/* Public knobs max_reject / max_nstep (src/KryExpvFSP.h:68-69). */
+fsp_status fsp_expv_set_limits(fsp_expv e, int max_reject, int64_t max_nstep);
+/* Dense matrix exponential on the device (one workgroup, FP64, Pade q=6
+ * with scaling and squaring; stands in for arma::expmat at
+ * src/KryExpvFSP.cu:125).  Column-major N x N, N <= 52. */
+fsp_status fsp_expm_dense(int N, const double *d_A, double *d_F, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
