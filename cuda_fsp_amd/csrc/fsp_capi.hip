@@ -280,9 +280,6 @@ fsp_status fsp_mat_create(const fsp_model *model, fsp_format format, fsp_precisi
     if (precision != FSP_F64 && precision != FSP_MIXED)
         return fail(FSP_ERR_UNSUPPORTED, "requested precision is not supported");
     if (!coefs) return fail(FSP_ERR_INVALID, "coefs is NULL");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-        return fail(FSP_ERR_NO_DEVICE, "no HIP device visible");
     auto *h = new fsp_mat_s();
     MatImpl &M = h->impl;
     fsp_status st = build_host_model(model, M.hm);
@@ -290,9 +287,10 @@ fsp_status fsp_mat_create(const fsp_model *model, fsp_format format, fsp_precisi
         delete h;
         return st;
     }
-    if (format == FSP_KRON && !M.hm.has_factors && !M.hm.prop_table_dev) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         delete h;
-        return fail(FSP_ERR_INVALID, "KRON needs factor tables");
+        return fail(FSP_ERR_NO_DEVICE, "no HIP device visible");
     }
     M.format = format;
     M.prec = precision;

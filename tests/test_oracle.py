@@ -1,0 +1,257 @@
+"""Pins of the CPU oracle (runs without a GPU).
+
+Each oracle function is checked against something other than itself: the
+reference tests' closed forms and invariants, exhaustive enumeration, closed-
+form structure counts, a dense generator built from the CME definition
+(tests/golden/make_golden.py), scipy.linalg.expm fixtures and the Poisson
+solution of birth-death processes.  A few "planted mistake" tests show the
+pins catch plausible errors (a flipped net change, a transposed operator).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.stats as ss
+
+from cuda_fsp_amd import models as M
+from oracle import oracle as O
+from tests.golden.make_golden import dense_generator
+
+FIX = np.load(os.path.join(os.path.dirname(__file__), "golden", "fixtures.npz"))
+
+
+def _dense_from_csr(rp, c, v, n_cols):
+    A = np.zeros((len(rp) - 1, n_cols))
+    for i in range(len(rp) - 1):
+        for k in range(rp[i], rp[i + 1]):
+            A[i, c[k]] += v[k]
+    return A
+
+
+# ------------------------------------------------------------ state space
+def test_enumeration_exhaustive_reference_box():
+    # tests/test_fsp_get_states.cu:21-54 (species 0 fastest)
+    b = (1023, 1023, 3)
+    st = O.enumerate_states(b)
+    x2, x1, x0 = np.meshgrid(np.arange(4), np.arange(1024), np.arange(1024), indexing="ij")
+    assert np.array_equal(st, np.stack([x0.ravel(), x1.ravel(), x2.ravel()], 1))
+
+
+def test_index_round_trip():
+    b = np.array([3, 31, 31, 63, 63], np.int32)
+    assert O.num_states(b) == 16_777_216
+    rng = np.random.default_rng(0)
+    for i in rng.integers(0, 16_777_216, 500):
+        x = np.zeros(5, np.int32)
+        O.lib().orc_indx2state(int(i), x.ctypes.data, 5, b.ctypes.data)
+        assert np.all(x <= b)
+        assert O.lib().orc_state2indx(x.ctypes.data, 5, b.ctypes.data) == i
+
+
+# -------------------------------------------------------------- structure
+def _closed_form_counts(net):
+    """nnz of reaction r's term = n + #{x : x - nu_r in box}; the merged
+    matrix has one entry per state and per unique nonzero net change."""
+    b = net.bounds.astype(np.int64) + 1
+    n = int(np.prod(b))
+    per = [n + int(np.prod(np.maximum(b - np.abs(nu), 0))) for nu in net.nu]
+    uniq = {tuple(nu) for nu in net.nu if np.any(nu)}
+    merged = n + sum(int(np.prod(np.maximum(b - np.abs(np.array(u)), 0))) for u in uniq)
+    return per, merged, len(uniq)
+
+
+@pytest.mark.parametrize("mk,want_ref,want_merged", [
+    (lambda: M.toggle_switch(), 130_560, 81_408),
+    (lambda: M.gene_expression(), 13_097_987, 4_973_059),
+    (lambda: M.toggle_ref_test((256, 256)), 791_046, 329_217),
+])
+def test_structure_counts(mk, want_ref, want_merged):
+    net = mk()
+    per, merged, U = _closed_form_counts(net)
+    assert sum(per) == want_ref and merged == want_merged
+    got = [len(O.term_csr(net, r)[1]) for r in range(net.n_reactions)]
+    assert got == per
+    rp, c, v = O.merged_csr(net)
+    assert len(c) == merged
+    assert O.unique_shifts(net)[0] == U
+
+
+def test_term_csr_columns_ascending_and_layout():
+    # src/fspmat_csr_kernels.cu:64-92: diagonal -a_r(x), off-diagonal
+    # a_r(x - nu) at column idx(x - nu), columns ascending per row
+    net = M.toggle_ref_test((31, 31))
+    A_all = dense_generator(net, sink=False)
+    tot = np.zeros_like(A_all)
+    for r in range(6):
+        rp, c, v = O.term_csr(net, r)
+        for i in range(len(rp) - 1):
+            assert np.all(np.diff(c[rp[i]:rp[i + 1]]) > 0)
+        tot += net.coefs[r] * _dense_from_csr(rp, c, v, net.n_states)
+    assert np.abs(tot - A_all).max() <= 1e-15 * np.abs(A_all).max()
+
+
+@pytest.mark.parametrize("name", ["toggle", "gene", "bd", "chain"])
+def test_merged_csr_and_formats_match_definition(name):
+    A_def = FIX[f"cme_{name}_A"]
+    net = {"toggle": M.toggle_switch((9, 8)), "gene": M.gene_expression((2, 6, 5)),
+           "bd": M.birth_death((12, 7), k=2.0, g=0.7),
+           "chain": M.five_species_chain((2, 3, 2, 3, 2))}[name]
+    n = net.n_states
+    rp, c, v = O.merged_csr(net, with_sink=True)
+    A = _dense_from_csr(rp, c, v, n + 1)
+    scale = np.abs(A_def).max()
+    assert np.abs(A - A_def).max() <= 1e-15 * scale
+    # merged rows: columns strictly ascending
+    for i in range(n + 1):
+        assert np.all(np.diff(c[rp[i]:rp[i + 1]]) > 0)
+    x = np.random.default_rng(1).random(n + 1)
+    want = A_def @ x
+    for apply in ("csr", "ell", "kron", "hyb_ref"):
+        y = np.zeros(n + 1)
+        if apply == "csr":
+            O.csr_apply(rp, c, v, x, y)
+        elif apply == "ell":
+            U, dg, sv, sc, sk = O.ell(net, with_sink=True)
+            O.ell_apply(U, dg, sv, sc, sk, x, y)
+        elif apply == "kron":
+            O.kron_apply(net, x, y, with_sink=True)
+        else:
+            d, off, col = O.hyb_ref(net)
+            O.hyb_ref_apply(d, off, col, net.coefs, x[:n], y[:n])
+            want_box = A_def[:n, :n] @ x[:n]
+            assert np.abs(y[:n] - want_box).max() <= 1e-13 * np.abs(want_box).max()
+            continue
+        assert np.abs(y - want).max() <= 1e-13 * np.abs(want).max(), apply
+
+
+def test_generator_invariants():
+    # A.0 = 0 (tests/test_fsp_mat.cu:153-169); with the sink every column
+    # sums to zero, so 1^T A x = 0
+    net = M.gene_expression((2, 20, 20))
+    rp, c, v = O.merged_csr(net, with_sink=True)
+    y = np.zeros(len(rp) - 1)
+    O.csr_apply(rp, c, v, np.zeros(len(rp) - 1), y)
+    assert np.all(y == 0.0)
+    x = np.random.default_rng(2).random(len(rp) - 1)
+    y = np.zeros_like(x)
+    O.csr_apply(rp, c, v, x, y)
+    assert abs(y.sum()) <= 1e-13 * np.abs(y).sum()
+
+
+def test_kronecker_shift_closed_form():
+    # tests/test_kronecker_kernel.cu:325-337 with x == 1 and x0 -> 0 at rate i0
+    net = M.birth_death((1024, 40)).with_coefs([0.0, 1.0, 0.0, 0.0])
+    y = np.zeros(net.n_states)
+    O.kron_apply(net, np.ones(net.n_states), y)
+    i0 = np.arange(1025.0)
+    want = np.where(i0 < 1024, i0 + 1, 0.0) - i0
+    assert np.array_equal(y.reshape(41, 1025), np.broadcast_to(want, (41, 1025)))
+
+
+def test_planted_mistake_flipped_net_change_is_caught():
+    # an oracle that used x + nu as the inflow source (a sign slip) must fail
+    # the definition check above
+    net = M.toggle_switch((9, 8))
+    bad = M.ReactionNetwork(net.name, net.bounds, -net.nu, net.factors, net.coefs)
+    rp, c, v = O.merged_csr(bad, with_sink=True)
+    A = _dense_from_csr(rp, c, v, net.n_states + 1)
+    assert np.abs(A - FIX["cme_toggle_A"]).max() > 1e-3
+
+
+def test_planted_mistake_transposed_operator_is_caught():
+    A = FIX["cme_gene_A"]
+    p0 = np.zeros(A.shape[0])
+    p0[0] = 1.0
+    import scipy.linalg as sl
+    bad = sl.expm(4.0 * A.T) @ p0
+    assert np.abs(bad - FIX["cme_gene_p"][1]).sum() > 1e-2
+
+
+# -------------------------------------------------------------------- expm
+@pytest.mark.parametrize("k", range(5))
+def test_expm_matches_scipy(k):
+    A, F = FIX[f"expm_A{k}"], FIX[f"expm_F{k}"]
+    G = O.expm(A)
+    assert np.abs(G - F).max() <= 1e-12 * max(1.0, np.abs(F).max())
+
+
+def test_expm_special_cases():
+    assert np.array_equal(O.expm(np.zeros((7, 7))), np.eye(7))
+    d = np.array([-3.0, -0.5, 0.0, 2.0])
+    assert np.allclose(O.expm(np.diag(d)), np.diag(np.exp(d)), rtol=1e-13, atol=0)
+    # nilpotent Jordan block: exp(N) = sum N^k / k!
+    N = np.diag(np.ones(4), 1)
+    want = np.eye(5) + N + N @ N / 2 + N @ N @ N / 6 + N @ N @ N @ N / 24
+    assert np.abs(O.expm(N) - want).max() <= 1e-15
+
+
+# ------------------------------------------------------------------- expv
+@pytest.mark.parametrize("name", ["toggle", "gene", "bd", "chain"])
+def test_expv_matches_dense_expm(name):
+    A = FIX[f"cme_{name}_A"]
+    net = {"toggle": M.toggle_switch((9, 8)), "gene": M.gene_expression((2, 6, 5)),
+           "bd": M.birth_death((12, 7), k=2.0, g=0.7),
+           "chain": M.five_species_chain((2, 3, 2, 3, 2))}[name]
+    rp, c, v = O.merged_csr(net, with_sink=True)
+    p0 = np.zeros(net.n_states + 1)
+    p0[0] = 1.0
+    t = FIX[f"cme_{name}_t"]
+    for tol in (1e-8, 1e-12):
+        rc, out, st = O.expv_csr_outputs(rp, c, v, p0.copy(), t, tol=tol)
+        assert rc == 0
+        err = np.abs(out - FIX[f"cme_{name}_p"]).sum(axis=1).max()
+        assert err <= (1e-6 if tol == 1e-8 else 1e-10), (tol, err)
+        assert np.abs(out.sum(axis=1) - 1.0).max() <= 1e-12
+
+
+def test_expv_poisson_closed_form():
+    # independent birth-death from 0: Poisson(lambda(t)) per species,
+    # lambda = (k/g)(1 - e^{-g t}); box large enough that truncation < 1e-14
+    k, g = 3.0, 0.8
+    net = M.birth_death((60, 45), k=k, g=g)
+    rp, c, v = O.merged_csr(net, with_sink=True)
+    p0 = np.zeros(net.n_states + 1)
+    p0[0] = 1.0
+    ts = [0.5, 2.0, 7.0]
+    rc, out, st = O.expv_csr_outputs(rp, c, v, p0, ts, tol=1e-12)
+    for kk, t in enumerate(ts):
+        lam = (k / g) * (1 - math.exp(-g * t))
+        p = np.outer(ss.poisson.pmf(np.arange(46), lam), ss.poisson.pmf(np.arange(61), lam)).ravel()
+        assert np.abs(out[kk, :-1] - p).sum() <= 1e-9
+        assert out[kk, -1] <= 1e-12
+
+
+def test_expv_reference_kry_mass():
+    # tests/test_kry_expv.cu:80-110 at full size: (256, 256) toggle, HYB-
+    # equivalent operator, IOP q = 2, t = 8 h, m = 30, tol 1e-8
+    O.set_threads(min(os.cpu_count() or 1, 8))
+    net = M.toggle_ref_test((256, 256))
+    rp, c, v = O.merged_csr(net)
+    p0 = np.zeros(net.n_states)
+    p0[0] = 1.0
+    rc, out, st = O.expv_csr_outputs(rp, c, v, p0, [8 * 3600.0], iop=True, q_iop=2)
+    assert rc == 0 and abs(1.0 - out[0].sum()) <= 1e-10
+
+
+def test_round_up_2sig():
+    # src/KryExpvFSP.cu:148-150: rounds up to two significant digits
+    assert O.round_up_2sig(7.0) == 7.0
+    assert O.round_up_2sig(123.0) == 130.0
+    assert O.round_up_2sig(0.1) == 0.1
+    rng = np.random.default_rng(4)
+    for t in 10.0 ** rng.uniform(-9, 6, 2000):
+        r = O.round_up_2sig(t)
+        e = math.floor(math.log10(t))
+        q = r / 10.0 ** (e - 1)
+        assert r >= t * (1 - 1e-15)
+        assert abs(q - round(q)) <= 1e-9 and 10 <= round(q) <= 100
+        assert r - t <= 10.0 ** (e - 1) * (1 + 1e-9)
+
+
+def test_loglik_indices():
+    # src/Likelihood.cpp:14-20: sum of log(max(floor, p[idx]))
+    p = np.array([0.5, 0.25, 0.0, 0.25])
+    idx = np.array([0, 1, 1, 2, 3])
+    want = math.log(0.5) + 2 * math.log(0.25) + math.log(1e-44) + math.log(0.25)
+    assert abs(O.loglik_indices(p, idx) - want) <= 1e-12
