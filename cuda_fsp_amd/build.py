@@ -36,7 +36,15 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 CXX = os.environ.get("CXX", "g++")
-CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-ffp-contract=off"]
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-ffp-contract=off", "-fopenmp"]
+CPP_INCLUDE = os.path.join(CPP, "include")
+
+
+def compile_cpp_program(src: str, out: str) -> str:
+    """Builds a C++ program against the facade (used by the C++ tests)."""
+    _run([CXX, *CXX_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CPP_INCLUDE, src,
+          "-o", out, "-L", LIBDIR, "-lcufsp", "-lfsp_hip", f"-Wl,-rpath,{LIBDIR}"])
+    return out
 
 
 def _run(cmd):

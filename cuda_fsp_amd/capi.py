@@ -94,6 +94,10 @@ _SIGS = {
     "fsp_expv_get_stats": (C.c_int, [_vp, _vp]),
     "fsp_expv_set_limits": (C.c_int, [_vp, _i, _i64]),
     "fsp_expv_get_tau_history": (C.c_int, [_vp, _vp, _i64, _vp]),
+    "fsp_expv_set_iop": (C.c_int, [_vp, _i, _i]),
+    "fsp_loglik_indices": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
+    "fsp_marginal": (C.c_int, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "fsp_loglik_weighted": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
     "fsp_expm_dense": (C.c_int, [_i, _vp, _vp, _vp]),
 }
 
@@ -317,6 +321,26 @@ def num_states(bounds) -> int:
 def vec_sum(x, n, stream=None) -> float:
     s = C.c_double(0.0)
     check(lib().fsp_vec_sum(ptr(x), int(n), C.byref(s), stream))
+    return s.value
+
+
+def loglik_indices(p, idx, count, floor_val=1e-44, stream=None) -> float:
+    s = C.c_double(0.0)
+    check(lib().fsp_loglik_indices(ptr(p), ptr(idx), int(count), float(floor_val), C.byref(s),
+                                   stream))
+    return s.value
+
+
+def marginal(p, bounds, keep, out, stream=None):
+    b = np.ascontiguousarray(bounds, np.int32)
+    k = np.ascontiguousarray([1 if v else 0 for v in keep], np.int32)
+    check(lib().fsp_marginal(ptr(p), len(b), b.ctypes.data, k.ctypes.data, ptr(out), stream))
+
+
+def loglik_weighted(q, w, count, floor_val=1e-12, stream=None) -> float:
+    s = C.c_double(0.0)
+    check(lib().fsp_loglik_weighted(ptr(q), ptr(w), int(count), float(floor_val), C.byref(s),
+                                    stream))
     return s.value
 
 

@@ -48,23 +48,30 @@ __global__ void k_to_index(const DevModel *__restrict__ dm, const int *__restric
     idx[k] = ok ? (int)v : -1;
 }
 
-__global__ void k_sum_partials(const double *__restrict__ x, int64_t n, double *partials)
+__device__ inline void put_partial(double *partials, dd v, double *red)
 {
-    __shared__ double red[kBlock / kWave];
-    double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kBlock)
-        s += x[i];
-    s = block_sum(s, red);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    v = block_sum_dd(v, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = v.s;
+        partials[2 * blockIdx.x + 1] = v.c;
+    }
 }
 
+__global__ void k_sum_partials(const double *__restrict__ x, int64_t n, double *partials)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    dd s = dd_zero();
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        s = dd_add(s, x[i]);
+    put_partial(partials, s, red);
+}
+
+// out = fl(sum of G dd partials)
 __global__ void k_sum_final(const double *__restrict__ partials, int G, double *out)
 {
-    __shared__ double red[kBlock / kWave];
-    double s = 0.0;
-    for (int b = threadIdx.x; b < G; b += kBlock) s += partials[b];
-    s = block_sum(s, red);
+    __shared__ double red[2 * kBlock / kWave];
+    const double s = dd_value(sum_dd_partials(partials, G, red));
     if (threadIdx.x == 0) *out = s;
 }
 
@@ -80,6 +87,79 @@ __global__ void k_fill(double *__restrict__ x, double v, int64_t n)
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kBlock)
         x[i] = v;
+}
+
+// log-likelihood partials: sum_k log(max(floor, p[idx[k]])) (idx < 0 -> 0)
+__global__ void k_loglik_idx(const double *__restrict__ p, const int *__restrict__ idx,
+                             int64_t count, double floor_val, double *partials)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    dd s = dd_zero();
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < count;
+         k += (int64_t)gridDim.x * kBlock) {
+        const int i = idx[k];
+        const double v = i >= 0 ? p[i] : 0.0;
+        s = dd_add(s, log(v > floor_val ? v : floor_val));
+    }
+    put_partial(partials, s, red);
+}
+
+__global__ void k_loglik_w(const double *__restrict__ q, const double *__restrict__ w,
+                           int64_t count, double floor_val, double *partials)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    dd s = dd_zero();
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < count;
+         k += (int64_t)gridDim.x * kBlock) {
+        const double v = q[k];
+        if (w[k] != 0.0) s = dd_add_prod(s, w[k], log(v > floor_val ? v : floor_val));
+    }
+    put_partial(partials, s, red);
+}
+
+struct MargGeom {
+    int nk, nd;
+    int krad[kMaxSpecies], drad[kMaxSpecies];
+    int64_t kstride[kMaxSpecies], dstride[kMaxSpecies];
+    int64_t n_out, n_drop;
+};
+
+// One output cell per thread; the dropped coordinates are summed in
+// ascending index order (deterministic, no atomics).
+__global__ void k_marginal(const double *__restrict__ p, MargGeom g, double *__restrict__ out)
+{
+    const int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (o >= g.n_out) return;
+    int64_t base = 0, q = o;
+    for (int k = 0; k < g.nk; ++k) {
+        base += (q % g.krad[k]) * g.kstride[k];
+        q /= g.krad[k];
+    }
+    int y[kMaxSpecies] = {0};
+    int64_t off = 0;
+    double s = 0.0;
+    for (int64_t c = 0; c < g.n_drop; ++c) {
+        s += p[base + off];
+        for (int k = 0; k < g.nd; ++k) {  // odometer over the dropped species
+            if (++y[k] < g.drad[k]) {
+                off += g.dstride[k];
+                break;
+            }
+            off -= (int64_t)(g.drad[k] - 1) * g.dstride[k];
+            y[k] = 0;
+        }
+    }
+    out[o] = s;
+}
+
+// tmp holds G dd partials (2G doubles) and one more double for the result.
+fsp_status reduce_partials(const double *tmp, int G, double *h_out, hipStream_t s)
+{
+    k_sum_final<<<1, kBlock, 0, s>>>(tmp, G, const_cast<double *>(tmp) + 2 * G);
+    FSP_HIP(hipGetLastError());
+    FSP_HIP(hipMemcpyAsync(h_out, tmp + 2 * G, sizeof(double), hipMemcpyDeviceToHost, s));
+    FSP_HIP(hipStreamSynchronize(s));
+    return FSP_OK;
 }
 
 // Geometry-only model (bounds) for the state-space calls.
@@ -105,14 +185,12 @@ fsp_status vec_sum(const double *d_x, int64_t n, double *h_sum, hipStream_t s)
 {
     const int G = grid_for((n + kBlock - 1) / kBlock, 1024);
     double *tmp = nullptr;
-    FSP_HIP(hipMalloc(&tmp, sizeof(double) * (size_t)(G + 1)));
+    FSP_HIP(hipMalloc(&tmp, sizeof(double) * (size_t)(2 * G + 1)));
     k_sum_partials<<<G, kBlock, 0, s>>>(d_x, n, tmp);
-    k_sum_final<<<1, kBlock, 0, s>>>(tmp, G, tmp + G);
     FSP_HIP(hipGetLastError());
-    FSP_HIP(hipMemcpyAsync(h_sum, tmp + G, sizeof(double), hipMemcpyDeviceToHost, s));
-    FSP_HIP(hipStreamSynchronize(s));
-    hipFree(tmp);
-    return FSP_OK;
+    fsp_status st = reduce_partials(tmp, G, h_sum, s);
+    (void)hipFree(tmp);
+    return st;
 }
 
 }  // namespace fsp
@@ -413,6 +491,77 @@ fsp_status fsp_mat_export_hyb(fsp_mat A, double *d_diag, double *d_slot_vals, in
     if (d_sink_out && M.with_sink)
         FSP_HIP(hipMemcpyAsync(d_sink_out, M.sink_out, sizeof(double) * (size_t)M.n, hipMemcpyDeviceToDevice, s));
     FSP_HIP(hipStreamSynchronize(s));
+    return FSP_OK;
+}
+
+fsp_status fsp_loglik_indices(const double *d_p, const int *d_idx, int64_t count,
+                              double floor_val, double *h_ll, void *stream)
+{
+    if (!h_ll || count < 0 || (count > 0 && (!d_p || !d_idx)))
+        return fail(FSP_ERR_INVALID, "bad argument");
+    if (count == 0) {
+        *h_ll = 0.0;
+        return FSP_OK;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int G = grid_for((count + kBlock - 1) / kBlock, 1024);
+    double *tmp = nullptr;
+    FSP_HIP(hipMalloc(&tmp, sizeof(double) * (size_t)(2 * G + 1)));
+    k_loglik_idx<<<G, kBlock, 0, s>>>(d_p, d_idx, count, floor_val, tmp);
+    FSP_HIP(hipGetLastError());
+    fsp_status st = reduce_partials(tmp, G, h_ll, s);
+    (void)hipFree(tmp);
+    return st;
+}
+
+fsp_status fsp_loglik_weighted(const double *d_q, const double *d_w, int64_t count,
+                               double floor_val, double *h_ll, void *stream)
+{
+    if (!h_ll || count < 0 || (count > 0 && (!d_q || !d_w)))
+        return fail(FSP_ERR_INVALID, "bad argument");
+    if (count == 0) {
+        *h_ll = 0.0;
+        return FSP_OK;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int G = grid_for((count + kBlock - 1) / kBlock, 1024);
+    double *tmp = nullptr;
+    FSP_HIP(hipMalloc(&tmp, sizeof(double) * (size_t)(2 * G + 1)));
+    k_loglik_w<<<G, kBlock, 0, s>>>(d_q, d_w, count, floor_val, tmp);
+    FSP_HIP(hipGetLastError());
+    fsp_status st = reduce_partials(tmp, G, h_ll, s);
+    (void)hipFree(tmp);
+    return st;
+}
+
+fsp_status fsp_marginal(const double *d_p, int n_species, const int *bounds, const int *keep_mask,
+                        double *d_out, void *stream)
+{
+    if (!d_p || !d_out || !bounds || !keep_mask) return fail(FSP_ERR_INVALID, "bad argument");
+    if (n_species < 1 || n_species > kMaxSpecies) return fail(FSP_ERR_UNSUPPORTED, "n_species");
+    MargGeom g;
+    std::memset(&g, 0, sizeof(g));
+    int64_t stride = 1;
+    g.n_out = 1;
+    g.n_drop = 1;
+    for (int s = 0; s < n_species; ++s) {
+        if (bounds[s] < 0) return fail(FSP_ERR_INVALID, "negative bound");
+        const int rad = bounds[s] + 1;
+        if (keep_mask[s]) {
+            g.krad[g.nk] = rad;
+            g.kstride[g.nk++] = stride;
+            g.n_out *= rad;
+        } else {
+            g.drad[g.nd] = rad;
+            g.dstride[g.nd++] = stride;
+            g.n_drop *= rad;
+        }
+        stride *= rad;
+        if (stride > 2147483647LL) return fail(FSP_ERR_OVERFLOW, "more than 2^31-1 states");
+    }
+    hipStream_t s = (hipStream_t)stream;
+    k_marginal<<<(unsigned)((g.n_out + kBlock - 1) / kBlock), kBlock, 0, s>>>(d_p, g, d_out);
+    FSP_HIP(hipGetLastError());
     return FSP_OK;
 }
 

@@ -36,6 +36,91 @@ __device__ inline double block_sum(double v, double *red)
     return s;
 }
 
+// --------------------------------------------- order-independent reductions
+// Cross-row sums (dot products, norms, the sink row) are accumulated as
+// unevaluated sums s + c with error-free transformations (Knuth's TwoSum,
+// TwoProd through fma), as in Ogita, Rump & Oishi, "Accurate sum and dot
+// product", SIAM J. Sci. Comput. 26 (2005), Dot2.  The result fl(s + c) is
+// as accurate as a twice-working-precision sum, so it does not depend on the
+// reduction tree except when the exact sum lies within ~eps^2 of a rounding
+// boundary: the device and the CPU oracle then agree bit for bit.  Needs
+// -ffp-contract=off (a contracted TwoSum is not error-free).
+struct dd {
+    double s, c;
+};
+
+__host__ __device__ inline dd dd_zero() { return dd{0.0, 0.0}; }
+
+__host__ __device__ inline void two_sum(double a, double b, double &s, double &e)
+{
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ inline dd dd_add(dd x, double v)
+{
+    double s, e;
+    two_sum(x.s, v, s, e);
+    return dd{s, x.c + e};
+}
+
+__device__ inline dd dd_add_prod(dd x, double a, double b)
+{
+    const double p = a * b;
+    const double pe = fma(a, b, -p);
+    double s, e;
+    two_sum(x.s, p, s, e);
+    return dd{s, x.c + (e + pe)};
+}
+
+__device__ inline dd dd_merge(dd x, dd y)
+{
+    double s, e;
+    two_sum(x.s, y.s, s, e);
+    return dd{s, (x.c + y.c) + e};
+}
+
+__device__ inline double dd_value(dd x) { return x.s + x.c; }
+
+__device__ inline dd wave_sum_dd(dd v)
+{
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        dd o;
+        o.s = __shfl_xor(v.s, off, kWave);
+        o.c = __shfl_xor(v.c, off, kWave);
+        v = dd_merge(v, o);
+    }
+    return v;
+}
+
+// Block dd sum for blockDim.x == kBlock; valid in every thread.  `red`:
+// LDS scratch of at least 2 * kBlock / kWave doubles.
+__device__ inline dd block_sum_dd(dd v, double *red)
+{
+    v = wave_sum_dd(v);
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    __syncthreads();
+    if (l == 0) {
+        red[2 * w] = v.s;
+        red[2 * w + 1] = v.c;
+    }
+    __syncthreads();
+    dd s = dd_zero();
+#pragma unroll
+    for (int k = 0; k < kBlock / kWave; ++k) s = dd_merge(s, dd{red[2 * k], red[2 * k + 1]});
+    return s;
+}
+
+// Sum of `count` dd partials stored as (s, c) pairs, fixed order.
+__device__ inline dd sum_dd_partials(const double *__restrict__ p, int count, double *red)
+{
+    dd s = dd_zero();
+    for (int b = threadIdx.x; b < count; b += blockDim.x) s = dd_merge(s, dd{p[2 * b], p[2 * b + 1]});
+    return block_sum_dd(s, red);
+}
+
 // ------------------------------------------------------------ state decode
 template <int D>
 __device__ inline void decode(uint32_t i, const DevModel *__restrict__ dm, int (&x)[D])

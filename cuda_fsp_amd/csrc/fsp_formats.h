@@ -37,7 +37,7 @@ struct CsrF {
 // phase 2 lets each thread add its row's products in column order.
 template <typename V, class X>
 __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
-                                  double *prod, double &sink)
+                                  double *prod, dd &sink)
 {
     const uint32_t r0 = tile * kBlock;
     const uint32_t r1 = min(r0 + (uint32_t)kBlock, A.n);
@@ -59,7 +59,7 @@ __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
     if (A.sink_off) {
         const int s0 = A.sink_off[r0], s1 = A.sink_off[r1];
         for (int k = s0 + (int)threadIdx.x; k < s1; k += kBlock)
-            sink += A.sink_vals[k] * x(A.sink_cols[k]);
+            sink = dd_add_prod(sink, A.sink_vals[k], x(A.sink_cols[k]));
     }
     return y;
 }
@@ -78,7 +78,7 @@ struct HybF {
 };
 
 template <typename V, class X>
-__device__ inline double hyb_row(const HybF<V> &A, uint32_t i, const X &x, double &sink)
+__device__ inline double hyb_row(const HybF<V> &A, uint32_t i, const X &x, dd &sink)
 {
     if (i >= A.n) return 0.0;
     const double xi = x(i);
@@ -87,7 +87,7 @@ __device__ inline double hyb_row(const HybF<V> &A, uint32_t i, const X &x, doubl
         const size_t o = (size_t)u * A.n + i;
         y += (double)A.slot_vals[o] * x(A.slot_cols[o]);
     }
-    if (A.sink_out) sink += A.sink_out[i] * xi;
+    if (A.sink_out) sink = dd_add_prod(sink, A.sink_out[i], xi);
     return y;
 }
 
@@ -113,7 +113,7 @@ struct KronF {
 
 template <int D, typename V, class X>
 __device__ inline double kron_row(const KronF<V> &A, uint32_t i, const X &x, const V *tab,
-                                  double &sink)
+                                  dd &sink)
 {
     if (i >= A.n) return 0.0;
     int xs[D];
@@ -145,7 +145,7 @@ __device__ inline double kron_row(const KronF<V> &A, uint32_t i, const X &x, con
         y += c * (xs_src * alpha - xh * beta);
         if (A.with_sink && target_leaves<D>(rx, xs, A.dm)) out += c * beta;
     }
-    sink += out * xh;
+    if (A.with_sink) sink = dd_add_prod(sink, out, xh);
     return y;
 }
 

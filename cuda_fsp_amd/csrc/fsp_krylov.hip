@@ -115,11 +115,21 @@ struct ArnArgs {
     int G;
 };
 
-// Per-wave accumulation of up to kMaxKrylov+2 dot products in LDS.
-__device__ inline void wave_accum(double *acc, int k, double v)
+// Per-wave dd accumulation of up to kMaxKrylov+2 dot products in LDS:
+// acc[(wave * (kMaxKrylov+2) + k) * 2 + {0: s, 1: c}].
+constexpr int kAccLen = (kBlock / kWave) * (kMaxKrylov + 2) * 2;
+constexpr int kRedLen = 2 * kBlock / kWave;
+
+__device__ inline void wave_accum(double *acc, int k, double a, double b)
 {
-    v = wave_sum(v);
-    if ((threadIdx.x % kWave) == 0) acc[(threadIdx.x / kWave) * (kMaxKrylov + 2) + k] += v;
+    dd v = dd_add_prod(dd_zero(), a, b);
+    v = wave_sum_dd(v);
+    if ((threadIdx.x % kWave) == 0) {
+        double *p = acc + ((threadIdx.x / kWave) * (kMaxKrylov + 2) + k) * 2;
+        const dd t = dd_merge(dd{p[0], p[1]}, v);
+        p[0] = t.s;
+        p[1] = t.c;
+    }
 }
 
 template <typename B>
@@ -141,30 +151,39 @@ __device__ inline void arn_row_epilogue(const ArnArgs &a, uint32_t i, double y, 
         if (live) a.out[i] = y;
         for (int k = a.istart; k <= a.j; ++k) {
             const double vk = live ? (k == a.j ? xi : basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i)) : 0.0;
-            wave_accum(acc, k - a.istart, live ? y * vk : 0.0);
+            wave_accum(acc, k - a.istart, live ? y : 0.0, vk);
         }
     } else {
-        wave_accum(acc, 0, live ? y * y : 0.0);
+        wave_accum(acc, 0, live ? y : 0.0, live ? y : 0.0);
     }
 }
 
+// Partials layout: dd pairs, dot k of block b at 2 * (k * G + b); the sink
+// sum after the nk dots.
 template <typename B>
-__device__ inline void arn_block_end(const ArnArgs &a, double *acc, double sink, double *red)
+__device__ inline void arn_block_end(const ArnArgs &a, double *acc, dd sink, double *red)
 {
     __syncthreads();
     const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
     for (int k = threadIdx.x; k < nk; k += kBlock) {
-        double s = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) s += acc[w * (kMaxKrylov + 2) + k];
-        a.partials[(size_t)k * a.G + blockIdx.x] = s;
+        dd s = dd_zero();
+        for (int w = 0; w < kBlock / kWave; ++w) {
+            const double *p = acc + (w * (kMaxKrylov + 2) + k) * 2;
+            s = dd_merge(s, dd{p[0], p[1]});
+        }
+        a.partials[2 * ((size_t)k * a.G + blockIdx.x)] = s.s;
+        a.partials[2 * ((size_t)k * a.G + blockIdx.x) + 1] = s.c;
     }
-    const double t = block_sum(sink, red);
-    if (threadIdx.x == 0) a.partials[(size_t)nk * a.G + blockIdx.x] = t;
+    const dd t = block_sum_dd(sink, red);
+    if (threadIdx.x == 0) {
+        a.partials[2 * ((size_t)nk * a.G + blockIdx.x)] = t.s;
+        a.partials[2 * ((size_t)nk * a.G + blockIdx.x) + 1] = t.c;
+    }
 }
 
 __device__ inline void acc_init(double *acc)
 {
-    for (int k = threadIdx.x; k < (kBlock / kWave) * (kMaxKrylov + 2); k += kBlock) acc[k] = 0.0;
+    for (int k = threadIdx.x; k < kAccLen; k += kBlock) acc[k] = 0.0;
     __syncthreads();
 }
 
@@ -174,11 +193,11 @@ __global__ void __launch_bounds__(kBlock) k_arn_csr(CsrF<V> A, ArnArgs a)
     if (arn_skip<B>(a)) return;
     extern __shared__ double smem[];
     double *acc = smem;
-    double *red = acc + (kBlock / kWave) * (kMaxKrylov + 2);
-    double *prod = red + kBlock / kWave;
+    double *red = acc + kAccLen;
+    double *prod = red + kRedLen;
     acc_init(acc);
     const KryX<B> X{a.in, a.st->sigma[a.j]};
-    double sink = 0.0;
+    dd sink = dd_zero();
     for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
         const double y = csr_tile<V>(A, t, X, prod, sink);
         const uint32_t i = t * kBlock + threadIdx.x;
@@ -193,10 +212,10 @@ __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
     if (arn_skip<B>(a)) return;
     extern __shared__ double smem[];
     double *acc = smem;
-    double *red = acc + (kBlock / kWave) * (kMaxKrylov + 2);
+    double *red = acc + kAccLen;
     acc_init(acc);
     const KryX<B> X{a.in, a.st->sigma[a.j]};
-    double sink = 0.0;
+    dd sink = dd_zero();
     for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
         const uint32_t i = t * kBlock + threadIdx.x;
         const double y = hyb_row<V>(A, i, X, sink);
@@ -211,8 +230,8 @@ __global__ void __launch_bounds__(kBlock) k_arn_kron(KronF<V> A, ArnArgs a, int 
     if (arn_skip<B>(a)) return;
     extern __shared__ double smem[];
     double *acc = smem;
-    double *red = acc + (kBlock / kWave) * (kMaxKrylov + 2);
-    V *tabl = (V *)(red + kBlock / kWave);
+    double *red = acc + kAccLen;
+    V *tabl = (V *)(red + kRedLen);
     acc_init(acc);
     const V *tab = A.tab_g;
     if (use_lds && A.tab_g) {
@@ -220,7 +239,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_kron(KronF<V> A, ArnArgs a, int 
         tab = tabl;
     }
     const KryX<B> X{a.in, a.st->sigma[a.j]};
-    double sink = 0.0;
+    dd sink = dd_zero();
     for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
         const uint32_t i = t * kBlock + threadIdx.x;
         const double y = kron_row<D, V>(A, i, X, tab, sink);
@@ -235,27 +254,24 @@ template <typename B>
 __global__ void __launch_bounds__(kBlock) k_arn_dots(ArnArgs a, int with_sink, int M2)
 {
     if (arn_skip<B>(a)) return;
-    __shared__ double red[kBlock / kWave];
+    __shared__ double red[kRedLen];
     KState *st = a.st;
     const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
     double ys = 0.0, xs = 0.0;
     if (with_sink) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < a.G; b += kBlock) s += a.partials[(size_t)nk * a.G + b];
-        ys = block_sum(s, red);
+        ys = dd_value(sum_dd_partials(a.partials + 2 * (size_t)nk * a.G, a.G, red));
         xs = (double)(B)(a.in[a.n] * st->sigma[a.j]);
     }
     const int k = blockIdx.x;
-    double s = 0.0;
-    for (int b = threadIdx.x; b < a.G; b += kBlock) s += a.partials[(size_t)k * a.G + b];
-    s = block_sum(s, red);
+    dd sd = sum_dd_partials(a.partials + 2 * (size_t)k * a.G, a.G, red);
     if (threadIdx.x == 0) {
         if (a.mode == 0) {
             const int kk = a.istart + k;
             if (with_sink) {
                 const double vk = (kk == a.j) ? xs : basis<B>(a.V64, (const B *)a.Vb, a.nv, kk, a.n);
-                s += ys * vk;
+                sd = dd_add_prod(sd, ys, vk);
             }
+            const double s = dd_value(sd);
             st->H[kk + a.j * M2] = s;
             if (k == 0) {
                 st->n_matvec += 1;
@@ -265,7 +281,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_dots(ArnArgs a, int with_sink, i
                 }
             }
         } else {
-            st->avnorm = sqrt(s + ys * ys);
+            st->avnorm = sqrt(dd_value(dd_add_prod(sd, ys, ys)));
             st->H[(a.j + 1) + a.j * M2] = 1.0;  // H(m+1, m) = 1 (src/KryExpvFSP.cu:115)
             st->n_matvec += 1;
             if (with_sink) store_basis<B>(a.V64, (B *)a.Vb, a.nv, a.j, a.n, xs);
@@ -278,29 +294,30 @@ template <typename B>
 __global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2)
 {
     if (arn_skip<B>(a)) return;
-    __shared__ double red[kBlock / kWave];
+    __shared__ double red[kRedLen];
     const KState *st = a.st;
-    double nrm = 0.0;
+    dd nrm = dd_zero();
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += gridDim.x * kBlock) {
         double acc = 0.0;
         for (int k = a.istart; k <= a.j; ++k)
             acc += basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) * st->H[k + a.j * M2];
         const double w = -1.0 * acc + a.out[i];
         a.out[i] = w;
-        nrm += w * w;
+        nrm = dd_add_prod(nrm, w, w);
     }
-    const double t = block_sum(nrm, red);
-    if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+    const dd t = block_sum_dd(nrm, red);
+    if (threadIdx.x == 0) {
+        a.partials[2 * blockIdx.x] = t.s;
+        a.partials[2 * blockIdx.x + 1] = t.c;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_arn_norm(ArnArgs a, int M2, int G)
 {
     KState *st = a.st;
     if (!st->active || st->broke) return;
-    __shared__ double red[kBlock / kWave];
-    double s = 0.0;
-    for (int b = threadIdx.x; b < G; b += kBlock) s += a.partials[b];
-    s = sqrt(block_sum(s, red));
+    __shared__ double red[kRedLen];
+    const double s = sqrt(dd_value(sum_dd_partials(a.partials, G, red)));
     if (threadIdx.x == 0) {
         if (s < st->btol) {  // happy breakdown (src/KryExpvFSP.cu:90-98)
             st->k1 = 0;
@@ -320,12 +337,15 @@ __global__ void __launch_bounds__(kBlock) k_norm_partials(const double *__restri
                                                          double *partials, const KState *st)
 {
     if (st && st->t_now >= st->t_final) return;
-    __shared__ double red[kBlock / kWave];
-    double s = 0.0;
+    __shared__ double red[kRedLen];
+    dd s = dd_zero();
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nv; i += gridDim.x * kBlock)
-        s += v[i] * v[i];
-    s = block_sum(s, red);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+        s = dd_add_prod(s, v[i], v[i]);
+    s = block_sum_dd(s, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = s.s;
+        partials[2 * blockIdx.x + 1] = s.c;
+    }
 }
 
 // Step prologue (src/KryExpvFSP.cu:41-65): beta = |v|, first step size,
@@ -334,7 +354,7 @@ __global__ void __launch_bounds__(kBlock) k_norm_partials(const double *__restri
 __global__ void __launch_bounds__(kBlock) k_step_begin(KState *st, const double *partials, int G,
                                                       int m, int M2)
 {
-    __shared__ double red[kBlock / kWave];
+    __shared__ double red[kRedLen];
     __shared__ int act;
     if (threadIdx.x == 0) act = st->t_now < st->t_final;
     __syncthreads();
@@ -342,9 +362,7 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(KState *st, const double 
         if (threadIdx.x == 0) st->active = 0;
         return;
     }
-    double s = 0.0;
-    for (int b = threadIdx.x; b < G; b += kBlock) s += partials[b];
-    s = block_sum(s, red);
+    const double s = dd_value(sum_dd_partials(partials, G, red));
     for (int k = threadIdx.x; k < M2 * M2; k += kBlock) st->H[k] = 0.0;
     if (threadIdx.x == 0) {
         st->active = 1;
@@ -602,8 +620,8 @@ template <typename B>
 __global__ void __launch_bounds__(kBlock) k_cb_dots(ArnArgs a)
 {
     if (arn_skip<B>(a)) return;
-    __shared__ double acc[(kBlock / kWave) * (kMaxKrylov + 2)];
-    __shared__ double red[kBlock / kWave];
+    __shared__ double acc[kAccLen];
+    __shared__ double red[kRedLen];
     acc_init(acc);
     for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
         const uint32_t i = t * kBlock + threadIdx.x;
@@ -611,13 +629,13 @@ __global__ void __launch_bounds__(kBlock) k_cb_dots(ArnArgs a)
         const double y = live ? a.out[i] : 0.0;
         if (a.mode == 0) {
             for (int k = a.istart; k <= a.j; ++k)
-                wave_accum(acc, k - a.istart,
-                           live ? y * basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) : 0.0);
+                wave_accum(acc, k - a.istart, y,
+                           live ? basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) : 0.0);
         } else {
-            wave_accum(acc, 0, y * y);
+            wave_accum(acc, 0, y, y);
         }
     }
-    arn_block_end<B>(a, acc, 0.0, red);
+    arn_block_end<B>(a, acc, dd_zero(), red);
 }
 
 }  // namespace
@@ -639,7 +657,7 @@ static fsp_status expv_alloc(ExpvImpl &E)
     FSP_HIP(hipMalloc(&E.W[1], sizeof(double) * nv));
     E.tiles = ((int64_t)nv + kBlock - 1) / kBlock;
     E.G = grid_for(E.tiles, 2048);
-    FSP_HIP(hipMalloc(&E.partials, sizeof(double) * (size_t)E.G * (kMaxKrylov + 4)));
+    FSP_HIP(hipMalloc(&E.partials, 2 * sizeof(double) * (size_t)E.G * (kMaxKrylov + 4)));
     KState &h = *E.h_st;
     h.t_final = 0.0;
     h.t_now = 0.0;
@@ -677,7 +695,7 @@ template <typename B>
 static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
 {
     MatImpl &M = *E.A;
-    const size_t acc_b = sizeof(double) * ((kBlock / kWave) * (kMaxKrylov + 2) + kBlock / kWave);
+    const size_t acc_b = sizeof(double) * (kAccLen + kRedLen);
     const bool mixed = M.prec == FSP_MIXED;
     if (M.format == FSP_CSR) {
         if (mixed) {
@@ -857,6 +875,9 @@ extern "C" fsp_status fsp_expv_create(fsp_mat A, fsp_matvec_cb cb, void *cb_ctx,
     E.q_iop = q_iop;
     E.anorm = anorm;
     E.s = (hipStream_t)stream;
+    // FSP_EXPV_GRAPH=0 launches each step's kernels directly instead of
+    // replaying a captured HIP graph (for tools that do not follow graphs)
+    if (const char *g = std::getenv("FSP_EXPV_GRAPH")) E.use_graph = std::atoi(g) != 0;
     fsp_status st = expv_alloc(E);
     if (st != FSP_OK) {
         expv_free(E);
@@ -981,6 +1002,21 @@ extern "C" fsp_status fsp_expv_get_tau_history(fsp_expv e, double *h_taus, int64
     if (h_taus && cap > 0 && n > 0)
         FSP_HIP(hipMemcpy(h_taus, e->impl.tau_log, sizeof(double) * (size_t)std::min(n, cap),
                           hipMemcpyDeviceToHost));
+    return FSP_OK;
+}
+
+extern "C" fsp_status fsp_expv_set_iop(fsp_expv e, int iop, int q_iop)
+{
+    using namespace fsp;
+    if (!e || (iop && q_iop < 1)) return fail(FSP_ERR_INVALID, "bad IOP options");
+    ExpvImpl &E = e->impl;
+    if ((E.iop != 0) == (iop != 0) && (!iop || E.q_iop == q_iop)) return FSP_OK;
+    E.iop = iop ? 1 : 0;
+    E.q_iop = q_iop;
+    if (E.gexec) (void)hipGraphExecDestroy(E.gexec);
+    if (E.graph) (void)hipGraphDestroy(E.graph);
+    E.gexec = nullptr;
+    E.graph = nullptr;
     return FSP_OK;
 }
 

@@ -601,14 +601,22 @@ struct PlainX {
     __device__ double operator()(uint32_t c) const { return x[c]; }
 };
 
+// Sink element y[n] from per-block dd partials (order-independent sum).
 __global__ void k_finalize_sink(const double *__restrict__ partials, int count, double *y,
                                 int accumulate)
 {
-    __shared__ double red[kBlock / kWave];
-    double s = 0.0;
-    for (int k = threadIdx.x; k < count; k += kBlock) s += partials[k];
-    s = block_sum(s, red);
+    __shared__ double red[2 * kBlock / kWave];
+    const double s = dd_value(sum_dd_partials(partials, count, red));
     if (threadIdx.x == 0) *y = accumulate ? *y + s : s;
+}
+
+__device__ inline void put_dd_partial(double *partials, dd sink, double *red)
+{
+    const dd s = block_sum_dd(sink, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = s.s;
+        partials[2 * blockIdx.x + 1] = s.c;
+    }
 }
 
 template <typename V>
@@ -619,16 +627,13 @@ __global__ void __launch_bounds__(kBlock) k_apply_csr(CsrF<V> A, PlainX X, doubl
     extern __shared__ double smem[];
     double *prod = smem;
     double *red = smem + A.cap;
-    double sink = 0.0;
+    dd sink = dd_zero();
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const double v = csr_tile<V>(A, t, X, prod, sink);
         const uint32_t i = t * kBlock + threadIdx.x;
         if (i < A.n) y[i] = accumulate ? y[i] + v : v;
     }
-    if (A.sink_off) {
-        const double s = block_sum(sink, red);
-        if (threadIdx.x == 0) partials[blockIdx.x] = s;
-    }
+    if (A.sink_off) put_dd_partial(partials, sink, red);
 }
 
 template <typename V>
@@ -636,17 +641,14 @@ __global__ void __launch_bounds__(kBlock) k_apply_hyb(HybF<V> A, PlainX X, doubl
                                                       int accumulate, uint32_t tiles,
                                                       double *__restrict__ partials)
 {
-    __shared__ double red[kBlock / kWave];
-    double sink = 0.0;
+    __shared__ double red[2 * kBlock / kWave];
+    dd sink = dd_zero();
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const uint32_t i = t * kBlock + threadIdx.x;
         const double v = hyb_row<V>(A, i, X, sink);
         if (i < A.n) y[i] = accumulate ? y[i] + v : v;
     }
-    if (A.sink_out) {
-        const double s = block_sum(sink, red);
-        if (threadIdx.x == 0) partials[blockIdx.x] = s;
-    }
+    if (A.sink_out) put_dd_partial(partials, sink, red);
 }
 
 template <int D, typename V>
@@ -656,23 +658,20 @@ __global__ void __launch_bounds__(kBlock) k_apply_kron(KronF<V> A, PlainX X, dou
                                                        int transpose)
 {
     extern __shared__ double smem[];
-    __shared__ double red[kBlock / kWave];
+    __shared__ double red[2 * kBlock / kWave];
     const V *tab = A.tab_g;
     if (use_lds && A.tab_g) {
         stage_table((V *)smem, A.tab_g, A.tab_size);
         tab = (const V *)smem;
     }
-    double sink = 0.0;
+    dd sink = dd_zero();
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const uint32_t i = t * kBlock + threadIdx.x;
         const double v = transpose ? kron_row_t<D, V>(A, i, X, tab)
                                    : kron_row<D, V>(A, i, X, tab, sink);
         if (i < A.n) y[i] = accumulate ? y[i] + v : v;
     }
-    if (A.with_sink && !transpose) {
-        const double s = block_sum(sink, red);
-        if (threadIdx.x == 0) partials[blockIdx.x] = s;
-    }
+    if (A.with_sink && !transpose) put_dd_partial(partials, sink, red);
 }
 
 }  // namespace
@@ -712,12 +711,12 @@ fsp_status mat_apply(MatImpl &M, const double *d_x, double *d_y, int accumulate,
     if (M.format == FSP_CSR) {
         if (mixed) {
             CsrF<float> A = csr_view<float>(M);
-            const size_t lds = sizeof(double) * ((size_t)A.cap + kBlock / kWave);
+            const size_t lds = sizeof(double) * ((size_t)A.cap + 2 * kBlock / kWave);
             k_apply_csr<float><<<grid, kBlock, lds, s>>>(A, PlainX{d_x}, d_y, accumulate, tiles,
                                                          M.partials);
         } else {
             CsrF<double> A = csr_view<double>(M);
-            const size_t lds = sizeof(double) * ((size_t)A.cap + kBlock / kWave);
+            const size_t lds = sizeof(double) * ((size_t)A.cap + 2 * kBlock / kWave);
             k_apply_csr<double><<<grid, kBlock, lds, s>>>(A, PlainX{d_x}, d_y, accumulate, tiles,
                                                           M.partials);
         }

@@ -196,6 +196,22 @@ fsp_status fsp_expv_get_tau_history(fsp_expv e, double *h_taus, int64_t cap,
                                     int64_t *count);
 /* Public knobs max_reject / max_nstep (src/KryExpvFSP.h:68-69). */
 fsp_status fsp_expv_set_limits(fsp_expv e, int max_reject, int64_t max_nstep);
+/* Public knobs IOP / q_iop (src/KryExpvFSP.h:64-65), read by the next step. */
+fsp_status fsp_expv_set_iop(fsp_expv e, int iop, int q_iop);
+
+/* ---- likelihood: src/Likelihood.cpp:8-33 ---- */
+/* sum_k log(max(floor, p[idx[k]])) over device index lists; idx < 0 (a state
+ * outside the box) counts as probability 0.  Deterministic reduction. */
+fsp_status fsp_loglik_indices(const double *d_p, const int *d_idx, int64_t count,
+                              double floor_val, double *h_ll, void *stream);
+/* Marginal of p over the species with keep_mask[s] != 0: d_out has
+ * prod_{kept} (bounds[s]+1) entries, kept species in order, first fastest.
+ * No reference counterpart (the joint-histogram likelihood needs it). */
+fsp_status fsp_marginal(const double *d_p, int n_species, const int *bounds,
+                        const int *keep_mask, double *d_out, void *stream);
+/* sum_k w[k] log(max(floor, q[k])) (weighted histogram log-likelihood). */
+fsp_status fsp_loglik_weighted(const double *d_q, const double *d_w, int64_t count,
+                               double floor_val, double *h_ll, void *stream);
 /* Dense matrix exponential on the device (one workgroup, FP64, Pade q=6
  * with scaling and squaring; stands in for arma::expmat at
  * src/KryExpvFSP.cu:125).  Column-major N x N, N <= 52. */

@@ -15,6 +15,62 @@
 
 static int g_threads = 1;
 
+/* Order-independent sums: unevaluated s + c with error-free transformations
+ * (TwoSum; TwoProd through fma), Ogita, Rump & Oishi, SIAM J. Sci. Comput.
+ * 26 (2005), "Dot2".  fl(s + c) is as accurate as a twice-working-precision
+ * sum, so it agrees with the GPU's tree reduction bit for bit except when the
+ * exact sum lies within ~eps^2 of a rounding boundary. */
+typedef struct { double s, c; } ddv;
+
+static inline void two_sum(double a, double b, double *s, double *e)
+{
+    *s = a + b;
+    double bb = *s - a;
+    *e = (a - (*s - bb)) + (b - bb);
+}
+
+static inline ddv dd_add_prod(ddv x, double a, double b)
+{
+    double p = a * b, pe = fma(a, b, -p), s, e;
+    two_sum(x.s, p, &s, &e);
+    ddv r = {s, x.c + (e + pe)};
+    return r;
+}
+
+static inline ddv dd_merge(ddv x, ddv y)
+{
+    double s, e;
+    two_sum(x.s, y.s, &s, &e);
+    ddv r = {s, (x.c + y.c) + e};
+    return r;
+}
+
+static inline double dd_value(ddv x) { return x.s + x.c; }
+
+/* sum_i a[i]*b[i] (b may equal a), threads merged in thread order */
+static double dot_dd(int64_t n, const double *a, const double *b)
+{
+    ddv tot = {0.0, 0.0};
+    int nt = g_threads;
+    ddv part[256];
+    if (nt > 256) nt = 256;
+    for (int t = 0; t < nt; ++t) { part[t].s = 0.0; part[t].c = 0.0; }
+#pragma omp parallel num_threads(nt)
+    {
+#ifdef _OPENMP
+        int t = omp_get_thread_num();
+#else
+        int t = 0;
+#endif
+        ddv acc = {0.0, 0.0};
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < n; ++i) acc = dd_add_prod(acc, a[i], b[i]);
+        part[t] = acc;
+    }
+    for (int t = 0; t < nt; ++t) tot = dd_merge(tot, part[t]);
+    return dd_value(tot);
+}
+
 void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 int orc_get_threads(void) { return g_threads; }
 
@@ -423,11 +479,7 @@ void orc_ell_apply(int64_t n, int U, const double *diag,
             acc += slot_vals[(int64_t)u * n + i] * x[slot_cols[(int64_t)u * n + i]];
         y[i] += acc;
     }
-    if (sink_out) {
-        double s = 0.0;
-        for (int64_t i = 0; i < n; ++i) s += sink_out[i] * x[i];
-        y[n] += s;
-    }
+    if (sink_out) y[n] += dot_dd(n, sink_out, x);
 }
 
 /* ------------------------------------------------------------------ */
@@ -493,11 +545,10 @@ void orc_kron_apply(const orc_model *m, const double *coefs, int with_sink,
     const int d = g.d;
     const int R = g.R;
     const int stride = factor_stride(m);
-    double sink_acc = 0.0;
+    double *outv = with_sink ? (double *)malloc(sizeof(double) * (size_t)n) : NULL;
 #pragma omp parallel num_threads(g_threads)
     {
         int xi[64];
-        double local_sink = 0.0;
 #pragma omp for schedule(static)
         for (int64_t i = 0; i < n; ++i) {
             orc_indx2state(i, xi, d, m->bounds);
@@ -528,12 +579,13 @@ void orc_kron_apply(const orc_model *m, const double *coefs, int with_sink,
                 if (leaves) out += coefs[r] * beta;
             }
             y[i] += yv;
-            local_sink += out * xh;
+            if (outv) outv[i] = out;
         }
-#pragma omp critical
-        sink_acc += local_sink;
     }
-    if (with_sink) y[n] += sink_acc;
+    if (with_sink) {
+        y[n] += dot_dd(n, outv, x);
+        free(outv);
+    }
     geom_free(&g);
 }
 
@@ -651,15 +703,11 @@ struct orc_expv {
     int replay_count, replay_pos;
 };
 
-static double dot_n(int64_t n, const double *a, const double *b)
-{
-    double s = 0.0;
-#pragma omp parallel for num_threads(g_threads) reduction(+ : s) schedule(static)
-    for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
-    return s;
-}
+/* cublasDdot / cublasDnrm2 (src/KryExpvFSP.cu:41,76,86,117) as
+ * order-independent sums */
+static double dot_n(int64_t n, const double *a, const double *b) { return dot_dd(n, a, b); }
 
-static double nrm2_n(int64_t n, const double *a) { return sqrt(dot_n(n, a, a)); }
+static double nrm2_n(int64_t n, const double *a) { return sqrt(dot_dd(n, a, a)); }
 
 orc_expv *orc_expv_create(int64_t n, double t_final, orc_matvec_fn mv,
                           void *ctx, int m, double tol, int iop, int q_iop,
@@ -888,6 +936,46 @@ int orc_expv_csr_outputs(int64_t n, const int *row_ptrs, const int *cols,
 {
     csr_ctx c = {n, row_ptrs, cols, vals};
     orc_expv *e = orc_expv_create(n, t_out[0], csr_mv, &c, m, tol, iop, q_iop, anorm);
+    if (replay && n_replay > 0) orc_expv_set_replay(e, replay, n_replay);
+    int rc = 0;
+    for (int k = 0; k < n_out; ++k) {
+        orc_expv_set_final_time(e, t_out[k]);
+        int r = orc_expv_solve(e, v);
+        if (r) rc = r;
+        memcpy(out + (int64_t)k * n, v, sizeof(double) * (size_t)n);
+    }
+    if (stats) orc_expv_get_stats(e, stats);
+    orc_expv_destroy(e);
+    return rc;
+}
+
+void orc_op_apply(const orc_op *op, const double *x, double *y)
+{
+    if (op->kind == 0) {
+        const int64_t rows = op->n_rows - (op->dd_last ? 1 : 0);
+        orc_csr_apply(rows, op->rp, op->cols, op->vals, 1.0, x, y);
+        if (op->dd_last) { /* the sink row, summed order-independently */
+            ddv s = {0.0, 0.0};
+            for (int64_t k = op->rp[rows]; k < op->rp[rows + 1]; ++k)
+                s = dd_add_prod(s, op->vals[k], x[op->cols[k]]);
+            y[rows] += dd_value(s);
+        }
+    } else if (op->kind == 1) {
+        orc_ell_apply(op->n_rows - (op->sink_out ? 1 : 0), op->U, op->diag, op->slot_vals,
+                      op->slot_cols, op->sink_out, x, y);
+    } else {
+        orc_kron_apply(op->model, op->coefs, op->with_sink, x, y);
+    }
+}
+
+static void op_mv(void *ctx, const double *x, double *y) { orc_op_apply((const orc_op *)ctx, x, y); }
+
+int orc_expv_op_outputs(const orc_op *op, double *v, const double *t_out, int n_out,
+                        double *out, int m, double tol, int iop, int q_iop, double anorm,
+                        const double *replay, int n_replay, orc_expv_stats *stats)
+{
+    const int64_t n = op->n_rows;
+    orc_expv *e = orc_expv_create(n, t_out[0], op_mv, (void *)op, m, tol, iop, q_iop, anorm);
     if (replay && n_replay > 0) orc_expv_set_replay(e, replay, n_replay);
     int rc = 0;
     for (int k = 0; k < n_out; ++k) {

@@ -153,6 +153,29 @@ int orc_expv_csr_outputs(int64_t n, const int *row_ptrs, const int *cols,
                          int n_out, double *out, int m, double tol, int iop,
                          int q_iop, double anorm, const double *replay,
                          int n_replay, orc_expv_stats *stats);
+/* Operators for the expv restatement (y += A x).  kind 0: CSR rows summed
+ * in column order (dd_last: the last row is the sink row, summed
+ * order-independently); 1: this project's ELL/HYB layout; 2: Kronecker.
+ * Sums across rows (the sink element) are order-independent in every kind. */
+typedef struct {
+    int kind;
+    int64_t n_rows;
+    int dd_last;
+    const int *rp, *cols;
+    const double *vals;
+    int U;
+    const double *diag, *slot_vals, *sink_out;
+    const int *slot_cols;
+    const orc_model *model;
+    const double *coefs;
+    int with_sink;
+} orc_op;
+
+void orc_op_apply(const orc_op *op, const double *x, double *y);
+int orc_expv_op_outputs(const orc_op *op, double *v, const double *t_out, int n_out,
+                        double *out, int m, double tol, int iop, int q_iop, double anorm,
+                        const double *replay, int n_replay, orc_expv_stats *stats);
+
 /* 2-significant-digit upward rounding of a step size (see fsp_oracle.c). */
 double orc_round_up_2sig(double t);
 

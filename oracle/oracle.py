@@ -235,3 +235,69 @@ def round_up_2sig(t: float) -> float:
 def loglik_indices(p, idx, floor_val=1e-44):
     idx = np.ascontiguousarray(idx, np.int64)
     return float(lib().orc_loglik_indices(_p(p), _p(idx), len(idx), float(floor_val)))
+
+
+class _Op(C.Structure):
+    _fields_ = [("kind", C.c_int), ("n_rows", C.c_int64), ("dd_last", C.c_int),
+                ("rp", C.c_void_p), ("cols", C.c_void_p), ("vals", C.c_void_p),
+                ("U", C.c_int), ("diag", C.c_void_p), ("slot_vals", C.c_void_p),
+                ("sink_out", C.c_void_p), ("slot_cols", C.c_void_p),
+                ("model", C.c_void_p), ("coefs", C.c_void_p), ("with_sink", C.c_int)]
+
+
+class Operator:
+    """Oracle operator y += A x with the same row arithmetic as one of the
+    GPU formats: 'csr' (merged CSR, column order), 'hyb' (diag + ELL slots),
+    'kron' (Kronecker formula).  Keeps its arrays alive."""
+
+    def __init__(self, net, fmt="csr", with_sink=False, coefs=None):
+        c = np.ascontiguousarray(net.coefs if coefs is None else coefs, np.float64)
+        self.keep = [c]
+        n = net.n_states
+        rows = n + (1 if with_sink else 0)
+        op = _Op()
+        op.n_rows = rows
+        if fmt == "csr":
+            rp, cols, vals = merged_csr(net, coefs=c, with_sink=with_sink)
+            self.keep += [rp, cols, vals]
+            op.kind, op.dd_last = 0, int(with_sink)
+            op.rp, op.cols, op.vals = _p(rp), _p(cols), _p(vals)
+        elif fmt == "hyb":
+            U, dg, sv, sc, sk = ell(net, coefs=c, with_sink=with_sink)
+            sv, sc = np.ascontiguousarray(sv), np.ascontiguousarray(sc)
+            self.keep += [dg, sv, sc, sk]
+            op.kind, op.U = 1, U
+            op.diag, op.slot_vals, op.slot_cols, op.sink_out = _p(dg), _p(sv), _p(sc), _p(sk)
+        elif fmt == "kron":
+            m = OracleModel(net)
+            self.keep.append(m)
+            op.kind, op.with_sink = 2, int(with_sink)
+            op.model, op.coefs = m.ptr, _p(c)
+        else:
+            raise ValueError(fmt)
+        self.op = op
+        self.n_rows = rows
+
+    def apply(self, x, y):
+        L = lib()
+        L.orc_op_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_op_apply(C.addressof(self.op), _p(x), _p(y))
+
+
+def expv_outputs(opr: Operator, v, t_out, m=30, tol=1e-8, iop=False, q_iop=2, anorm=1.0,
+                 replay=None):
+    """KryExpvFSP restatement through output times with an Operator; returns
+    (rc, outputs, stats).  replay: step sizes to force (accepted as given)."""
+    L = lib()
+    L.orc_expv_op_outputs.restype = C.c_int
+    L.orc_expv_op_outputs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                      C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
+                                      C.c_void_p, C.c_int, C.c_void_p]
+    t = np.ascontiguousarray(t_out, np.float64)
+    out = np.empty((len(t), opr.n_rows), np.float64)
+    st = ExpvStats()
+    rep = None if replay is None else np.ascontiguousarray(replay, np.float64)
+    rc = L.orc_expv_op_outputs(C.addressof(opr.op), _p(v), _p(t), len(t), _p(out), int(m),
+                               float(tol), int(iop), int(q_iop), float(anorm), _p(rep),
+                               0 if rep is None else len(rep), C.addressof(st))
+    return rc, out, st

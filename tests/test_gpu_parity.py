@@ -226,13 +226,15 @@ def _expv_parity(net, fmt, p0, t_out, sink, **kw):
     (2) free-running, the step sequences agree and so do the solutions
     (1e-10 L1).  Returns the device outputs and stats."""
     O.set_threads(8)
-    rp, c, v = O.merged_csr(net, with_sink=sink)
+    opr = O.Operator(net, fmt, with_sink=sink)   # same row arithmetic as the format
     got, st, taus = _gpu_expv(net, fmt, p0, t_out, sink=sink, with_taus=True, **kw)
-    rc, rep, st_r = O.expv_csr_outputs(rp, c, v, p0.copy(), t_out, replay=taus, **kw)
+    rc, rep, st_r = O.expv_outputs(opr, p0.copy(), t_out, replay=taus, **kw)
     assert rc == 0 and st_r.i_step == st.i_step
     for k in range(len(t_out)):
+        print(f"[parity] {fmt} t={t_out[k]} replay L1 {l1(got[k], rep[k]):.3e} "
+              f"bit-identical {np.array_equal(got[k], rep[k])}", flush=True)
         assert l1(got[k], rep[k]) <= 1e-10, (fmt, k, l1(got[k], rep[k]))
-    rc, want, st_o = O.expv_csr_outputs(rp, c, v, p0.copy(), t_out, **kw)
+    rc, want, st_o = O.expv_outputs(opr, p0.copy(), t_out, **kw)
     assert rc == 0
     assert st.i_step == st_o.i_step, (fmt, st.i_step, st_o.i_step)
     for k in range(len(t_out)):
