@@ -157,6 +157,28 @@ def cpu_baseline(net, t_out, budget_s=15.0, threads=None):
     return dt, steps, threads
 
 
+def kernels_only(reps):
+    """Short run for hardware-counter passes: calibration kernels of known
+    traffic (fsp_vec_fill writes 8n bytes, fsp_vec_sum reads 8n bytes) on
+    n = 16,777,216, then the SpMV kernels on the five-species network."""
+    big = M.five_species_chain()
+    n = big.n_states
+    x = capi.DeviceBuffer.zeros(n)
+    for _ in range(reps):
+        capi.check(capi.lib().fsp_vec_fill(x.data_ptr(), 1.0, n, None))
+    for _ in range(reps):
+        capi.vec_sum(x, n)
+    for fmt, prec in (("csr", "f64"), ("hyb", "f64"), ("csr", "mixed")):
+        A = capi.Mat(big, fmt, prec)
+        y = capi.DeviceBuffer.zeros(n)
+        for _ in range(reps):
+            A.apply(x, y, accumulate=False)
+        capi.check(capi.lib().fsp_stream_sync(None))
+        print(f"[kernels] {fmt}/{prec} bytes_per_apply={int(A.info.bytes_per_apply)}", flush=True)
+        A.close()
+    print(f"[kernels] calibration bytes per launch: {8 * n}", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,7 +186,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the calibration and SpMV kernels (for counter passes)")
     args = ap.parse_args()
+    if args.kernels_only:
+        kernels_only(5)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path "
               "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)

@@ -52,6 +52,13 @@ class FspExpvStats(C.Structure):
                 ("n_reject", C.c_int64), ("n_breakdown", C.c_int64)]
 
 
+class FspBatchOptions(C.Structure):
+    _fields_ = [("m", C.c_int), ("tol", C.c_double), ("anorm", C.c_double),
+                ("precision", C.c_int), ("iop", C.c_int), ("q_iop", C.c_int),
+                ("block", C.c_int), ("wg_per_cu", C.c_int), ("max_reject", C.c_int),
+                ("max_nstep", C.c_int64)]
+
+
 MATVEC_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p)
 
 # (name, restype, argtypes); every entry point returns fsp_status unless noted
@@ -95,6 +102,14 @@ _SIGS = {
     "fsp_expv_set_limits": (C.c_int, [_vp, _i, _i64]),
     "fsp_expv_get_tau_history": (C.c_int, [_vp, _vp, _i64, _vp]),
     "fsp_expv_set_iop": (C.c_int, [_vp, _i, _i]),
+    "fsp_batch_create": (C.c_int, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "fsp_batch_destroy": (C.c_int, [_vp]),
+    "fsp_batch_set_initial": (C.c_int, [_vp, _i, _vp]),
+    "fsp_batch_solve": (C.c_int, [_vp, _vp]),
+    "fsp_batch_get_solution": (C.c_int, [_vp, _i, _vp, _i64]),
+    "fsp_batch_get_stats": (C.c_int, [_vp, _i, _vp, _vp, _vp]),
+    "fsp_batch_get_config": (C.c_int, [_vp, _vp, _vp]),
+    "fsp_batch_autotune": (C.c_int, [_vp, C.c_char_p, _i, _vp]),
     "fsp_loglik_indices": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
     "fsp_marginal": (C.c_int, [_vp, _i, _vp, _vp, _vp, _vp]),
     "fsp_loglik_weighted": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
@@ -408,6 +423,75 @@ class Expv:
     def close(self):
         if getattr(self, "h", None):
             check(lib().fsp_expv_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """fsp_batch handle: many independent solves v_i <- exp(t_i A_i) v_i in
+    one persistent, graph-captured launch."""
+
+    def __init__(self, nets, t_final, with_sink=None, m=30, tol=1e-8, anorm=1.0,
+                 precision="f64", iop=False, q_iop=2, block=0, wg_per_cu=0,
+                 max_reject=0, max_nstep=0):
+        self.nets = list(nets)
+        count = len(self.nets)
+        arr = (FspModel * count)()
+        self._models = [Model(n) for n in self.nets]
+        for i, mdl in enumerate(self._models):
+            arr[i] = mdl.s
+        self._arr = arr
+        coefs = np.ascontiguousarray(np.concatenate([n.coefs for n in self.nets]), np.float64)
+        tf = np.ascontiguousarray(t_final, np.float64)
+        ws = None if with_sink is None else np.ascontiguousarray([1 if s else 0 for s in with_sink], np.int32)
+        self._keep = [coefs, tf, ws]
+        opt = FspBatchOptions(m, tol, anorm, PRECISIONS[precision], int(iop), int(q_iop), int(block),
+                              int(wg_per_cu), int(max_reject), int(max_nstep))
+        h = C.c_void_p()
+        check(lib().fsp_batch_create(C.cast(arr, C.c_void_p), coefs.ctypes.data, tf.ctypes.data,
+                                     None if ws is None else ws.ctypes.data, count, C.byref(opt),
+                                     None, C.byref(h)))
+        self.h = h
+        self.count = count
+        self.nv = [n.n_states + (1 if (with_sink is not None and with_sink[i]) else 0)
+                   for i, n in enumerate(self.nets)]
+
+    def set_initial(self, i, p0):
+        p = np.ascontiguousarray(p0, np.float64)
+        check(lib().fsp_batch_set_initial(self.h, int(i), p.ctypes.data))
+
+    def solve(self, stream=None):
+        check(lib().fsp_batch_solve(self.h, stream))
+
+    def solution(self, i) -> np.ndarray:
+        out = np.empty(self.nv[i], np.float64)
+        check(lib().fsp_batch_get_solution(self.h, int(i), out.ctypes.data, out.size))
+        return out
+
+    def stats(self, i):
+        st, rj, ss = C.c_int64(0), C.c_int64(0), C.c_int(0)
+        check(lib().fsp_batch_get_stats(self.h, int(i), C.byref(st), C.byref(rj), C.byref(ss)))
+        return st.value, rj.value, ss.value
+
+    def config(self):
+        b, w = C.c_int(0), C.c_int(0)
+        check(lib().fsp_batch_get_config(self.h, C.byref(b), C.byref(w)))
+        return b.value, w.value
+
+    def autotune(self, cache_path=None, force=False) -> float:
+        ms = C.c_double(0.0)
+        check(lib().fsp_batch_autotune(self.h, None if cache_path is None else cache_path.encode(),
+                                       int(force), C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().fsp_batch_destroy(self.h))
             self.h = None
 
     def __del__(self):

@@ -122,18 +122,23 @@ __device__ inline dd sum_dd_partials(const double *__restrict__ p, int count, do
 }
 
 // ------------------------------------------------------------ state decode
+// Mixed-radix digits of row i (species 0 fastest) for a model with
+// dm->d <= D species; digits past d are 0.
 template <int D>
 __device__ inline void decode(uint32_t i, const DevModel *__restrict__ dm, int (&x)[D])
 {
+    const int d = dm->d;
     uint32_t q = i;
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-        if (s == D - 1) {
-            x[s] = (int)q;
-        } else {
-            uint32_t q2 = fast_div(q, dm->fd[s]);
+        if (s < d - 1) {
+            const uint32_t q2 = fast_div(q, dm->fd[s]);
             x[s] = (int)(q - q2 * (uint32_t)dm->radix[s]);
             q = q2;
+        } else if (s == d - 1) {
+            x[s] = (int)q;
+        } else {
+            x[s] = 0;
         }
     }
 }
@@ -195,6 +200,28 @@ __device__ inline void stage_table(TT *lds, const TT *__restrict__ g, int count)
 {
     for (int k = threadIdx.x; k < count; k += blockDim.x) lds[k] = g[k];
     __syncthreads();
+}
+
+// XCD-aware tile walk.  MI355X dispatches workgroup b to XCD b % 8, and each
+// XCD has its own L2: every XCD walks one contiguous range of row tiles (its
+// workgroups interleaved within it), so the x gathers of neighbouring rows
+// hit the same L2 instead of eight.  Falls back to the plain grid-stride
+// order for grids smaller than the XCD count.
+constexpr uint32_t kXcds = 8;
+
+struct TileWalk {
+    uint32_t t, end, step;
+};
+
+__device__ inline TileWalk tile_walk(uint32_t tiles)
+{
+    const uint32_t G = gridDim.x;
+    if (G < kXcds || tiles <= G) return TileWalk{blockIdx.x, tiles, G};
+    const uint32_t x = blockIdx.x % kXcds, l = blockIdx.x / kXcds;
+    const uint32_t cnt = G / kXcds + (x < G % kXcds ? 1u : 0u);
+    const uint32_t start = (uint32_t)(((uint64_t)x * tiles) / kXcds);
+    const uint32_t end = (uint32_t)(((uint64_t)(x + 1) * tiles) / kXcds);
+    return TileWalk{start + l, end, cnt};
 }
 
 inline int grid_for(int64_t tiles, int cap)

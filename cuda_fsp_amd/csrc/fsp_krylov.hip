@@ -20,11 +20,12 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "fsp_expm.h"
 #include "fsp_formats.h"
 
 namespace fsp {
 
-constexpr int kP10Lo = -300, kP10Hi = 300;
+
 constexpr long long kTauLogCap = 1 << 16;
 
 struct KState {
@@ -115,22 +116,7 @@ struct ArnArgs {
     int G;
 };
 
-// Per-wave dd accumulation of up to kMaxKrylov+2 dot products in LDS:
-// acc[(wave * (kMaxKrylov+2) + k) * 2 + {0: s, 1: c}].
-constexpr int kAccLen = (kBlock / kWave) * (kMaxKrylov + 2) * 2;
 constexpr int kRedLen = 2 * kBlock / kWave;
-
-__device__ inline void wave_accum(double *acc, int k, double a, double b)
-{
-    dd v = dd_add_prod(dd_zero(), a, b);
-    v = wave_sum_dd(v);
-    if ((threadIdx.x % kWave) == 0) {
-        double *p = acc + ((threadIdx.x / kWave) * (kMaxKrylov + 2) + k) * 2;
-        const dd t = dd_merge(dd{p[0], p[1]}, v);
-        p[0] = t.s;
-        p[1] = t.c;
-    }
-}
 
 template <typename B>
 __device__ inline bool arn_skip(const ArnArgs &a)
@@ -141,38 +127,45 @@ __device__ inline bool arn_skip(const ArnArgs &a)
     return false;
 }
 
-template <typename B>
-__device__ inline void arn_row_epilogue(const ArnArgs &a, uint32_t i, double y, double xi,
-                                        double *acc)
+// Row epilogue: store V_j[i]; w_{j+1}[i] = y; the thread's dd partial dot
+// products y * V_k[i] (k = istart..j) in registers (NKB >= number of dots,
+// a compile-time bucket so the accumulators stay in VGPRs).
+template <typename B, int NKB>
+__device__ inline void arn_epi(const ArnArgs &a, uint32_t i, double y, double xi, dd (&acc)[NKB])
 {
-    const bool live = i < a.n;
-    if (live) store_basis<B>(a.V64, (B *)a.Vb, a.nv, a.j, i, xi);
+    if (i >= a.n) return;
+    store_basis<B>(a.V64, (B *)a.Vb, a.nv, a.j, i, xi);
     if (a.mode == 0) {
-        if (live) a.out[i] = y;
-        for (int k = a.istart; k <= a.j; ++k) {
-            const double vk = live ? (k == a.j ? xi : basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i)) : 0.0;
-            wave_accum(acc, k - a.istart, live ? y : 0.0, vk);
+        a.out[i] = y;
+        const int nk = a.j - a.istart + 1;
+#pragma unroll
+        for (int kk = 0; kk < NKB; ++kk) {
+            if (kk < nk) {
+                const int k = a.istart + kk;
+                const double vk = (k == a.j) ? xi : basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i);
+                acc[kk] = dd_add_prod(acc[kk], y, vk);
+            }
         }
     } else {
-        wave_accum(acc, 0, live ? y : 0.0, live ? y : 0.0);
+        acc[0] = dd_add_prod(acc[0], y, y);
     }
 }
 
 // Partials layout: dd pairs, dot k of block b at 2 * (k * G + b); the sink
 // sum after the nk dots.
-template <typename B>
-__device__ inline void arn_block_end(const ArnArgs &a, double *acc, dd sink, double *red)
+template <int NKB>
+__device__ inline void arn_end(const ArnArgs &a, dd (&acc)[NKB], dd sink, double *red)
 {
-    __syncthreads();
     const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
-    for (int k = threadIdx.x; k < nk; k += kBlock) {
-        dd s = dd_zero();
-        for (int w = 0; w < kBlock / kWave; ++w) {
-            const double *p = acc + (w * (kMaxKrylov + 2) + k) * 2;
-            s = dd_merge(s, dd{p[0], p[1]});
+#pragma unroll
+    for (int kk = 0; kk < NKB; ++kk) {
+        if (kk < nk) {
+            const dd t = block_sum_dd(acc[kk], red);
+            if (threadIdx.x == 0) {
+                a.partials[2 * ((size_t)kk * a.G + blockIdx.x)] = t.s;
+                a.partials[2 * ((size_t)kk * a.G + blockIdx.x) + 1] = t.c;
+            }
         }
-        a.partials[2 * ((size_t)k * a.G + blockIdx.x)] = s.s;
-        a.partials[2 * ((size_t)k * a.G + blockIdx.x) + 1] = s.c;
     }
     const dd t = block_sum_dd(sink, red);
     if (threadIdx.x == 0) {
@@ -181,72 +174,88 @@ __device__ inline void arn_block_end(const ArnArgs &a, double *acc, dd sink, dou
     }
 }
 
-__device__ inline void acc_init(double *acc)
+template <int NKB>
+__device__ inline void acc_zero(dd (&acc)[NKB])
 {
-    for (int k = threadIdx.x; k < kAccLen; k += kBlock) acc[k] = 0.0;
-    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < NKB; ++kk) acc[kk] = dd_zero();
 }
 
-template <typename V, typename B>
+template <typename V, typename B, int NKB>
 __global__ void __launch_bounds__(kBlock) k_arn_csr(CsrF<V> A, ArnArgs a)
 {
     if (arn_skip<B>(a)) return;
     extern __shared__ double smem[];
-    double *acc = smem;
-    double *red = acc + kAccLen;
+    double *red = smem;
     double *prod = red + kRedLen;
-    acc_init(acc);
+    dd acc[NKB];
+    acc_zero(acc);
     const KryX<B> X{a.in, a.st->sigma[a.j]};
     dd sink = dd_zero();
-    for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
-        const double y = csr_tile<V>(A, t, X, prod, sink);
-        const uint32_t i = t * kBlock + threadIdx.x;
-        arn_row_epilogue<B>(a, i, y, i < a.n ? X(i) : 0.0, acc);
+    for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
+        const double y = csr_tile<V>(A, w.t, X, prod, sink);
+        const uint32_t i = w.t * kBlock + threadIdx.x;
+        arn_epi<B, NKB>(a, i, y, i < a.n ? X(i) : 0.0, acc);
     }
-    arn_block_end<B>(a, acc, sink, red);
+    arn_end<NKB>(a, acc, sink, red);
 }
 
-template <typename V, typename B>
+template <typename V, typename B, int NKB>
 __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
 {
     if (arn_skip<B>(a)) return;
     extern __shared__ double smem[];
-    double *acc = smem;
-    double *red = acc + kAccLen;
-    acc_init(acc);
+    double *red = smem;
+    dd acc[NKB];
+    acc_zero(acc);
     const KryX<B> X{a.in, a.st->sigma[a.j]};
     dd sink = dd_zero();
-    for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
-        const uint32_t i = t * kBlock + threadIdx.x;
+    for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
+        const uint32_t i = w.t * kBlock + threadIdx.x;
         const double y = hyb_row<V>(A, i, X, sink);
-        arn_row_epilogue<B>(a, i, y, i < a.n ? X(i) : 0.0, acc);
+        arn_epi<B, NKB>(a, i, y, i < a.n ? X(i) : 0.0, acc);
     }
-    arn_block_end<B>(a, acc, sink, red);
+    arn_end<NKB>(a, acc, sink, red);
 }
 
-template <int D, typename V, typename B>
+template <int D, typename V, typename B, int NKB>
 __global__ void __launch_bounds__(kBlock) k_arn_kron(KronF<V> A, ArnArgs a, int use_lds)
 {
     if (arn_skip<B>(a)) return;
     extern __shared__ double smem[];
-    double *acc = smem;
-    double *red = acc + kAccLen;
+    double *red = smem;
     V *tabl = (V *)(red + kRedLen);
-    acc_init(acc);
     const V *tab = A.tab_g;
     if (use_lds && A.tab_g) {
         stage_table(tabl, A.tab_g, A.tab_size);
         tab = tabl;
     }
+    dd acc[NKB];
+    acc_zero(acc);
     const KryX<B> X{a.in, a.st->sigma[a.j]};
     dd sink = dd_zero();
-    for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
-        const uint32_t i = t * kBlock + threadIdx.x;
+    for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
+        const uint32_t i = w.t * kBlock + threadIdx.x;
         const double y = kron_row<D, V>(A, i, X, tab, sink);
-        arn_row_epilogue<B>(a, i, y, i < a.n ? X(i) : 0.0, acc);
+        arn_epi<B, NKB>(a, i, y, i < a.n ? X(i) : 0.0, acc);
     }
-    arn_block_end<B>(a, acc, sink, red);
+    arn_end<NKB>(a, acc, sink, red);
 }
+
+#define FSP_SWITCH_NK(nk, CALL)                                                \
+    if ((nk) <= 8) {                                                           \
+        constexpr int NK_ = 8;                                                 \
+        CALL;                                                                  \
+    } else if ((nk) <= 16) {                                                   \
+        constexpr int NK_ = 16;                                                \
+        CALL;                                                                  \
+    } else if ((nk) <= 32) {                                                   \
+        constexpr int NK_ = 32;                                                \
+        CALL;                                                                  \
+    } else {                                                                   \
+        constexpr int NK_ = kMaxKrylov + 1;                                    \
+        CALL;                                                                  \
+    }
 
 // Finalize dots (mode 0) or the avnorm (mode 1).  Block k sums partials of
 // dot k in fixed order; the sink element (index n) is handled here.
@@ -380,156 +389,6 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(KState *st, const double 
     }
 }
 
-// --------------------------------------------------------------- dense expm
-// Golub & Van Loan, Matrix Computations, Alg. 11.3.1 (diagonal Pade q = 6,
-// scaling by 2^-j with j = max(0, 1 + floor(log2 |A|_inf)), Gaussian
-// elimination with partial pivoting, j squarings), column-major N x N in
-// LDS, one workgroup.  Same operation order as the oracle's orc_expm, so the
-// two agree bit for bit on equal input.  Result in Nm.
-__device__ void expm_block(int N, double *A, double *X, double *T, double *Nm, double *Dm,
-                           double *red)
-{
-    const int NN = N * N;
-    const int tid = threadIdx.x;
-    // infinity norm
-    double rs = 0.0;
-    if (tid < N)
-        for (int j = 0; j < N; ++j) rs += fabs(A[tid + j * N]);
-    __syncthreads();
-    if (tid < kBlock / kWave) red[tid] = 0.0;
-    __syncthreads();
-    // max over rows: every row sum goes through thread 0 in order
-    __shared__ double rows[64];
-    if (tid < N) rows[tid] = rs;
-    __syncthreads();
-    __shared__ int js_sh;
-    if (tid == 0) {
-        double nrm = 0.0;
-        for (int i = 0; i < N; ++i)
-            if (rows[i] > nrm) nrm = rows[i];
-        int js = 0;
-        if (nrm > 0.0) {
-            int e;
-            frexp(nrm, &e);
-            js = e > 0 ? e : 0;
-        }
-        js_sh = js;
-    }
-    __syncthreads();
-    const int js = js_sh;
-    const double scale = ldexp(1.0, -js);
-    for (int k = tid; k < NN; k += blockDim.x) {
-        A[k] = A[k] * scale;
-        const double id = ((k % N) == (k / N)) ? 1.0 : 0.0;
-        X[k] = id;
-        Nm[k] = id;
-        Dm[k] = id;
-    }
-    __syncthreads();
-    const int q = 6;
-    double c = 1.0;
-    for (int kk = 1; kk <= q; ++kk) {
-        c = c * (double)(q - kk + 1) / (double)((2 * q - kk + 1) * kk);
-        for (int e = tid; e < NN; e += blockDim.x) {
-            const int i = e % N, j = e / N;
-            double acc = 0.0;
-            for (int l = 0; l < N; ++l) acc += A[i + l * N] * X[l + j * N];
-            T[e] = acc;
-        }
-        __syncthreads();
-        const double sgn = (kk % 2) ? -1.0 : 1.0;
-        for (int e = tid; e < NN; e += blockDim.x) {
-            X[e] = T[e];
-            Nm[e] += c * T[e];
-            Dm[e] += sgn * c * T[e];
-        }
-        __syncthreads();
-    }
-    // solve Dm F = Nm
-    __shared__ int piv_sh;
-    for (int k = 0; k < N; ++k) {
-        if (tid == 0) {
-            int p = k;
-            double best = fabs(Dm[k + k * N]);
-            for (int i = k + 1; i < N; ++i)
-                if (fabs(Dm[i + k * N]) > best) {
-                    best = fabs(Dm[i + k * N]);
-                    p = i;
-                }
-            piv_sh = p;
-        }
-        __syncthreads();
-        const int p = piv_sh;
-        if (p != k) {
-            for (int j = tid; j < N; j += blockDim.x) {
-                double t = Dm[k + j * N];
-                Dm[k + j * N] = Dm[p + j * N];
-                Dm[p + j * N] = t;
-                t = Nm[k + j * N];
-                Nm[k + j * N] = Nm[p + j * N];
-                Nm[p + j * N] = t;
-            }
-        }
-        __syncthreads();
-        const double pv = Dm[k + k * N];
-        // multipliers into column k (rows below), then update
-        for (int i = k + 1 + tid; i < N; i += blockDim.x) T[i] = Dm[i + k * N] / pv;
-        __syncthreads();
-        for (int e = tid; e < N * N; e += blockDim.x) {
-            const int i = e % N, j = e / N;
-            if (i <= k) continue;
-            const double l = T[i];
-            if (j > k) Dm[i + j * N] -= l * Dm[k + j * N];
-            Nm[i + j * N] -= l * Nm[k + j * N];
-        }
-        __syncthreads();
-        for (int i = k + 1 + tid; i < N; i += blockDim.x) Dm[i + k * N] = T[i];
-        __syncthreads();
-    }
-    // back substitution, one right-hand side per thread
-    for (int j = tid; j < N; j += blockDim.x) {
-        for (int i = N - 1; i >= 0; --i) {
-            double acc = Nm[i + j * N];
-            for (int k = i + 1; k < N; ++k) acc -= Dm[i + k * N] * Nm[k + j * N];
-            Nm[i + j * N] = acc / Dm[i + i * N];
-        }
-    }
-    __syncthreads();
-    for (int sq = 0; sq < js; ++sq) {
-        for (int e = tid; e < NN; e += blockDim.x) {
-            const int i = e % N, j = e / N;
-            double acc = 0.0;
-            for (int l = 0; l < N; ++l) acc += Nm[i + l * N] * Nm[l + j * N];
-            T[e] = acc;
-        }
-        __syncthreads();
-        for (int e = tid; e < NN; e += blockDim.x) Nm[e] = T[e];
-        __syncthreads();
-    }
-}
-
-// src/KryExpvFSP.cu:148-150 and :172-174: s = 10^(floor(log10 t) - 1),
-// t = ceil(t / s) * s.  The decade is found by comparison with correctly
-// rounded decimal powers (built on the host with strtod) instead of
-// log10/pow, so the device and the host oracle round identically.
-__device__ inline double round_up_2sig(const KState *st, double t)
-{
-    if (!(t > 0.0) || isinf(t)) return t;
-    int lo = kP10Lo, hi = kP10Hi, e = kP10Lo;
-    while (lo <= hi) {
-        const int mid = (lo + hi) / 2;
-        if (st->p10[mid - kP10Lo] <= t) {
-            e = mid;
-            lo = mid + 1;
-        } else {
-            hi = mid - 1;
-        }
-    }
-    if (e - 1 < kP10Lo) return t;
-    const double s = st->p10[e - 1 - kP10Lo];
-    return ceil(t / s) * s;
-}
-
 // Error estimate, reject loop and step update (src/KryExpvFSP.cu:122-174).
 __global__ void __launch_bounds__(kBlock) k_expm_control(KState *st, int m, int max_reject)
 {
@@ -566,7 +425,7 @@ __global__ void __launch_bounds__(kBlock) k_expm_control(KState *st, int m, int 
         }
         if (err <= 1.2 * tau * tol) break;
         tau = 0.9 * tau * pow(tau * tol / err, xm);
-        tau = round_up_2sig(st, tau);
+        tau = round_up_2sig(st->p10, tau);
         ++ireject;
         __syncthreads();
     }
@@ -583,7 +442,7 @@ __global__ void __launch_bounds__(kBlock) k_expm_control(KState *st, int m, int 
         if (st->tau_log && st->i_step < st->tau_cap) st->tau_log[st->i_step] = tau;
         st->t_now = st->t_now + tau;
         double tn = 0.9 * tau * pow(tau * tol / err, xm);
-        st->t_new = round_up_2sig(st, tn);
+        st->t_new = round_up_2sig(st->p10, tn);
         st->i_step += 1;
     }
 }
@@ -616,26 +475,27 @@ __global__ void __launch_bounds__(kBlock) k_cb_scale(ArnArgs a)
     }
 }
 
-template <typename B>
+template <typename B, int NKB>
 __global__ void __launch_bounds__(kBlock) k_cb_dots(ArnArgs a)
 {
     if (arn_skip<B>(a)) return;
-    __shared__ double acc[kAccLen];
     __shared__ double red[kRedLen];
-    acc_init(acc);
-    for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
-        const uint32_t i = t * kBlock + threadIdx.x;
-        const bool live = i < a.nv;
-        const double y = live ? a.out[i] : 0.0;
-        if (a.mode == 0) {
-            for (int k = a.istart; k <= a.j; ++k)
-                wave_accum(acc, k - a.istart, y,
-                           live ? basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) : 0.0);
-        } else {
-            wave_accum(acc, 0, y, y);
+    dd acc[NKB];
+    acc_zero(acc);
+    const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
+    for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
+        const uint32_t i = w.t * kBlock + threadIdx.x;
+        if (i >= a.nv) continue;
+        const double y = a.out[i];
+#pragma unroll
+        for (int kk = 0; kk < NKB; ++kk) {
+            if (kk < nk) {
+                const double vk = a.mode == 0 ? basis<B>(a.V64, (const B *)a.Vb, a.nv, a.istart + kk, i) : y;
+                acc[kk] = dd_add_prod(acc[kk], y, vk);
+            }
         }
     }
-    arn_block_end<B>(a, acc, dd_zero(), red);
+    arn_end<NKB>(a, acc, dd_zero(), red);
 }
 
 }  // namespace
@@ -656,7 +516,10 @@ static fsp_status expv_alloc(ExpvImpl &E)
     FSP_HIP(hipMalloc(&E.W[0], sizeof(double) * nv));
     FSP_HIP(hipMalloc(&E.W[1], sizeof(double) * nv));
     E.tiles = ((int64_t)nv + kBlock - 1) / kBlock;
-    E.G = grid_for(E.tiles, 2048);
+    // row-kernel grid: FSP_ARN_GRID overrides the default cap (timing sweeps)
+    int cap = 2048;
+    if (const char *g = std::getenv("FSP_ARN_GRID")) cap = std::max(8, std::atoi(g));
+    E.G = grid_for(E.tiles, cap);
     FSP_HIP(hipMalloc(&E.partials, 2 * sizeof(double) * (size_t)E.G * (kMaxKrylov + 4)));
     KState &h = *E.h_st;
     h.t_final = 0.0;
@@ -691,40 +554,55 @@ static void expv_free(ExpvImpl &E)
     if (E.h_st) hipHostFree(E.h_st);
 }
 
-template <typename B>
-static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
+template <typename B, int NKB>
+static fsp_status launch_arn_spmv_nk(ExpvImpl &E, ArnArgs &a)
 {
     MatImpl &M = *E.A;
-    const size_t acc_b = sizeof(double) * (kAccLen + kRedLen);
+    const size_t red_b = sizeof(double) * kRedLen;
     const bool mixed = M.prec == FSP_MIXED;
     if (M.format == FSP_CSR) {
         if (mixed) {
             CsrF<float> A = csr_view<float>(M);
-            k_arn_csr<float, B><<<E.G, kBlock, acc_b + sizeof(double) * A.cap, E.s>>>(A, a);
+            k_arn_csr<float, B, NKB><<<E.G, kBlock, red_b + sizeof(double) * A.cap, E.s>>>(A, a);
         } else {
             CsrF<double> A = csr_view<double>(M);
-            k_arn_csr<double, B><<<E.G, kBlock, acc_b + sizeof(double) * A.cap, E.s>>>(A, a);
+            k_arn_csr<double, B, NKB><<<E.G, kBlock, red_b + sizeof(double) * A.cap, E.s>>>(A, a);
         }
     } else if (M.format == FSP_HYB) {
         if (mixed)
-            k_arn_hyb<float, B><<<E.G, kBlock, acc_b, E.s>>>(hyb_view<float>(M), a);
+            k_arn_hyb<float, B, NKB><<<E.G, kBlock, red_b, E.s>>>(hyb_view<float>(M), a);
         else
-            k_arn_hyb<double, B><<<E.G, kBlock, acc_b, E.s>>>(hyb_view<double>(M), a);
+            k_arn_hyb<double, B, NKB><<<E.G, kBlock, red_b, E.s>>>(hyb_view<double>(M), a);
     } else {
         if (mixed && M.hm.has_factors) {
             KronF<float> A = kron_view<float>(M, -1, 0.0);
             const size_t tb = sizeof(float) * (size_t)A.tab_size;
             const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
-            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, float, B><<<E.G, kBlock, acc_b + (use_lds ? tb : 0), E.s>>>(A, a, use_lds)));
+            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, float, B, NKB><<<E.G, kBlock, red_b + (use_lds ? tb : 0), E.s>>>(A, a, use_lds)));
         } else {
             KronF<double> A = kron_view<double>(M, -1, 0.0);
             const size_t tb = sizeof(double) * (size_t)A.tab_size;
             const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
-            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, double, B><<<E.G, kBlock, acc_b + (use_lds ? tb : 0), E.s>>>(A, a, use_lds)));
+            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, double, B, NKB><<<E.G, kBlock, red_b + (use_lds ? tb : 0), E.s>>>(A, a, use_lds)));
         }
     }
     FSP_HIP(hipGetLastError());
     return FSP_OK;
+}
+
+template <typename B>
+static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
+{
+    const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
+    FSP_SWITCH_NK(nk, return (launch_arn_spmv_nk<B, NK_>(E, a)));
+    return FSP_OK;
+}
+
+template <typename B>
+static void launch_cb_dots(ExpvImpl &E, ArnArgs &a)
+{
+    const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
+    FSP_SWITCH_NK(nk, (k_cb_dots<B, NK_><<<E.G, kBlock, 0, E.s>>>(a)));
 }
 
 // Enqueue one full step (no host synchronisation).
@@ -763,7 +641,7 @@ static fsp_status enqueue_step(ExpvImpl &E, double *d_v)
             k_cb_scale<B><<<E.G, kBlock, 0, E.s>>>(a);
             FSP_HIP(hipGetLastError());
             E.cb(E.ctx, a.V64 + (size_t)j * nv, a.out);
-            k_cb_dots<B><<<E.G, kBlock, 0, E.s>>>(a);
+            launch_cb_dots<B>(E, a);
             k_arn_dots<B><<<nk, kBlock, 0, E.s>>>(a, 0, M2);
         }
         FSP_HIP(hipGetLastError());
@@ -773,7 +651,7 @@ static fsp_status enqueue_step(ExpvImpl &E, double *d_v)
         FSP_HIP(hipGetLastError());
     }
     const size_t lds = sizeof(double) * 5 * (size_t)M2 * M2;
-    k_expm_control<<<1, kBlock, lds, E.s>>>(E.d_st, m, E.max_reject);
+    k_expm_control<<<1, kWave, lds, E.s>>>(E.d_st, m, E.max_reject);
     k_solution<B><<<E.G, kBlock, 0, E.s>>>(E.d_st, E.V64, (const B *)(E.mixed ? (void *)E.V32 : (void *)E.V64), nv, d_v);
     FSP_HIP(hipGetLastError());
     return FSP_OK;
@@ -1053,7 +931,7 @@ extern "C" fsp_status fsp_expm_dense(int N, const double *d_A, double *d_F, void
     using namespace fsp;
     if (N < 1 || N > kMaxKrylov + 2) return fail(FSP_ERR_UNSUPPORTED, "N must be in 1..52");
     const size_t lds = sizeof(double) * 5 * (size_t)N * N;
-    k_expm_dense<<<1, kBlock, lds, (hipStream_t)stream>>>(N, d_A, d_F);
+    k_expm_dense<<<1, kWave, lds, (hipStream_t)stream>>>(N, d_A, d_F);
     FSP_HIP(hipGetLastError());
     return FSP_OK;
 }

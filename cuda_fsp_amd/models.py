@@ -256,3 +256,28 @@ def delta_p0(net: ReactionNetwork, state=None, with_sink=False) -> np.ndarray:
             mul *= int(net.bounds[s]) + 1
     p[idx] = 1.0
     return p
+
+
+TOGGLE_RATES = (50.0, 1.0, 16.0, 1.0)
+
+
+def batch_instances(count=512, seed=1, n_min=2048, n_max=262144, t_range=(1.0, 100.0)):
+    """Uneven batch: toggle switches and gene-expression networks drawn 50/50,
+    every rate log-uniform within one decade of its nominal value, box sized
+    so the state count is log-uniform in [n_min, n_max], final time uniform
+    in t_range.  Returns (networks, t_final)."""
+    rng = np.random.default_rng(seed)
+    nets, tf = [], []
+    for i in range(count):
+        target = math.exp(rng.uniform(math.log(n_min), math.log(n_max)))
+        if i % 2 == 0:
+            side = max(2, int(round(math.sqrt(target))))
+            rates = np.asarray(TOGGLE_RATES) * 10.0 ** rng.uniform(-1, 1, 4)
+            net = toggle_switch((side - 1, side - 1), rates=rates)
+        else:
+            side = max(2, int(round(math.sqrt(target / 3.0))))
+            rates = np.asarray(GENE_RATES) * 10.0 ** rng.uniform(-1, 1, 9)
+            net = gene_expression((2, side - 1, side - 1), rates=rates)
+        nets.append(net)
+        tf.append(rng.uniform(*t_range))
+    return nets, np.asarray(tf)

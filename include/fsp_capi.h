@@ -199,6 +199,44 @@ fsp_status fsp_expv_set_limits(fsp_expv e, int max_reject, int64_t max_nstep);
 /* Public knobs IOP / q_iop (src/KryExpvFSP.h:64-65), read by the next step. */
 fsp_status fsp_expv_set_iop(fsp_expv e, int iop, int q_iop);
 
+/* ---- batched solves (no reference counterpart: the reference solves one
+ * system at a time, src/KryExpvFSP.cu:184-188) ---- */
+typedef struct fsp_batch_s *fsp_batch;
+
+typedef struct fsp_batch_options {
+    int m;            /* Krylov dimension (default 30, <= 50)                  */
+    double tol;       /* Krylov tolerance (default 1e-8)                       */
+    double anorm;     /* norm estimate (default 1.0, as the reference)         */
+    int precision;    /* FSP_F64, or FSP_MIXED: FP32 Krylov basis              */
+    int iop, q_iop;   /* incomplete orthogonalisation                          */
+    int block;        /* workgroup size 256/512/1024 (0: 512, or autotuned)    */
+    int wg_per_cu;    /* persistent workgroups per CU, 1 or 2 (0: 2)           */
+    int max_reject;   /* 0: 1000                                               */
+    int64_t max_nstep;/* 0: 100000                                             */
+} fsp_batch_options;
+
+/* count independent solves v_i <- exp(t_final[i] A_i) v_i (Krylov expv,
+ * KryExpvFSP semantics) run by one persistent kernel.  models[i] needs
+ * factor tables; coefs holds the R_i coefficients of every instance in
+ * order; with_sink may be NULL.  Initial vectors default to delta(state 0). */
+fsp_status fsp_batch_create(const fsp_model *models, const double *coefs,
+                            const double *t_final, const int *with_sink, int count,
+                            const fsp_batch_options *options, void *stream,
+                            fsp_batch *out);
+fsp_status fsp_batch_destroy(fsp_batch b);
+fsp_status fsp_batch_set_initial(fsp_batch b, int i, const double *h_p0);
+/* One solve of every instance from its initial vector: a HIP graph (queue
+ * reset + persistent kernel) launched on `stream` (asynchronous). */
+fsp_status fsp_batch_solve(fsp_batch b, void *stream);
+fsp_status fsp_batch_get_solution(fsp_batch b, int i, double *h_p, int64_t cap);
+fsp_status fsp_batch_get_stats(fsp_batch b, int i, int64_t *steps, int64_t *rejects,
+                               int *status);
+fsp_status fsp_batch_get_config(fsp_batch b, int *block, int *wg_per_cu);
+/* Times every (workgroup size, workgroups per CU) candidate on this batch,
+ * keeps the fastest and records it in the JSON file cache_path (NULL: no
+ * cache); a cached entry for the same device / shape is reused unless force. */
+fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, int force, double *best_ms);
+
 /* ---- likelihood: src/Likelihood.cpp:8-33 ---- */
 /* sum_k log(max(floor, p[idx[k]])) over device index lists; idx < 0 (a state
  * outside the box) counts as probability 0.  Deterministic reduction. */

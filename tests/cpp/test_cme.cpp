@@ -10,7 +10,8 @@
 
 #include "FSPSolver.h"
 
-static const double kRates[9] = {0.05, 0.1, 0.02, 0.5, 5.0, 50.0, 2.0, 0.01, 0.1};
+// rates chosen so that the (2, 40, 40) box holds essentially all the mass
+static const double kRates[9] = {0.5, 0.5, 0.5, 0.5, 2.0, 4.0, 1.0, 0.1, 0.5};
 
 // promoter G in {0,1,2}, nascent n, mature m (9 terms)
 static double factor(int x, int s, int r)
@@ -88,6 +89,8 @@ int main(int argc, char **argv)
         }
     std::fwrite(&ll_factor, sizeof(double), 1, f);
     std::fwrite(&ll_table, sizeof(double), 1, f);
+    const std::vector<double> c = model.t_fun(0.0, model.parameters);
+    std::fwrite(c.data(), sizeof(double), c.size(), f);
     std::fclose(f);
     std::printf("cme test done\n");
     return 0;

@@ -1,0 +1,22 @@
+// Error plumbing for the CPU (SIMT stand-in) build of the batched solver:
+// the same thread-local last-error string and fail() as the HIP library's
+// C ABI, so fsp_batch_* report errors through fsp_last_error().
+#include <string>
+
+#include "fsp_internal.h"
+
+namespace fsp {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+fsp_status fail(fsp_status st, const std::string &msg)
+{
+    set_error(msg);
+    return st;
+}
+
+}  // namespace fsp
+
+extern "C" const char *fsp_last_error(void) { return fsp::g_last_error.c_str(); }

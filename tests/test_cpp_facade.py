@@ -51,13 +51,15 @@ def test_cpp_cme_layer(tmp_path_factory):
     idx = np.frombuffer(raw[off:off + 4 * nt * nc], np.int32).reshape(nt, nc)
     off += 4 * nt * nc
     ll_f, ll_t = np.frombuffer(raw[off:off + 16], np.float64)
+    coefs = np.frombuffer(raw[off + 16:off + 16 + 72], np.float64)  # exactly what t_fun gave
     # FSPSolver solution vs the oracle (m = 30, IOP q = 2, tol 1e-8)
-    net = M.gene_expression((2, 40, 40))
+    net = M.gene_expression((2, 40, 40), rates=coefs)
     rp, c, v = O.merged_csr(net)
     p0 = M.delta_p0(net)
     rc, want, st = O.expv_csr_outputs(rp, c, v, p0, [1.0, 5.0, 20.0], iop=True, q_iop=2)
     for k in range(nt):
-        assert np.abs(P[k] - want[k]).sum() <= 1e-7
+        assert np.abs(P[k] - want[k]).sum() <= 1e-10
+        assert P[k].sum() > 1 - 1e-6
     # Likelihood::evaluate = sum log max(1e-44, p[idx]) (src/Likelihood.cpp:14-20)
     ll = sum(np.log(np.maximum(1e-44, np.where(idx[k] >= 0, P[k][np.maximum(idx[k], 0)], 0.0))).sum()
              for k in range(nt))
