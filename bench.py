@@ -157,6 +157,20 @@ def cpu_baseline(net, t_out, budget_s=15.0, threads=None):
     return dt, steps, threads
 
 
+def pmc_traffic(kernel="k_apply_csr<double>"):
+    """HBM bytes per launch of the SpMV kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (counter passes cannot run inside the
+    timed bench); None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "chain_16m_spmv_pmc_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def kernels_only(reps):
     """Short run for hardware-counter passes: calibration kernels of known
     traffic (fsp_vec_fill writes 8n bytes, fsp_vec_sum reads 8n bytes) on
@@ -222,7 +236,9 @@ def main():
     head = spmv[0]
     roofline = {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(head["achieved_GBs"] / HBM_PEAK_GBS, 4),
-                "traffic": None,
+                "traffic": pmc_traffic(),
+                "traffic_source": "profiles/chain_16m_spmv_pmc_traffic.md (FETCH_SIZE, WRITE_SIZE "
+                                  "scaled by a read and a write kernel of known size)",
                 "kernel": "k_apply_csr<double> (merged CSR SpMV, FP64)",
                 "workload": f"five_species_chain n={big.n_states} nnz={head['nnz']}",
                 "algorithmic_bytes": head["algorithmic_bytes"],

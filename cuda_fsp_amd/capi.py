@@ -60,6 +60,19 @@ class FspBatchOptions(C.Structure):
 
 
 MATVEC_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p)
+TCOEF_CB = C.CFUNCTYPE(None, C.c_double, C.POINTER(C.c_double), C.c_void_p)
+
+
+class FspBdfOptions(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("max_order", C.c_int),
+                ("krylov_dim", C.c_int), ("max_restarts", C.c_int), ("h0", C.c_double),
+                ("max_steps", C.c_int64), ("lin_tol_factor", C.c_double)]
+
+
+class FspBdfStats(C.Structure):
+    _fields_ = [("n_steps", C.c_int64), ("n_rejects", C.c_int64), ("n_rhs", C.c_int64),
+                ("n_lin_iters", C.c_int64), ("n_lin_fails", C.c_int64), ("order", C.c_int),
+                ("h_last", C.c_double), ("t", C.c_double)]
 
 # (name, restype, argtypes); every entry point returns fsp_status unless noted
 _vp, _i64, _d, _i = C.c_void_p, C.c_int64, C.c_double, C.c_int
@@ -114,6 +127,12 @@ _SIGS = {
     "fsp_marginal": (C.c_int, [_vp, _i, _vp, _vp, _vp, _vp]),
     "fsp_loglik_weighted": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
     "fsp_expm_dense": (C.c_int, [_i, _vp, _vp, _vp]),
+    "fsp_rhs_apply": (C.c_int, [_vp, _d, _vp, _vp, _vp, _vp, _vp]),
+    "fsp_bdf_create": (C.c_int, [_vp, _vp, _vp, _d, _vp, _vp, _vp]),
+    "fsp_bdf_destroy": (C.c_int, [_vp]),
+    "fsp_bdf_advance": (C.c_int, [_vp, _vp, _d]),
+    "fsp_bdf_reset": (C.c_int, [_vp, _d]),
+    "fsp_bdf_get_stats": (C.c_int, [_vp, _vp]),
 }
 
 _lib = None
@@ -361,6 +380,59 @@ def loglik_weighted(q, w, count, floor_val=1e-12, stream=None) -> float:
 
 def expm_dense(A, F, N, stream=None):
     check(lib().fsp_expm_dense(int(N), ptr(A), ptr(F), stream))
+
+
+def _tcoef_callback(coef_fn, R):
+    """ctypes trampoline for a Python time function t -> R coefficients."""
+    def _cb(t, out, ctx):
+        c = np.asarray(coef_fn(float(t)), np.float64)
+        for r in range(R):
+            out[r] = c[r]
+    return TCOEF_CB(_cb)
+
+
+def rhs_apply(A: "Mat", t, coef_fn, x, y, stream=None):
+    """udot = A(t) u with coefficients coef_fn(t) (the CVODE RHS body)."""
+    cb = _tcoef_callback(coef_fn, A.net.n_reactions)
+    check(lib().fsp_rhs_apply(A.h, float(t), C.cast(cb, C.c_void_p), None, ptr(x), ptr(y), stream))
+
+
+class Bdf:
+    """fsp_bdf handle: dp/dt = A(t) p by variable-order BDF with GMRES stages."""
+
+    def __init__(self, A: "Mat", coef_fn, t0=0.0, rtol=1e-6, atol=1e-12, max_order=5,
+                 krylov_dim=30, max_restarts=5, h0=0.0, max_steps=500000, lin_tol_factor=0.05,
+                 stream=None):
+        self.A = A
+        self._cb = _tcoef_callback(coef_fn, A.net.n_reactions)
+        opt = FspBdfOptions(rtol, atol, max_order, krylov_dim, max_restarts, h0, max_steps,
+                            lin_tol_factor)
+        h = C.c_void_p()
+        check(lib().fsp_bdf_create(A.h, C.cast(self._cb, C.c_void_p), None, float(t0),
+                                   C.byref(opt), stream, C.byref(h)))
+        self.h = h
+
+    def advance(self, v, t_out):
+        check(lib().fsp_bdf_advance(self.h, ptr(v), float(t_out)))
+
+    def reset(self, t0):
+        check(lib().fsp_bdf_reset(self.h, float(t0)))
+
+    def stats(self) -> FspBdfStats:
+        st = FspBdfStats()
+        check(lib().fsp_bdf_get_stats(self.h, C.byref(st)))
+        return st
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().fsp_bdf_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Expv:

@@ -1,0 +1,734 @@
+// fsp_bdf.hip -- time-varying right-hand side and a native BDF integrator.
+//
+// fsp_rhs_apply is the body of the reference's CVODE callbacks
+// (tests/test_sundials.cu:67-91): the coefficients c_r(t) come from a host
+// time function (TcoefFun, src/FSPMat.h:16) and udot = A(t) u overwrites the
+// output (the reference zero-fills then accumulates).
+//
+// fsp_bdf_* integrate dp/dt = A(t) p, the role CVODE's BDF method with an
+// unpreconditioned Krylov linear solver plays in tests/test_sundials.cu:131-186
+// (CVODE itself is third-party and absent, so the method is this file's own):
+//   * variable order 1..5, variable step, fully variable-coefficient BDF: the
+//     corrector uses the derivative at t_new of the Lagrange interpolant
+//     through t_new and the last k accepted points;
+//   * the implicit stage (alpha_0 I - A(t_new)) y = b is linear in y and is
+//     solved by restarted GMRES on the error-weighted system D M D^-1 (D = 1 /
+//     (rtol |y_n| + atol)), started from the predictor;
+//   * error test on the predictor/corrector difference scaled by the exact
+//     variable-step ratio r / (1 + r), r = 1 / (alpha_0 (t_new - t_{n-k}));
+//   * order selection from divided differences of the accepted history.
+// Host control; every vector operation is a device kernel; cross-row sums use
+// double-double accumulation, so results do not depend on the launch shape.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fsp_device.h"
+
+namespace fsp {
+namespace {
+
+constexpr int kMaxOrder = 5;
+constexpr int kHist = kMaxOrder + 2;   // accepted points kept (order 5 needs 7)
+constexpr int kMaxLc = 8;
+constexpr int kRedGrid = 1024;
+
+struct LinComb {
+    const double *v[kMaxLc];
+    double c[kMaxLc];
+    int k;
+};
+
+__global__ void k_lincomb(LinComb L, double *__restrict__ out, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int j = 0; j < L.k; ++j) s += L.c[j] * L.v[j][i];
+        out[i] = s;
+    }
+}
+
+__global__ void k_ewt(const double *__restrict__ y, double *__restrict__ ewt, int64_t n,
+                      double rtol, double atol)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        ewt[i] = 1.0 / (rtol * fabs(y[i]) + atol);
+}
+
+// partial dd sums of (x_i w_i)^2 (w == nullptr: x_i^2)
+__global__ void k_wsq_partials(const double *__restrict__ x, const double *__restrict__ w,
+                               int64_t n, double *partials)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    dd s = dd_zero();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = w ? x[i] * w[i] : x[i];
+        s = dd_add_prod(s, v, v);
+    }
+    s = block_sum_dd(s, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = s.s;
+        partials[2 * blockIdx.x + 1] = s.c;
+    }
+}
+
+// partials[k][b] = dd sum over rows of block b of V_k . w, k < nk
+__global__ void k_multi_dot(const double *__restrict__ V, int64_t ldv, int nk,
+                            const double *__restrict__ w, int64_t n, double *partials)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    for (int k = 0; k < nk; ++k) {
+        const double *vk = V + (int64_t)k * ldv;
+        dd s = dd_zero();
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * blockDim.x)
+            s = dd_add_prod(s, vk[i], w[i]);
+        s = block_sum_dd(s, red);
+        if (threadIdx.x == 0) {
+            partials[2 * ((int64_t)k * gridDim.x + blockIdx.x)] = s.s;
+            partials[2 * ((int64_t)k * gridDim.x + blockIdx.x) + 1] = s.c;
+        }
+        __syncthreads();
+    }
+}
+
+// out[k] = value of the dd sum of partials[k][0..G), one workgroup per k
+__global__ void k_reduce_partials(const double *__restrict__ partials, int G, double *out)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    const dd s = sum_dd_partials(partials + 2 * (int64_t)blockIdx.x * G, G, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = dd_value(s);
+}
+
+// w -= sum_k h_k V_k (k ascending); dd partial sums of the new w squared
+__global__ void k_gs_update(const double *__restrict__ V, int64_t ldv, int nk,
+                            const double *__restrict__ h, double *__restrict__ w, int64_t n,
+                            double *partials)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    dd s = dd_zero();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int k = 0; k < nk; ++k) acc += V[(int64_t)k * ldv + i] * h[k];
+        const double v = -1.0 * acc + w[i];
+        w[i] = v;
+        s = dd_add_prod(s, v, v);
+    }
+    s = block_sum_dd(s, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = s.s;
+        partials[2 * blockIdx.x + 1] = s.c;
+    }
+}
+
+__global__ void k_scale_copy(const double *__restrict__ x, double s, double *__restrict__ y,
+                             int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = x[i] * s;
+}
+
+// t = x / ewt (back to the unscaled space)
+__global__ void k_unscale(const double *__restrict__ x, const double *__restrict__ ewt,
+                          double *__restrict__ t, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        t[i] = x[i] / ewt[i];
+}
+
+// out = alpha0 x - ewt * u   (scaled operator D (alpha0 I - A) D^-1 applied to x)
+__global__ void k_op_post(const double *__restrict__ x, const double *__restrict__ ewt,
+                          const double *__restrict__ u, double alpha0,
+                          double *__restrict__ out, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = alpha0 * x[i] - ewt[i] * u[i];
+}
+
+// r = ewt * (b - alpha0 y + u), u = A y   (scaled residual)
+__global__ void k_resid(const double *__restrict__ b, const double *__restrict__ y,
+                        const double *__restrict__ u, const double *__restrict__ ewt,
+                        double alpha0, double *__restrict__ r, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        r[i] = ewt[i] * (b[i] - alpha0 * y[i] + u[i]);
+}
+
+// y += (sum_k g_k V_k) / ewt
+__global__ void k_sol_update(const double *__restrict__ V, int64_t ldv, int nk,
+                             const double *__restrict__ g, const double *__restrict__ ewt,
+                             double *__restrict__ y, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int k = 0; k < nk; ++k) acc += V[(int64_t)k * ldv + i] * g[k];
+        y[i] += acc / ewt[i];
+    }
+}
+
+inline int vec_grid(int64_t n)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 4096));
+}
+
+// derivative at x[0] of the Lagrange basis through x[0..m]
+void lagrange_deriv_at_first(const double *x, int m, double *w)
+{
+    // l_0'(x0) = sum_{j>0} 1/(x0 - xj); l_j'(x0) = prod_{i != 0, j}(x0 - xi) / prod_{i != j}(xj - xi)
+    double s = 0.0;
+    for (int j = 1; j <= m; ++j) s += 1.0 / (x[0] - x[j]);
+    w[0] = s;
+    for (int j = 1; j <= m; ++j) {
+        double num = 1.0, den = 1.0;
+        for (int i = 0; i <= m; ++i) {
+            if (i == j) continue;
+            den *= (x[j] - x[i]);
+            if (i != 0) num *= (x[0] - x[i]);
+        }
+        w[j] = num / den;
+    }
+}
+
+// value at t of the Lagrange interpolant through x[0..m]: weights w[j]
+void lagrange_value(const double *x, int m, double t, double *w)
+{
+    for (int j = 0; j <= m; ++j) {
+        double v = 1.0;
+        for (int i = 0; i <= m; ++i)
+            if (i != j) v *= (t - x[i]) / (x[j] - x[i]);
+        w[j] = v;
+    }
+}
+
+}  // namespace
+
+struct BdfImpl {
+    MatImpl *A = nullptr;
+    fsp_tcoef_fn coef = nullptr;
+    void *ctx = nullptr;
+    hipStream_t s = nullptr;
+    int64_t n = 0;
+    fsp_bdf_options opt;
+    // history: Y[j] at T[j], j = 0 newest
+    double *Y[kHist] = {};
+    double T[kHist] = {};
+    int nh = 0;
+    bool started = false;
+    double t = 0.0, h = 0.0;
+    int k = 1, steps_at_k = 0, fails = 0;
+    double *ewt = nullptr, *b = nullptr, *yp = nullptr, *ynew = nullptr, *u = nullptr, *tmp = nullptr;
+    double *V = nullptr;            // (kdim + 1) x n scaled Krylov basis
+    double *partials = nullptr;     // 2 x (kdim + 1) x kRedGrid
+    double *dred = nullptr;         // (kdim + 1) reduced values
+    double *h_red = nullptr;        // pinned host copy
+    double *d_small = nullptr;      // kdim + 1 coefficients on the device
+    std::vector<double> coefs;
+    fsp_bdf_stats st;
+};
+
+namespace {
+
+fsp_status set_time(BdfImpl &B, double t)
+{
+    B.coef(t, B.coefs.data(), B.ctx);
+    return mat_revalue(*B.A, B.coefs.data(), B.s);
+}
+
+fsp_status lincomb(BdfImpl &B, int k, const double *const *v, const double *c, double *out)
+{
+    LinComb L;
+    L.k = k;
+    for (int j = 0; j < k; ++j) {
+        L.v[j] = v[j];
+        L.c[j] = c[j];
+    }
+    hipLaunchKernelGGL(k_lincomb, dim3(vec_grid(B.n)), dim3(kBlock), 0, B.s, L, out, B.n);
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
+}
+
+// reduced values of `count` partial rows -> host
+fsp_status pull_reduced(BdfImpl &B, int count, double *host)
+{
+    hipLaunchKernelGGL(k_reduce_partials, dim3(count), dim3(kBlock), 0, B.s, B.partials,
+                       kRedGrid, B.dred);
+    FSP_HIP(hipGetLastError());
+    FSP_HIP(hipMemcpyAsync(B.h_red, B.dred, sizeof(double) * (size_t)count,
+                           hipMemcpyDeviceToHost, B.s));
+    FSP_HIP(hipStreamSynchronize(B.s));
+    std::memcpy(host, B.h_red, sizeof(double) * (size_t)count);
+    return FSP_OK;
+}
+
+// weighted RMS norm sqrt(mean((x ewt)^2))
+fsp_status wrms(BdfImpl &B, const double *x, const double *w, double *out)
+{
+    hipLaunchKernelGGL(k_wsq_partials, dim3(kRedGrid), dim3(kBlock), 0, B.s, x, w, B.n,
+                       B.partials);
+    FSP_HIP(hipGetLastError());
+    double s2 = 0.0;
+    FSP_TRY(pull_reduced(B, 1, &s2));
+    *out = std::sqrt(s2 / (double)B.n);
+    return FSP_OK;
+}
+
+fsp_status push_small(BdfImpl &B, const double *host, int count)
+{
+    std::memcpy(B.h_red, host, sizeof(double) * (size_t)count);
+    FSP_HIP(hipMemcpyAsync(B.d_small, B.h_red, sizeof(double) * (size_t)count,
+                           hipMemcpyHostToDevice, B.s));
+    FSP_HIP(hipStreamSynchronize(B.s));
+    return FSP_OK;
+}
+
+// Solves (alpha0 I - A) y = b, y starting at the predictor (already in y),
+// by restarted GMRES on the ewt-scaled system; converged when the weighted
+// RMS of the scaled residual is <= tol.
+fsp_status gmres(BdfImpl &B, double alpha0, double tol, double *y, bool *converged)
+{
+    const int m = B.opt.krylov_dim;
+    const int64_t n = B.n;
+    const int G = vec_grid(n);
+    const double sqn = std::sqrt((double)n);
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hcol(m + 1), hc2(m + 1);
+    *converged = false;
+    for (int cycle = 0; cycle <= B.opt.max_restarts; ++cycle) {
+        // r0 = D (b - alpha0 y + A y)
+        FSP_TRY(mat_apply(*B.A, y, B.u, 0, B.s));
+        B.st.n_rhs++;
+        double *r = B.V;
+        hipLaunchKernelGGL(k_resid, dim3(G), dim3(kBlock), 0, B.s, B.b, y, B.u, B.ewt, alpha0, r, n);
+        FSP_HIP(hipGetLastError());
+        double beta;
+        FSP_TRY(wrms(B, r, nullptr, &beta));
+        beta *= sqn;  // Euclidean norm
+        if (beta / sqn <= tol) {
+            *converged = true;
+            return FSP_OK;
+        }
+        hipLaunchKernelGGL(k_scale_copy, dim3(G), dim3(kBlock), 0, B.s, r, 1.0 / beta, r, n);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int j = 0;
+        bool done = false;
+        for (; j < m && !done; ++j) {
+            double *vj = B.V + (int64_t)j * n, *w = B.V + (int64_t)(j + 1) * n;
+            hipLaunchKernelGGL(k_unscale, dim3(G), dim3(kBlock), 0, B.s, vj, B.ewt, B.tmp, n);
+            FSP_TRY(mat_apply(*B.A, B.tmp, B.u, 0, B.s));
+            B.st.n_rhs++;
+            hipLaunchKernelGGL(k_op_post, dim3(G), dim3(kBlock), 0, B.s, vj, B.ewt, B.u, alpha0, w, n);
+            FSP_HIP(hipGetLastError());
+            // classical Gram-Schmidt, two passes
+            std::fill(hcol.begin(), hcol.end(), 0.0);
+            double wn = 0.0;
+            for (int pass = 0; pass < 2; ++pass) {
+                hipLaunchKernelGGL(k_multi_dot, dim3(kRedGrid), dim3(kBlock), 0, B.s, B.V, n, j + 1,
+                                   w, n, B.partials);
+                FSP_HIP(hipGetLastError());
+                FSP_TRY(pull_reduced(B, j + 1, hc2.data()));
+                FSP_TRY(push_small(B, hc2.data(), j + 1));
+                hipLaunchKernelGGL(k_gs_update, dim3(kRedGrid), dim3(kBlock), 0, B.s, B.V, n, j + 1,
+                                   B.d_small, w, n, B.partials);
+                FSP_HIP(hipGetLastError());
+                double s2 = 0.0;
+                FSP_TRY(pull_reduced(B, 1, &s2));
+                wn = std::sqrt(s2);
+                for (int i = 0; i <= j; ++i) hcol[i] += hc2[i];
+            }
+            hcol[j + 1] = wn;
+            if (wn > 0.0)
+                hipLaunchKernelGGL(k_scale_copy, dim3(G), dim3(kBlock), 0, B.s, w, 1.0 / wn, w, n);
+            // Givens rotations
+            for (int i = 0; i < j; ++i) {
+                const double a = cs[i] * hcol[i] + sn[i] * hcol[i + 1];
+                hcol[i + 1] = -sn[i] * hcol[i] + cs[i] * hcol[i + 1];
+                hcol[i] = a;
+            }
+            const double rr = std::hypot(hcol[j], hcol[j + 1]);
+            cs[j] = rr > 0.0 ? hcol[j] / rr : 1.0;
+            sn[j] = rr > 0.0 ? hcol[j + 1] / rr : 0.0;
+            hcol[j] = rr;
+            hcol[j + 1] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            for (int i = 0; i <= j; ++i) H[(size_t)i + (size_t)j * (m + 1)] = hcol[i];
+            B.st.n_lin_iters++;
+            if (std::fabs(g[j + 1]) / sqn <= tol || wn == 0.0) done = true;
+        }
+        // back substitution H(0:j,0:j) z = g(0:j)
+        std::vector<double> z(j);
+        for (int i = j - 1; i >= 0; --i) {
+            double a = g[i];
+            for (int c = i + 1; c < j; ++c) a -= H[(size_t)i + (size_t)c * (m + 1)] * z[c];
+            z[i] = a / H[(size_t)i + (size_t)i * (m + 1)];
+        }
+        FSP_TRY(push_small(B, z.data(), j));
+        hipLaunchKernelGGL(k_sol_update, dim3(G), dim3(kBlock), 0, B.s, B.V, n, j, B.d_small,
+                           B.ewt, y, n);
+        FSP_HIP(hipGetLastError());
+        if (done) {
+            *converged = true;
+            return FSP_OK;
+        }
+    }
+    B.st.n_lin_fails++;
+    return FSP_OK;
+}
+
+fsp_status bdf_alloc(BdfImpl &B)
+{
+    const size_t n = (size_t)B.n, kd = (size_t)B.opt.krylov_dim + 1;
+    for (int j = 0; j < kHist; ++j) FSP_HIP(hipMalloc(&B.Y[j], sizeof(double) * n));
+    double **vs[] = {&B.ewt, &B.b, &B.yp, &B.ynew, &B.u, &B.tmp};
+    for (double **p : vs) FSP_HIP(hipMalloc(p, sizeof(double) * n));
+    FSP_HIP(hipMalloc(&B.V, sizeof(double) * n * kd));
+    FSP_HIP(hipMalloc(&B.partials, sizeof(double) * 2 * kd * kRedGrid));
+    FSP_HIP(hipMalloc(&B.dred, sizeof(double) * kd));
+    FSP_HIP(hipMalloc(&B.d_small, sizeof(double) * kd));
+    FSP_HIP(hipHostMalloc((void **)&B.h_red, sizeof(double) * kd, hipHostMallocDefault));
+    return FSP_OK;
+}
+
+void bdf_free(BdfImpl &B)
+{
+    for (int j = 0; j < kHist; ++j)
+        if (B.Y[j]) (void)hipFree(B.Y[j]);
+    double *vs[] = {B.ewt, B.b, B.yp, B.ynew, B.u, B.tmp, B.V, B.partials, B.dred, B.d_small};
+    for (double *p : vs)
+        if (p) (void)hipFree(p);
+    if (B.h_red) (void)hipHostFree(B.h_red);
+}
+
+// One attempted step of order k and size h from the newest history point.
+// On return *accepted tells whether the error test passed; err is the
+// weighted error estimate (<= 1 passes).
+fsp_status try_step(BdfImpl &B, double h, int k, bool *accepted, double *err)
+{
+    const double t_new = B.T[0] + h;
+    double x[kHist + 1], w[kHist + 1];
+    const double *v[kMaxLc];
+    // corrector: alpha_0 y + sum_{j=1..k} alpha_j Y[j-1] = A y
+    x[0] = t_new;
+    for (int j = 1; j <= k; ++j) x[j] = B.T[j - 1];
+    lagrange_deriv_at_first(x, k, w);
+    const double alpha0 = w[0];
+    double c[kMaxLc];
+    for (int j = 1; j <= k; ++j) {
+        v[j - 1] = B.Y[j - 1];
+        c[j - 1] = -w[j];
+    }
+    FSP_TRY(lincomb(B, k, v, c, B.b));
+    // predictor through the newest k+1 points (k points + derivative at startup)
+    double ratio;  // predictor error / corrector error factor inverse, r in r/(1+r)
+    if (B.nh >= k + 1) {
+        for (int j = 0; j <= k; ++j) x[j] = B.T[j];
+        lagrange_value(x, k, t_new, w);
+        for (int j = 0; j <= k; ++j) {
+            v[j] = B.Y[j];
+            c[j] = w[j];
+        }
+        FSP_TRY(lincomb(B, k + 1, v, c, B.yp));
+        ratio = 1.0 / (alpha0 * (t_new - B.T[k]));
+    } else {
+        // first step (k = 1): explicit Euler predictor y0 + h A(t0) y0
+        FSP_TRY(set_time(B, B.T[0]));
+        FSP_TRY(mat_apply(*B.A, B.Y[0], B.u, 0, B.s));
+        B.st.n_rhs++;
+        v[0] = B.Y[0];
+        v[1] = B.u;
+        c[0] = 1.0;
+        c[1] = h;
+        FSP_TRY(lincomb(B, 2, v, c, B.yp));
+        ratio = 1.0;  // Taylor predictor error h^2 y''/2, BDF1 error h^2 y''/2
+    }
+    // solve the stage from the predictor
+    FSP_HIP(hipMemcpyAsync(B.ynew, B.yp, sizeof(double) * (size_t)B.n, hipMemcpyDeviceToDevice, B.s));
+    FSP_TRY(set_time(B, t_new));
+    bool conv = false;
+    FSP_TRY(gmres(B, alpha0, B.opt.lin_tol_factor * std::fabs(alpha0), B.ynew, &conv));
+    if (!conv) {
+        *accepted = false;
+        *err = 1e10;
+        return FSP_OK;
+    }
+    // error estimate
+    v[0] = B.ynew;
+    v[1] = B.yp;
+    c[0] = 1.0;
+    c[1] = -1.0;
+    FSP_TRY(lincomb(B, 2, v, c, B.tmp));
+    double dn;
+    FSP_TRY(wrms(B, B.tmp, B.ewt, &dn));
+    *err = dn * ratio / (1.0 + ratio);
+    *accepted = *err <= 1.0;
+    return FSP_OK;
+}
+
+// weighted norm of the divided difference of order q over (t_new, T[0..q-1])
+// times q!: an estimate of |y^(q)|
+fsp_status deriv_norm(BdfImpl &B, const double *ynew, double t_new, int q, double *out)
+{
+    double x[kHist + 1], c[kMaxLc];
+    const double *v[kMaxLc];
+    x[0] = t_new;
+    v[0] = ynew;
+    for (int j = 1; j <= q; ++j) {
+        x[j] = B.T[j - 1];
+        v[j] = B.Y[j - 1];
+    }
+    double fact = 1.0;
+    for (int j = 2; j <= q; ++j) fact *= j;
+    for (int j = 0; j <= q; ++j) {
+        double den = 1.0;
+        for (int i = 0; i <= q; ++i)
+            if (i != j) den *= (x[j] - x[i]);
+        c[j] = fact / den;
+    }
+    FSP_TRY(lincomb(B, q + 1, v, c, B.tmp));
+    return wrms(B, B.tmp, B.ewt, out);
+}
+
+// BDF-q local error for a constant step h: the residual error is
+// -h^(q+1) y^(q+1) / (q+1) and the stage divides it by a_0 = sum_{j<=q} 1/j,
+// so |LTE| = h^(q+1) |y^(q+1)| / ((q+1) a_0).
+double lte_const(int q)
+{
+    double a0 = 0.0;
+    for (int j = 1; j <= q; ++j) a0 += 1.0 / j;
+    return 1.0 / ((q + 1) * a0);
+}
+
+fsp_status bdf_advance(BdfImpl &B, double *d_v, double t_out)
+{
+    if (!B.started) {
+        FSP_HIP(hipMemcpyAsync(B.Y[0], d_v, sizeof(double) * (size_t)B.n, hipMemcpyDeviceToDevice, B.s));
+        B.T[0] = B.t;
+        B.nh = 1;
+        B.k = 1;
+        B.steps_at_k = 0;
+        B.started = true;
+        // initial step: 0.5 / |A(t0) y0|_w, bounded by the interval
+        FSP_HIP(hipStreamSynchronize(B.s));
+        hipLaunchKernelGGL(k_ewt, dim3(vec_grid(B.n)), dim3(kBlock), 0, B.s, B.Y[0], B.ewt, B.n,
+                           B.opt.rtol, B.opt.atol);
+        FSP_TRY(set_time(B, B.t));
+        FSP_TRY(mat_apply(*B.A, B.Y[0], B.u, 0, B.s));
+        B.st.n_rhs++;
+        double fn;
+        FSP_TRY(wrms(B, B.u, B.ewt, &fn));
+        double span = std::fabs(t_out - B.t);
+        B.h = B.opt.h0 > 0.0 ? B.opt.h0 : (fn > 0.0 ? std::min(0.5 / fn, 0.1 * span) : 0.1 * span);
+        if (!(B.h > 0.0)) B.h = 1e-6;
+    }
+    while (B.T[0] < t_out) {
+        if (B.st.n_steps >= B.opt.max_steps)
+            return fail(FSP_ERR_NOT_CONVERGED, "BDF: max_steps reached");
+        double h = B.h;
+        const double rem = t_out - B.T[0];
+        const bool clipped = h >= rem || rem - h < 0.05 * h;
+        if (clipped) h = rem;
+        hipLaunchKernelGGL(k_ewt, dim3(vec_grid(B.n)), dim3(kBlock), 0, B.s, B.Y[0], B.ewt, B.n,
+                           B.opt.rtol, B.opt.atol);
+        // order k needs k+1 accepted points for its predictor (the first
+        // step, with one point, uses an explicit Euler predictor)
+        const int k = std::max(1, std::min(B.k, B.nh - 1));
+        bool acc = false;
+        double err = 0.0;
+        FSP_TRY(try_step(B, h, k, &acc, &err));
+        if (!acc) {
+            B.st.n_rejects++;
+            B.fails++;
+            const double r = err >= 1e10 ? 0.25 : std::max(0.2, 0.9 * std::pow(err, -1.0 / (k + 1)));
+            B.h = h * std::min(r, 0.9);
+            if (B.fails >= 3) B.k = 1;
+            if (B.h < 1e-14 * std::max(1.0, std::fabs(B.T[0])))
+                return fail(FSP_ERR_NOT_CONVERGED, "BDF: step size underflow");
+            continue;
+        }
+        B.fails = 0;
+        const double t_new = B.T[0] + h;
+        // order selection: errors at k-1, k, k+1 from divided differences
+        double best_r = std::pow(0.9 / std::max(err, 1e-10), 1.0 / (k + 1));
+        int best_q = k;
+        B.steps_at_k++;
+        if (B.steps_at_k > k) {
+            // order k-1 from |y^(k)| over (t_new, T[0..k-1])
+            if (k > 1) {
+                double dq;
+                FSP_TRY(deriv_norm(B, B.ynew, t_new, k, &dq));
+                const double e = lte_const(k - 1) * std::pow(h, k) * dq;
+                const double r = std::pow(0.9 / std::max(e, 1e-10), 1.0 / k);
+                if (r > best_r) {
+                    best_r = r;
+                    best_q = k - 1;
+                }
+            }
+            // order k+1 from |y^(k+2)| over (t_new, T[0..k+1])
+            if (k < B.opt.max_order && B.nh >= k + 2) {
+                double dq;
+                FSP_TRY(deriv_norm(B, B.ynew, t_new, k + 2, &dq));
+                const double e = lte_const(k + 1) * std::pow(h, k + 2) * dq;
+                const double r = std::pow(0.9 / std::max(e, 1e-10), 1.0 / (k + 2));
+                if (r > 1.1 * best_r) {
+                    best_r = r;
+                    best_q = k + 1;
+                }
+            }
+        }
+        // accept: shift history
+        double *old = B.Y[kHist - 1];
+        for (int j = kHist - 1; j > 0; --j) {
+            B.Y[j] = B.Y[j - 1];
+            B.T[j] = B.T[j - 1];
+        }
+        B.Y[0] = B.ynew;
+        B.ynew = old;
+        B.T[0] = t_new;
+        B.nh = std::min(B.nh + 1, kHist);
+        B.st.n_steps++;
+        if (best_q != B.k) {
+            B.k = best_q;
+            B.steps_at_k = 0;
+        }
+        double r = std::min(2.0, best_r);
+        if (r < 1.2 && r > 0.9) r = 1.0;
+        const double h_nominal = B.h;
+        B.h = h * std::max(0.2, r);
+        // a step shortened to land on t_out does not shrink the next one
+        if (clipped) B.h = std::max(B.h, std::min(h_nominal, 2.0 * B.h));
+    }
+    B.t = B.T[0];
+    B.st.order = B.k;
+    B.st.h_last = B.h;
+    B.st.t = B.t;
+    FSP_HIP(hipMemcpyAsync(d_v, B.Y[0], sizeof(double) * (size_t)B.n, hipMemcpyDeviceToDevice, B.s));
+    FSP_HIP(hipStreamSynchronize(B.s));
+    return FSP_OK;
+}
+
+}  // namespace
+}  // namespace fsp
+
+// ===================================================================== C ABI
+using fsp::BdfImpl;
+using fsp::MatImpl;
+
+extern "C" fsp_status fsp_mat_internal(fsp_mat A, MatImpl **impl);
+
+struct fsp_bdf_s {
+    BdfImpl impl;
+};
+
+extern "C" fsp_status fsp_rhs_apply(fsp_mat A, double t, fsp_tcoef_fn coef, void *ctx,
+                                    const double *d_u, double *d_udot, void *stream)
+{
+    using namespace fsp;
+    if (!A || !coef || !d_u || !d_udot) return fail(FSP_ERR_INVALID, "NULL argument");
+    MatImpl *M = nullptr;
+    FSP_TRY(fsp_mat_internal(A, &M));
+    std::vector<double> c((size_t)M->hm.dm.R);
+    coef(t, c.data(), ctx);
+    FSP_TRY(mat_revalue(*M, c.data(), (hipStream_t)stream));
+    return mat_apply(*M, d_u, d_udot, 0, (hipStream_t)stream);
+}
+
+extern "C" fsp_status fsp_bdf_create(fsp_mat A, fsp_tcoef_fn coef, void *ctx, double t0,
+                                     const fsp_bdf_options *opt, void *stream, fsp_bdf *out)
+{
+    using namespace fsp;
+    if (!out) return fail(FSP_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!A || !coef) return fail(FSP_ERR_INVALID, "NULL argument");
+    auto *h = new fsp_bdf_s();
+    BdfImpl &B = h->impl;
+    fsp_status st = fsp_mat_internal(A, &B.A);
+    if (st != FSP_OK) {
+        delete h;
+        return st;
+    }
+    B.coef = coef;
+    B.ctx = ctx;
+    B.s = (hipStream_t)stream;
+    B.n = B.A->n_rows;
+    B.t = t0;
+    fsp_bdf_options d;
+    d.rtol = 1e-6;
+    d.atol = 1e-12;
+    d.max_order = 5;
+    d.krylov_dim = 30;
+    d.max_restarts = 5;
+    d.h0 = 0.0;
+    d.max_steps = 500000;
+    d.lin_tol_factor = 0.05;
+    if (opt) {
+        if (opt->rtol > 0) d.rtol = opt->rtol;
+        if (opt->atol > 0) d.atol = opt->atol;
+        if (opt->max_order > 0) d.max_order = std::min(opt->max_order, kMaxOrder);
+        if (opt->krylov_dim > 0) d.krylov_dim = std::min(opt->krylov_dim, 200);
+        if (opt->max_restarts >= 0) d.max_restarts = opt->max_restarts;
+        if (opt->h0 > 0) d.h0 = opt->h0;
+        if (opt->max_steps > 0) d.max_steps = opt->max_steps;
+        if (opt->lin_tol_factor > 0) d.lin_tol_factor = opt->lin_tol_factor;
+    }
+    B.opt = d;
+    B.coefs.assign((size_t)B.A->hm.dm.R, 0.0);
+    std::memset(&B.st, 0, sizeof(B.st));
+    st = bdf_alloc(B);
+    if (st != FSP_OK) {
+        bdf_free(B);
+        delete h;
+        return st;
+    }
+    *out = h;
+    return FSP_OK;
+}
+
+extern "C" fsp_status fsp_bdf_destroy(fsp_bdf b)
+{
+    if (!b) return FSP_OK;
+    fsp::bdf_free(b->impl);
+    delete b;
+    return FSP_OK;
+}
+
+extern "C" fsp_status fsp_bdf_reset(fsp_bdf b, double t0)
+{
+    using namespace fsp;
+    if (!b) return fail(FSP_ERR_INVALID, "NULL handle");
+    b->impl.t = t0;
+    b->impl.started = false;
+    b->impl.nh = 0;
+    return FSP_OK;
+}
+
+extern "C" fsp_status fsp_bdf_advance(fsp_bdf b, double *d_v, double t_out)
+{
+    using namespace fsp;
+    if (!b || !d_v) return fail(FSP_ERR_INVALID, "NULL argument");
+    BdfImpl &B = b->impl;
+    if (t_out < B.t) return fail(FSP_ERR_INVALID, "t_out is before the current time");
+    if (B.started && t_out == B.T[0]) {
+        FSP_HIP(hipMemcpyAsync(d_v, B.Y[0], sizeof(double) * (size_t)B.n, hipMemcpyDeviceToDevice, B.s));
+        FSP_HIP(hipStreamSynchronize(B.s));
+        return FSP_OK;
+    }
+    return bdf_advance(B, d_v, t_out);
+}
+
+extern "C" fsp_status fsp_bdf_get_stats(fsp_bdf b, fsp_bdf_stats *st)
+{
+    using namespace fsp;
+    if (!b || !st) return fail(FSP_ERR_INVALID, "NULL argument");
+    *st = b->impl.st;
+    return FSP_OK;
+}

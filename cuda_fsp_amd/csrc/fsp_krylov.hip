@@ -117,6 +117,9 @@ struct ArnArgs {
 };
 
 constexpr int kRedLen = 2 * kBlock / kWave;
+// LDS of the Arnoldi kernels' end reduction: one dd per (dot, wave), for up
+// to m + 1 dots plus the sink sum.
+constexpr int kArnRedLen = 2 * (kMaxKrylov + 2) * (kBlock / kWave);
 
 template <typename B>
 __device__ inline bool arn_skip(const ArnArgs &a)
@@ -152,25 +155,38 @@ __device__ inline void arn_epi(const ArnArgs &a, uint32_t i, double y, double xi
 }
 
 // Partials layout: dd pairs, dot k of block b at 2 * (k * G + b); the sink
-// sum after the nk dots.
+// sum after the nk dots.  One barrier for all dots: every wave reduces its
+// accumulators by butterflies and lane 0 stores them (red: kArnRedLen
+// doubles), then thread k merges the waves' sums of dot k in wave order.
 template <int NKB>
 __device__ inline void arn_end(const ArnArgs &a, dd (&acc)[NKB], dd sink, double *red)
 {
+    constexpr int NW = kBlock / kWave;
     const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
 #pragma unroll
     for (int kk = 0; kk < NKB; ++kk) {
         if (kk < nk) {
-            const dd t = block_sum_dd(acc[kk], red);
-            if (threadIdx.x == 0) {
-                a.partials[2 * ((size_t)kk * a.G + blockIdx.x)] = t.s;
-                a.partials[2 * ((size_t)kk * a.G + blockIdx.x) + 1] = t.c;
+            const dd t = wave_sum_dd(acc[kk]);
+            if (l == 0) {
+                red[2 * (kk * NW + w)] = t.s;
+                red[2 * (kk * NW + w) + 1] = t.c;
             }
         }
     }
-    const dd t = block_sum_dd(sink, red);
-    if (threadIdx.x == 0) {
-        a.partials[2 * ((size_t)nk * a.G + blockIdx.x)] = t.s;
-        a.partials[2 * ((size_t)nk * a.G + blockIdx.x) + 1] = t.c;
+    const dd ts = wave_sum_dd(sink);
+    if (l == 0) {
+        red[2 * (nk * NW + w)] = ts.s;
+        red[2 * (nk * NW + w) + 1] = ts.c;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x <= nk) {
+        const int kk = threadIdx.x;
+        dd s = dd_zero();
+#pragma unroll
+        for (int q = 0; q < NW; ++q) s = dd_merge(s, dd{red[2 * (kk * NW + q)], red[2 * (kk * NW + q) + 1]});
+        a.partials[2 * ((size_t)kk * a.G + blockIdx.x)] = s.s;
+        a.partials[2 * ((size_t)kk * a.G + blockIdx.x) + 1] = s.c;
     }
 }
 
@@ -187,7 +203,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_csr(CsrF<V> A, ArnArgs a)
     if (arn_skip<B>(a)) return;
     extern __shared__ double smem[];
     double *red = smem;
-    double *prod = red + kRedLen;
+    double *prod = red + kArnRedLen;
     dd acc[NKB];
     acc_zero(acc);
     const KryX<B> X{a.in, a.st->sigma[a.j]};
@@ -224,7 +240,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_kron(KronF<V> A, ArnArgs a, int 
     if (arn_skip<B>(a)) return;
     extern __shared__ double smem[];
     double *red = smem;
-    V *tabl = (V *)(red + kRedLen);
+    V *tabl = (V *)(red + kArnRedLen);
     const V *tab = A.tab_g;
     if (use_lds && A.tab_g) {
         stage_table(tabl, A.tab_g, A.tab_size);
@@ -479,7 +495,7 @@ template <typename B, int NKB>
 __global__ void __launch_bounds__(kBlock) k_cb_dots(ArnArgs a)
 {
     if (arn_skip<B>(a)) return;
-    __shared__ double red[kRedLen];
+    __shared__ double red[kArnRedLen];
     dd acc[NKB];
     acc_zero(acc);
     const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
@@ -558,7 +574,7 @@ template <typename B, int NKB>
 static fsp_status launch_arn_spmv_nk(ExpvImpl &E, ArnArgs &a)
 {
     MatImpl &M = *E.A;
-    const size_t red_b = sizeof(double) * kRedLen;
+    const size_t red_b = sizeof(double) * kArnRedLen;
     const bool mixed = M.prec == FSP_MIXED;
     if (M.format == FSP_CSR) {
         if (mixed) {

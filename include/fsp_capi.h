@@ -237,6 +237,47 @@ fsp_status fsp_batch_get_config(fsp_batch b, int *block, int *wg_per_cu);
  * cache); a cached entry for the same device / shape is reused unless force. */
 fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, int force, double *best_ms);
 
+/* ---- time-varying right-hand side and BDF integration ---- */
+/* Host time function: writes the n_reactions coefficients c_r(t) (the
+ * reference's TcoefFun, src/FSPMat.h:16, with a context pointer). */
+typedef void (*fsp_tcoef_fn)(double t, double *h_coefs, void *ctx);
+
+/* udot = A(t) u with c_r = coef(t): the body of the reference's CVODE RHS
+ * and Jacobian-times-vector callbacks (tests/test_sundials.cu:67-91, which
+ * zero-fill udot and call FSPMat::action).  Overwrites d_udot. */
+fsp_status fsp_rhs_apply(fsp_mat A, double t, fsp_tcoef_fn coef, void *ctx,
+                         const double *d_u, double *d_udot, void *stream);
+
+typedef struct fsp_bdf_s *fsp_bdf;
+typedef struct fsp_bdf_options {
+    double rtol, atol;     /* error weights 1/(rtol |y| + atol) (1e-6, 1e-12)   */
+    int max_order;         /* 1..5 (5)                                          */
+    int krylov_dim;        /* GMRES subspace (30)                               */
+    int max_restarts;      /* GMRES restarts per stage (5; negative: default)   */
+    double h0;             /* first step (0: from |A(t0) y0|)                   */
+    int64_t max_steps;     /* (500000)                                          */
+    double lin_tol_factor; /* stage residual tolerance / alpha_0 (0.05)         */
+} fsp_bdf_options;
+typedef struct fsp_bdf_stats {
+    int64_t n_steps, n_rejects, n_rhs, n_lin_iters, n_lin_fails;
+    int order;
+    double h_last, t;
+} fsp_bdf_stats;
+/* dp/dt = A(t) p integrated by a variable-order (1..5), variable-step BDF
+ * with GMRES stages: the role of CVODE's BDF + SPBCGS with the RHS above in
+ * tests/test_sundials.cu:131-186 (CVODE itself is not used).  The matrix A
+ * is revalued at each stage time; the caller keeps A alive. */
+fsp_status fsp_bdf_create(fsp_mat A, fsp_tcoef_fn coef, void *ctx, double t0,
+                          const fsp_bdf_options *options, void *stream, fsp_bdf *out);
+fsp_status fsp_bdf_destroy(fsp_bdf b);
+/* Integrates to t_out (CVode with CV_NORMAL, landing on t_out).  The first
+ * call after create / reset reads the initial vector from d_v; every call
+ * writes p(t_out) to d_v and continues from its own history. */
+fsp_status fsp_bdf_advance(fsp_bdf b, double *d_v, double t_out);
+/* Restart at t0: the next advance reads a new initial vector. */
+fsp_status fsp_bdf_reset(fsp_bdf b, double t0);
+fsp_status fsp_bdf_get_stats(fsp_bdf b, fsp_bdf_stats *stats);
+
 /* ---- likelihood: src/Likelihood.cpp:8-33 ---- */
 /* sum_k log(max(floor, p[idx[k]])) over device index lists; idx < 0 (a state
  * outside the box) counts as probability 0.  Deterministic reduction. */
