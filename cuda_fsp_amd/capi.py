@@ -69,6 +69,16 @@ class FspBdfOptions(C.Structure):
                 ("max_steps", C.c_int64), ("lin_tol_factor", C.c_double)]
 
 
+class FspLikData(C.Structure):
+    _fields_ = [("n_times", C.c_int), ("times", C.c_void_p), ("keep_mask", C.c_void_p),
+                ("counts", C.c_void_p), ("floor_val", C.c_double)]
+
+
+class FspGradOptions(C.Structure):
+    _fields_ = [("format", C.c_int), ("m", C.c_int), ("tol", C.c_double),
+                ("quad_nodes", C.c_int), ("grading_levels", C.c_int)]
+
+
 class FspBdfStats(C.Structure):
     _fields_ = [("n_steps", C.c_int64), ("n_rejects", C.c_int64), ("n_rhs", C.c_int64),
                 ("n_lin_iters", C.c_int64), ("n_lin_fails", C.c_int64), ("order", C.c_int),
@@ -128,6 +138,8 @@ _SIGS = {
     "fsp_loglik_weighted": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
     "fsp_expm_dense": (C.c_int, [_i, _vp, _vp, _vp]),
     "fsp_rhs_apply": (C.c_int, [_vp, _d, _vp, _vp, _vp, _vp, _vp]),
+    "fsp_loglik": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fsp_loglik_grad": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fsp_bdf_create": (C.c_int, [_vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "fsp_bdf_destroy": (C.c_int, [_vp]),
     "fsp_bdf_advance": (C.c_int, [_vp, _vp, _d]),
@@ -380,6 +392,41 @@ def loglik_weighted(q, w, count, floor_val=1e-12, stream=None) -> float:
 
 def expm_dense(A, F, N, stream=None):
     check(lib().fsp_expm_dense(int(N), ptr(A), ptr(F), stream))
+
+
+def loglik(net, log_coefs, times, keep_mask, counts, floor_val=1e-12, p0=None, fmt="csr",
+           m=30, tol=1e-10, quad_nodes=8, grading_levels=10, grad=False, stream=None):
+    """Histogram log-likelihood of the FSP solution (fsp_loglik); with
+    grad=True also dLL/dlog_coefs by the adjoint method (fsp_loglik_grad).
+    counts: [n_times, n_bins] over the kept species (first fastest)."""
+    model = Model(net)
+    th = np.ascontiguousarray(log_coefs, np.float64)
+    tt = np.ascontiguousarray(times, np.float64)
+    km = np.ascontiguousarray([1 if k else 0 for k in keep_mask], np.int32)
+    cn = np.ascontiguousarray(counts, np.float64)
+    data = FspLikData(len(tt), tt.ctypes.data, km.ctypes.data, cn.ctypes.data, float(floor_val))
+    opt = FspGradOptions(FORMATS[fmt], int(m), float(tol), int(quad_nodes), int(grading_levels))
+    own = p0 is None
+    pbuf = DeviceBuffer.from_host(_delta0(net.n_states)) if own else p0
+    ll = C.c_double(0.0)
+    try:
+        if grad:
+            g = np.zeros(net.n_reactions, np.float64)
+            check(lib().fsp_loglik_grad(model.ref, th.ctypes.data, ptr(pbuf), C.byref(data),
+                                        C.byref(opt), C.byref(ll), g.ctypes.data, stream))
+            return ll.value, g
+        check(lib().fsp_loglik(model.ref, th.ctypes.data, ptr(pbuf), C.byref(data), C.byref(opt),
+                               C.byref(ll), stream))
+        return ll.value
+    finally:
+        if own:
+            pbuf.free()
+
+
+def _delta0(n):
+    p = np.zeros(n, np.float64)
+    p[0] = 1.0
+    return p
 
 
 def _tcoef_callback(coef_fn, R):

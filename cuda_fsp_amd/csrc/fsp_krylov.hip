@@ -120,6 +120,7 @@ constexpr int kRedLen = 2 * kBlock / kWave;
 // LDS of the Arnoldi kernels' end reduction: one dd per (dot, wave), for up
 // to m + 1 dots plus the sink sum.
 constexpr int kArnRedLen = 2 * (kMaxKrylov + 2) * (kBlock / kWave);
+constexpr int kDotChunk = 8;
 
 template <typename B>
 __device__ inline bool arn_skip(const ArnArgs &a)
@@ -130,55 +131,25 @@ __device__ inline bool arn_skip(const ArnArgs &a)
     return false;
 }
 
-// Row epilogue: store V_j[i]; w_{j+1}[i] = y; the thread's dd partial dot
-// products y * V_k[i] (k = istart..j) in registers (NKB >= number of dots,
-// a compile-time bucket so the accumulators stay in VGPRs).
-template <typename B, int NKB>
-__device__ inline void arn_epi(const ArnArgs &a, uint32_t i, double y, double xi, dd (&acc)[NKB])
+// Phase 1 row epilogue: store V_j[i] and w_{j+1}[i] = y (mode 0), or
+// accumulate |y|^2 (mode 1: the avnorm product A V_m).
+template <typename B>
+__device__ inline void arn_epi(const ArnArgs &a, uint32_t i, double y, double xi, dd &nrm)
 {
     if (i >= a.n) return;
     store_basis<B>(a.V64, (B *)a.Vb, a.nv, a.j, i, xi);
-    if (a.mode == 0) {
+    if (a.mode == 0)
         a.out[i] = y;
-        const int nk = a.j - a.istart + 1;
-#pragma unroll
-        for (int kk = 0; kk < NKB; ++kk) {
-            if (kk < nk) {
-                const int k = a.istart + kk;
-                const double vk = (k == a.j) ? xi : basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i);
-                acc[kk] = dd_add_prod(acc[kk], y, vk);
-            }
-        }
-    } else {
-        acc[0] = dd_add_prod(acc[0], y, y);
-    }
+    else
+        nrm = dd_add_prod(nrm, y, y);
 }
 
-// Partials layout: dd pairs, dot k of block b at 2 * (k * G + b); the sink
-// sum after the nk dots.  One barrier for all dots: every wave reduces its
-// accumulators by butterflies and lane 0 stores them (red: kArnRedLen
-// doubles), then thread k merges the waves' sums of dot k in wave order.
-template <int NKB>
-__device__ inline void arn_end(const ArnArgs &a, dd (&acc)[NKB], dd sink, double *red)
+// Merges the per-wave dd sums stored in red (dot k of wave w at 2 (k NW + w))
+// in wave order and writes the block's partials: dot k of block b at
+// 2 (k G + b), the sink sum after the nk dots.
+__device__ inline void arn_store_partials(const ArnArgs &a, int nk, const double *red)
 {
     constexpr int NW = kBlock / kWave;
-    const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
-    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
-#pragma unroll
-    for (int kk = 0; kk < NKB; ++kk) {
-        if (kk < nk) {
-            const dd t = wave_sum_dd(acc[kk]);
-            if (l == 0) {
-                red[2 * (kk * NW + w)] = t.s;
-                red[2 * (kk * NW + w) + 1] = t.c;
-            }
-        }
-    }
-    const dd ts = wave_sum_dd(sink);
-    if (l == 0) {
-        red[2 * (nk * NW + w)] = ts.s;
-        red[2 * (nk * NW + w) + 1] = ts.c;
-    }
     __syncthreads();
     if ((int)threadIdx.x <= nk) {
         const int kk = threadIdx.x;
@@ -190,6 +161,69 @@ __device__ inline void arn_end(const ArnArgs &a, dd (&acc)[NKB], dd sink, double
     }
 }
 
+__device__ inline void arn_put_wave(double *red, int k, dd v)
+{
+    constexpr int NW = kBlock / kWave;
+    const dd t = wave_sum_dd(v);
+    if (threadIdx.x % kWave == 0) {
+        const int w = threadIdx.x / kWave;
+        red[2 * (k * NW + w)] = t.s;
+        red[2 * (k * NW + w) + 1] = t.c;
+    }
+}
+
+// Phase 2 and the block reduction.  Mode 0: the dots V_k . w_{j+1}
+// (k = istart..j) over this block's rows, kDotChunk dots at a time: each
+// thread re-reads its own w_{j+1}[i] (written in phase 1, so cached) and a
+// chunk of independent basis loads (column index clamped to j, always
+// valid), so few registers are live and a chunk costs one memory round trip.
+// Every wave reduces each dot by butterflies into red (kArnRedLen doubles);
+// one barrier, then the block partials.
+template <typename B>
+__device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red)
+{
+    const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
+    if (a.mode == 0) {
+        for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
+            dd acc[kDotChunk];
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q) acc[q] = dd_zero();
+            for (TileWalk t = tile_walk(a.tiles); t.t < t.end; t.t += t.step) {
+                const uint32_t i = t.t * kBlock + threadIdx.x;
+                if (i >= a.n) continue;
+                const double y = a.out[i];
+                double vk[kDotChunk];
+#pragma unroll
+                for (int q = 0; q < kDotChunk; ++q)
+                    vk[q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + c0 + q, a.j), i);
+#pragma unroll
+                for (int q = 0; q < kDotChunk; ++q)
+                    if (c0 + q < nk) acc[q] = dd_add_prod(acc[q], y, vk[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q)
+                if (c0 + q < nk) arn_put_wave(red, c0 + q, acc[q]);
+        }
+    } else {
+        arn_put_wave(red, 0, nrm);
+    }
+    arn_put_wave(red, nk, sink);
+    arn_store_partials(a, nk, red);
+}
+
+// Register-accumulator variant for the callback path (k_cb_dots), which
+// reads rows it did not write: NKB >= number of dots.
+template <int NKB>
+__device__ inline void arn_end(const ArnArgs &a, dd (&acc)[NKB], dd sink, double *red)
+{
+    const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
+#pragma unroll
+    for (int kk = 0; kk < NKB; ++kk)
+        if (kk < nk) arn_put_wave(red, kk, acc[kk]);
+    arn_put_wave(red, nk, sink);
+    arn_store_partials(a, nk, red);
+}
+
 template <int NKB>
 __device__ inline void acc_zero(dd (&acc)[NKB])
 {
@@ -197,44 +231,42 @@ __device__ inline void acc_zero(dd (&acc)[NKB])
     for (int kk = 0; kk < NKB; ++kk) acc[kk] = dd_zero();
 }
 
-template <typename V, typename B, int NKB>
+template <typename V, typename B>
 __global__ void __launch_bounds__(kBlock) k_arn_csr(CsrF<V> A, ArnArgs a)
 {
     if (arn_skip<B>(a)) return;
     extern __shared__ double smem[];
     double *red = smem;
     double *prod = red + kArnRedLen;
-    dd acc[NKB];
-    acc_zero(acc);
+    dd nrm = dd_zero();
     const KryX<B> X{a.in, a.st->sigma[a.j]};
     dd sink = dd_zero();
     for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
         const double y = csr_tile<V>(A, w.t, X, prod, sink);
         const uint32_t i = w.t * kBlock + threadIdx.x;
-        arn_epi<B, NKB>(a, i, y, i < a.n ? X(i) : 0.0, acc);
+        arn_epi<B>(a, i, y, i < a.n ? X(i) : 0.0, nrm);
     }
-    arn_end<NKB>(a, acc, sink, red);
+    arn_finish<B>(a, nrm, sink, red);
 }
 
-template <typename V, typename B, int NKB>
+template <typename V, typename B>
 __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
 {
     if (arn_skip<B>(a)) return;
     extern __shared__ double smem[];
     double *red = smem;
-    dd acc[NKB];
-    acc_zero(acc);
+    dd nrm = dd_zero();
     const KryX<B> X{a.in, a.st->sigma[a.j]};
     dd sink = dd_zero();
     for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
         const uint32_t i = w.t * kBlock + threadIdx.x;
         const double y = hyb_row<V>(A, i, X, sink);
-        arn_epi<B, NKB>(a, i, y, i < a.n ? X(i) : 0.0, acc);
+        arn_epi<B>(a, i, y, i < a.n ? X(i) : 0.0, nrm);
     }
-    arn_end<NKB>(a, acc, sink, red);
+    arn_finish<B>(a, nrm, sink, red);
 }
 
-template <int D, typename V, typename B, int NKB>
+template <int D, typename V, typename B>
 __global__ void __launch_bounds__(kBlock) k_arn_kron(KronF<V> A, ArnArgs a, int use_lds)
 {
     if (arn_skip<B>(a)) return;
@@ -246,16 +278,15 @@ __global__ void __launch_bounds__(kBlock) k_arn_kron(KronF<V> A, ArnArgs a, int 
         stage_table(tabl, A.tab_g, A.tab_size);
         tab = tabl;
     }
-    dd acc[NKB];
-    acc_zero(acc);
+    dd nrm = dd_zero();
     const KryX<B> X{a.in, a.st->sigma[a.j]};
     dd sink = dd_zero();
     for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
         const uint32_t i = w.t * kBlock + threadIdx.x;
         const double y = kron_row<D, V>(A, i, X, tab, sink);
-        arn_epi<B, NKB>(a, i, y, i < a.n ? X(i) : 0.0, acc);
+        arn_epi<B>(a, i, y, i < a.n ? X(i) : 0.0, nrm);
     }
-    arn_end<NKB>(a, acc, sink, red);
+    arn_finish<B>(a, nrm, sink, red);
 }
 
 #define FSP_SWITCH_NK(nk, CALL)                                                \
@@ -532,8 +563,10 @@ static fsp_status expv_alloc(ExpvImpl &E)
     FSP_HIP(hipMalloc(&E.W[0], sizeof(double) * nv));
     FSP_HIP(hipMalloc(&E.W[1], sizeof(double) * nv));
     E.tiles = ((int64_t)nv + kBlock - 1) / kBlock;
-    // row-kernel grid: FSP_ARN_GRID overrides the default cap (timing sweeps)
-    int cap = 2048;
+    // row-kernel grid: 4 workgroups per CU measured fastest on the 786K-state
+    // solve (1.70 s vs 1.93 s at 2048 and 2.07 s at 512); FSP_ARN_GRID
+    // overrides the cap for timing sweeps
+    int cap = 1024;
     if (const char *g = std::getenv("FSP_ARN_GRID")) cap = std::max(8, std::atoi(g));
     E.G = grid_for(E.tiles, cap);
     FSP_HIP(hipMalloc(&E.partials, 2 * sizeof(double) * (size_t)E.G * (kMaxKrylov + 4)));
@@ -570,8 +603,8 @@ static void expv_free(ExpvImpl &E)
     if (E.h_st) hipHostFree(E.h_st);
 }
 
-template <typename B, int NKB>
-static fsp_status launch_arn_spmv_nk(ExpvImpl &E, ArnArgs &a)
+template <typename B>
+static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
 {
     MatImpl &M = *E.A;
     const size_t red_b = sizeof(double) * kArnRedLen;
@@ -579,40 +612,33 @@ static fsp_status launch_arn_spmv_nk(ExpvImpl &E, ArnArgs &a)
     if (M.format == FSP_CSR) {
         if (mixed) {
             CsrF<float> A = csr_view<float>(M);
-            k_arn_csr<float, B, NKB><<<E.G, kBlock, red_b + sizeof(double) * A.cap, E.s>>>(A, a);
+            k_arn_csr<float, B><<<E.G, kBlock, red_b + sizeof(double) * A.cap, E.s>>>(A, a);
         } else {
             CsrF<double> A = csr_view<double>(M);
-            k_arn_csr<double, B, NKB><<<E.G, kBlock, red_b + sizeof(double) * A.cap, E.s>>>(A, a);
+            k_arn_csr<double, B><<<E.G, kBlock, red_b + sizeof(double) * A.cap, E.s>>>(A, a);
         }
     } else if (M.format == FSP_HYB) {
         if (mixed)
-            k_arn_hyb<float, B, NKB><<<E.G, kBlock, red_b, E.s>>>(hyb_view<float>(M), a);
+            k_arn_hyb<float, B><<<E.G, kBlock, red_b, E.s>>>(hyb_view<float>(M), a);
         else
-            k_arn_hyb<double, B, NKB><<<E.G, kBlock, red_b, E.s>>>(hyb_view<double>(M), a);
+            k_arn_hyb<double, B><<<E.G, kBlock, red_b, E.s>>>(hyb_view<double>(M), a);
     } else {
         if (mixed && M.hm.has_factors) {
             KronF<float> A = kron_view<float>(M, -1, 0.0);
             const size_t tb = sizeof(float) * (size_t)A.tab_size;
             const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
-            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, float, B, NKB><<<E.G, kBlock, red_b + (use_lds ? tb : 0), E.s>>>(A, a, use_lds)));
+            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, float, B><<<E.G, kBlock, red_b + (use_lds ? tb : 0), E.s>>>(A, a, use_lds)));
         } else {
             KronF<double> A = kron_view<double>(M, -1, 0.0);
             const size_t tb = sizeof(double) * (size_t)A.tab_size;
             const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
-            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, double, B, NKB><<<E.G, kBlock, red_b + (use_lds ? tb : 0), E.s>>>(A, a, use_lds)));
+            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, double, B><<<E.G, kBlock, red_b + (use_lds ? tb : 0), E.s>>>(A, a, use_lds)));
         }
     }
     FSP_HIP(hipGetLastError());
     return FSP_OK;
 }
 
-template <typename B>
-static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
-{
-    const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
-    FSP_SWITCH_NK(nk, return (launch_arn_spmv_nk<B, NK_>(E, a)));
-    return FSP_OK;
-}
 
 template <typename B>
 static void launch_cb_dots(ExpvImpl &E, ArnArgs &a)

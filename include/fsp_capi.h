@@ -291,6 +291,38 @@ fsp_status fsp_marginal(const double *d_p, int n_species, const int *bounds,
 /* sum_k w[k] log(max(floor, q[k])) (weighted histogram log-likelihood). */
 fsp_status fsp_loglik_weighted(const double *d_q, const double *d_w, int64_t count,
                                double floor_val, double *h_ll, void *stream);
+/* Histogram likelihood of FSP solutions: the intended semantics of
+ * cme::Likelihood::evaluate (src/Likelihood.cpp:8-33) on joint marginal
+ * histograms, LL = sum_k sum_b counts[k][b] log max(P_k[b], floor_val),
+ * P_k the marginal over the species with keep_mask[s] != 0 (first kept
+ * species fastest) of p(times[k]) = exp(times[k] A) p0, no sink state. */
+typedef struct fsp_lik_data {
+    int n_times;
+    const double *times;        /* increasing, > 0                             */
+    const int *keep_mask;       /* [n_species]                                 */
+    const double *counts;       /* host [n_times][n_bins]                      */
+    double floor_val;           /* e.g. 1e-12 (the reference uses 1e-44)       */
+} fsp_lik_data;
+typedef struct fsp_grad_options {
+    fsp_format format;          /* forward operator (FSP_CSR)                  */
+    int m;                      /* Krylov dimension (30)                       */
+    double tol;                 /* Krylov tolerance (1e-10)                    */
+    int quad_nodes;             /* Gauss-Legendre nodes per panel (8)          */
+    int grading_levels;         /* panels per observation interval / 2 (10)    */
+} fsp_grad_options;
+/* LL at coefficients exp(log_coefs); the model needs factor tables.  d_p0:
+ * initial distribution (n states).  options may be NULL. */
+fsp_status fsp_loglik(const fsp_model *model, const double *log_coefs, const double *d_p0,
+                      const fsp_lik_data *data, const fsp_grad_options *options,
+                      double *h_ll, void *stream);
+/* LL and dLL/dlog_coefs by the adjoint method: lambda integrated backwards
+ * with the transposed generator (Krylov expv on A^T), the sensitivity
+ * integrals by graded Gauss-Legendre panels, p recomputed per observation
+ * interval from checkpoints.  No reference counterpart. */
+fsp_status fsp_loglik_grad(const fsp_model *model, const double *log_coefs,
+                           const double *d_p0, const fsp_lik_data *data,
+                           const fsp_grad_options *options, double *h_ll, double *h_grad,
+                           void *stream);
 /* Dense matrix exponential on the device (one workgroup, FP64, Pade q=6
  * with scaling and squaring; stands in for arma::expmat at
  * src/KryExpvFSP.cu:125).  Column-major N x N, N <= 52. */
