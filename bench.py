@@ -157,6 +157,59 @@ def cpu_baseline(net, t_out, budget_s=15.0, threads=None):
     return dt, steps, threads
 
 
+def bench_bdf(net):
+    """The headline network with a time-varying promoter rate, G2->G1 =
+    0.5 (1 + sin(2 pi t / 30)), integrated by the native BDF (rtol 1e-6,
+    atol 1e-12) through the same 30 output times to t = 60."""
+    from cuda_fsp_amd import capi as _c
+    base = net.coefs.copy()
+
+    def coefs(t):
+        c = base.copy()
+        c[3] = 0.5 * (1.0 + np.sin(2.0 * np.pi * t / 30.0))
+        return c
+
+    A = _c.Mat(net, "csr")
+    p = _c.DeviceBuffer.from_host(M.delta_p0(net))
+    B = _c.Bdf(A, coefs, rtol=1e-6, atol=1e-12)
+    t0 = time.perf_counter()
+    for t in [2.0 * k for k in range(1, 31)]:
+        B.advance(p, t)
+    dt = time.perf_counter() - t0
+    st = B.stats()
+    mass = float(p.to_host().sum())
+    B.close()
+    A.close()
+    p.free()
+    return {"workload": "gene_expression 3x512x512, k21(t) = 0.5(1+sin(2 pi t/30)), BDF "
+                        "rtol 1e-6 atol 1e-12, 30 outputs to t=60, GMRES(30) Jacobi",
+            "solves_per_s": round(1.0 / dt, 4), "s_per_solve": round(dt, 3),
+            "steps": int(st.n_steps), "rejects": int(st.n_rejects),
+            "gmres_iterations": int(st.n_lin_iters), "final_order": int(st.order),
+            "mass_in_box_t60": mass}
+
+
+def bench_adjoint(net):
+    """One log-likelihood gradient by the adjoint on the headline network:
+    joint (n, m) histograms of 2,000 cells at t = 5, 10, 20, 30, 45, 60
+    (multinomial from the FSP marginal, seed 2), 9 log rates."""
+    from cuda_fsp_amd import likelihood as _L
+    times = [5.0, 10.0, 20.0, 30.0, 45.0, 60.0]
+    keep = [0, 1, 1]
+    counts = _L.sample_histograms(net, times, keep, 2000, seed=2, tol=1e-10)
+    prob = _L.LikelihoodProblem(net, times, keep, counts, tol=1e-10, grading_levels=6)
+    th = np.log(net.coefs)
+    t0 = time.perf_counter()
+    ll = prob.value(th)
+    t1 = time.perf_counter()
+    ll2, g = prob.value_and_grad(th)
+    t2 = time.perf_counter()
+    return {"workload": "gene_expression 3x512x512, 6 times x 2,000 cells, 9 log rates, "
+                        "Krylov tol 1e-10, 2x6 graded Gauss-Legendre panels of 8 nodes per interval",
+            "loglik": ll, "loglik_s": round(t1 - t0, 3), "gradient_s": round(t2 - t1, 3),
+            "gradients_per_s": round(1.0 / (t2 - t1), 4), "gradient": [round(x, 6) for x in g]}
+
+
 def pmc_traffic(kernel="k_apply_csr<double>"):
     """HBM bytes per launch of the SpMV kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (counter passes cannot run inside the
@@ -200,6 +253,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the BDF and adjoint-gradient sections")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the calibration and SpMV kernels (for counter passes)")
     args = ap.parse_args()
@@ -255,6 +310,13 @@ def main():
                          f"({cdt:.1f} s, {threads} OpenMP threads), extrapolated",
                "cpu": platform.processor() or platform.machine()}
 
+    extras = {}
+    if not args.no_extras:
+        print("[bench] BDF time-varying solve", flush=True)
+        extras["bdf"] = bench_bdf(net)
+        print("[bench] adjoint gradient", flush=True)
+        extras["adjoint"] = bench_adjoint(net)
+
     line = {
         "metric": METRIC, "value": round(solves_s, 4), "unit": "solves/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -270,6 +332,7 @@ def main():
         "mixed": {"solves_per_s": round(args.steps / dtm, 4),
                   "speedup_vs_f64": round(dt / dtm, 3), "worst_l1_vs_f64": l1_mixed,
                   "krylov_steps_per_solve": int(stm.i_step)},
+        **extras,
     }
     print(json.dumps(line), flush=True)
 

@@ -13,7 +13,8 @@
 //     through t_new and the last k accepted points;
 //   * the implicit stage (alpha_0 I - A(t_new)) y = b is linear in y and is
 //     solved by restarted GMRES on the error-weighted system D M D^-1 (D = 1 /
-//     (rtol |y_n| + atol)), started from the predictor;
+//     (rtol |y_n| + atol)), right-preconditioned by its diagonal
+//     alpha_0 - A_ii (Jacobi), started from the predictor;
 //   * error test on the predictor/corrector difference scaled by the exact
 //     variable-step ratio r / (1 + r), r = 1 / (alpha_0 (t_new - t_{n-k}));
 //   * order selection from divided differences of the accepted history.
@@ -134,23 +135,27 @@ __global__ void k_scale_copy(const double *__restrict__ x, double s, double *__r
         y[i] = x[i] * s;
 }
 
-// t = x / ewt (back to the unscaled space)
+// Right preconditioner: Jacobi, q_i = alpha0 - A_ii (> 0, the diagonal of
+// the scaled stage operator D (alpha0 I - A) D^-1).
+// t = x / (q ewt): back to the unscaled space, preconditioned
 __global__ void k_unscale(const double *__restrict__ x, const double *__restrict__ ewt,
+                          const double *__restrict__ diag, double alpha0,
                           double *__restrict__ t, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        t[i] = x[i] / ewt[i];
+        t[i] = x[i] / ((alpha0 - diag[i]) * ewt[i]);
 }
 
-// out = alpha0 x - ewt * u   (scaled operator D (alpha0 I - A) D^-1 applied to x)
+// out = alpha0 x / q - ewt * u   (D (alpha0 I - A) D^-1 Q^-1 applied to x,
+// u = A t from k_unscale)
 __global__ void k_op_post(const double *__restrict__ x, const double *__restrict__ ewt,
-                          const double *__restrict__ u, double alpha0,
-                          double *__restrict__ out, int64_t n)
+                          const double *__restrict__ diag, const double *__restrict__ u,
+                          double alpha0, double *__restrict__ out, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = alpha0 * x[i] - ewt[i] * u[i];
+        out[i] = alpha0 * (x[i] / (alpha0 - diag[i])) - ewt[i] * u[i];
 }
 
 // r = ewt * (b - alpha0 y + u), u = A y   (scaled residual)
@@ -163,16 +168,17 @@ __global__ void k_resid(const double *__restrict__ b, const double *__restrict__
         r[i] = ewt[i] * (b[i] - alpha0 * y[i] + u[i]);
 }
 
-// y += (sum_k g_k V_k) / ewt
+// y += Q^-1 D^-1 (sum_k g_k V_k)
 __global__ void k_sol_update(const double *__restrict__ V, int64_t ldv, int nk,
                              const double *__restrict__ g, const double *__restrict__ ewt,
+                             const double *__restrict__ diag, double alpha0,
                              double *__restrict__ y, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         double acc = 0.0;
         for (int k = 0; k < nk; ++k) acc += V[(int64_t)k * ldv + i] * g[k];
-        y[i] += acc / ewt[i];
+        y[i] += acc / ((alpha0 - diag[i]) * ewt[i]);
     }
 }
 
@@ -227,6 +233,7 @@ struct BdfImpl {
     double t = 0.0, h = 0.0;
     int k = 1, steps_at_k = 0, fails = 0;
     double *ewt = nullptr, *b = nullptr, *yp = nullptr, *ynew = nullptr, *u = nullptr, *tmp = nullptr;
+    double *diag = nullptr;         // generator diagonal at the stage time
     double *V = nullptr;            // (kdim + 1) x n scaled Krylov basis
     double *partials = nullptr;     // 2 x (kdim + 1) x kRedGrid
     double *dred = nullptr;         // (kdim + 1) reduced values
@@ -323,10 +330,12 @@ fsp_status gmres(BdfImpl &B, double alpha0, double tol, double *y, bool *converg
         bool done = false;
         for (; j < m && !done; ++j) {
             double *vj = B.V + (int64_t)j * n, *w = B.V + (int64_t)(j + 1) * n;
-            hipLaunchKernelGGL(k_unscale, dim3(G), dim3(kBlock), 0, B.s, vj, B.ewt, B.tmp, n);
+            hipLaunchKernelGGL(k_unscale, dim3(G), dim3(kBlock), 0, B.s, vj, B.ewt, B.diag, alpha0,
+                               B.tmp, n);
             FSP_TRY(mat_apply(*B.A, B.tmp, B.u, 0, B.s));
             B.st.n_rhs++;
-            hipLaunchKernelGGL(k_op_post, dim3(G), dim3(kBlock), 0, B.s, vj, B.ewt, B.u, alpha0, w, n);
+            hipLaunchKernelGGL(k_op_post, dim3(G), dim3(kBlock), 0, B.s, vj, B.ewt, B.diag, B.u,
+                               alpha0, w, n);
             FSP_HIP(hipGetLastError());
             // classical Gram-Schmidt, two passes
             std::fill(hcol.begin(), hcol.end(), 0.0);
@@ -374,7 +383,7 @@ fsp_status gmres(BdfImpl &B, double alpha0, double tol, double *y, bool *converg
         }
         FSP_TRY(push_small(B, z.data(), j));
         hipLaunchKernelGGL(k_sol_update, dim3(G), dim3(kBlock), 0, B.s, B.V, n, j, B.d_small,
-                           B.ewt, y, n);
+                           B.ewt, B.diag, alpha0, y, n);
         FSP_HIP(hipGetLastError());
         if (done) {
             *converged = true;
@@ -389,7 +398,7 @@ fsp_status bdf_alloc(BdfImpl &B)
 {
     const size_t n = (size_t)B.n, kd = (size_t)B.opt.krylov_dim + 1;
     for (int j = 0; j < kHist; ++j) FSP_HIP(hipMalloc(&B.Y[j], sizeof(double) * n));
-    double **vs[] = {&B.ewt, &B.b, &B.yp, &B.ynew, &B.u, &B.tmp};
+    double **vs[] = {&B.ewt, &B.b, &B.yp, &B.ynew, &B.u, &B.tmp, &B.diag};
     for (double **p : vs) FSP_HIP(hipMalloc(p, sizeof(double) * n));
     FSP_HIP(hipMalloc(&B.V, sizeof(double) * n * kd));
     FSP_HIP(hipMalloc(&B.partials, sizeof(double) * 2 * kd * kRedGrid));
@@ -403,7 +412,7 @@ void bdf_free(BdfImpl &B)
 {
     for (int j = 0; j < kHist; ++j)
         if (B.Y[j]) (void)hipFree(B.Y[j]);
-    double *vs[] = {B.ewt, B.b, B.yp, B.ynew, B.u, B.tmp, B.V, B.partials, B.dred, B.d_small};
+    double *vs[] = {B.ewt, B.b, B.yp, B.ynew, B.u, B.tmp, B.diag, B.V, B.partials, B.dred, B.d_small};
     for (double *p : vs)
         if (p) (void)hipFree(p);
     if (B.h_red) (void)hipHostFree(B.h_red);
@@ -454,6 +463,7 @@ fsp_status try_step(BdfImpl &B, double h, int k, bool *accepted, double *err)
     // solve the stage from the predictor
     FSP_HIP(hipMemcpyAsync(B.ynew, B.yp, sizeof(double) * (size_t)B.n, hipMemcpyDeviceToDevice, B.s));
     FSP_TRY(set_time(B, t_new));
+    FSP_TRY(mat_diagonal(*B.A, B.diag, B.s));
     bool conv = false;
     FSP_TRY(gmres(B, alpha0, B.opt.lin_tol_factor * std::fabs(alpha0), B.ynew, &conv));
     if (!conv) {

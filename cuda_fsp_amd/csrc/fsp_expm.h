@@ -128,28 +128,30 @@ __device__ void expm_block(int N, double *A, double *X, double *T, double *Nm, d
             }
             __syncthreads();
         }
+        // one thread per row below the pivot: multiplier, row update against
+        // the (unchanged, broadcast) pivot row, multiplier stored in place
         const double pv = Dm[k + k * N];
-        for (int i = k + 1 + tid; i < N; i += blockDim.x) T[i] = Dm[i + k * N] / pv;
-        __syncthreads();
-        const int rows = N - k - 1;
-        for (int e = tid; e < rows * N; e += blockDim.x) {
-            const int i = k + 1 + e % rows, j = e / rows;
-            const double l = T[i];
-            if (j > k) Dm[i + j * N] -= l * Dm[k + j * N];
-            Nm[i + j * N] -= l * Nm[k + j * N];
+        for (int i = k + 1 + tid; i < N; i += blockDim.x) {
+            const double l = Dm[i + k * N] / pv;
+            for (int j = k + 1; j < N; ++j) Dm[i + j * N] -= l * Dm[k + j * N];
+            for (int j = 0; j < N; ++j) Nm[i + j * N] -= l * Nm[k + j * N];
+            Dm[i + k * N] = l;
         }
         __syncthreads();
-        for (int i = k + 1 + tid; i < N; i += blockDim.x) Dm[i + k * N] = T[i];
-        __syncthreads();
     }
-    // back substitution, one right-hand side per thread
+    // back substitution, one right-hand side per thread, on a row-major copy
+    // of Nm in X (consecutive lanes read consecutive LDS words)
+    for (int e = tid; e < NN; e += blockDim.x) X[(e / N) + (e % N) * N] = Nm[e];
+    __syncthreads();
     for (int j = tid; j < N; j += blockDim.x) {
         for (int i = N - 1; i >= 0; --i) {
-            double acc = Nm[i + j * N];
-            for (int k = i + 1; k < N; ++k) acc -= Dm[i + k * N] * Nm[k + j * N];
-            Nm[i + j * N] = acc / Dm[i + i * N];
+            double acc = X[j + i * N];
+            for (int k = i + 1; k < N; ++k) acc -= Dm[i + k * N] * X[j + k * N];
+            X[j + i * N] = acc / Dm[i + i * N];
         }
     }
+    __syncthreads();
+    for (int e = tid; e < NN; e += blockDim.x) Nm[e] = X[(e / N) + (e % N) * N];
     __syncthreads();
     for (int sq = 0; sq < js; ++sq) {
         matmul_lds(N, Nm, Nm, T);

@@ -33,8 +33,13 @@ struct CsrF {
     int cap;  // LDS product slots per chunk (>= kBlock)
 };
 
-// Phase 1 streams the tile's nonzeros (coalesced) into LDS as products;
-// phase 2 lets each thread add its row's products in column order.
+constexpr int kCsrPer = 16;  // LDS products per thread per chunk (cap <= kCsrPer * kBlock)
+
+// Phase 1 streams the tile's nonzeros (coalesced) into LDS as products: all
+// of a thread's column indices and values of the chunk are loaded first
+// (independent, non-temporal: they are read once), then all its gathers of
+// x, so a chunk costs two memory round trips; phase 2 lets each thread add
+// its row's products in column order.
 template <typename V, class X>
 __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
                                   double *prod, dd &sink)
@@ -49,8 +54,23 @@ __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
     for (int c0 = k0; c0 < k1; c0 += A.cap) {
         const int c1 = min(c0 + A.cap, k1);
         __syncthreads();
-        for (int k = c0 + (int)threadIdx.x; k < c1; k += kBlock)
-            prod[k - c0] = (double)A.vals[k] * x(A.cols[k]);
+        int cl[kCsrPer];
+        V vl[kCsrPer];
+#pragma unroll
+        for (int q = 0; q < kCsrPer; ++q) {
+            const int k = c0 + (int)threadIdx.x + q * kBlock;
+            cl[q] = 0;
+            vl[q] = (V)0;
+            if (k < c1) {
+                cl[q] = __builtin_nontemporal_load(A.cols + k);
+                vl[q] = __builtin_nontemporal_load(A.vals + k);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kCsrPer; ++q) {
+            const int k = c0 + (int)threadIdx.x + q * kBlock;
+            if (k < c1) prod[k - c0] = (double)vl[q] * x(cl[q]);
+        }
         __syncthreads();
         const int a = max(kb, c0), b = min(ke, c1);
         for (int k = a; k < b; ++k) y += prod[k - c0];
@@ -203,7 +223,8 @@ inline CsrF<V> csr_view(const MatImpl &M)
     A.sink_vals = (const double *)M.sink_vals;
     A.n = (uint32_t)M.n;
     int cap = kBlock * (M.max_row > 1 ? M.max_row : 1);
-    A.cap = cap > 4096 ? 4096 : cap;  // 32 KB of LDS products per chunk
+    // at most kCsrPer products per thread per chunk (32 KB of LDS)
+    A.cap = cap > kCsrPer * kBlock ? kCsrPer * kBlock : cap;
     return A;
 }
 

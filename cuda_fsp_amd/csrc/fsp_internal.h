@@ -146,6 +146,8 @@ fsp_status mat_apply_transpose(MatImpl &M, const double *d_x, double *d_y,
                                int accumulate, hipStream_t s);
 fsp_status mat_apply_term(MatImpl &M, int r, double c, const double *d_x, double *d_y,
                           int accumulate, hipStream_t s);
+// Generator diagonal -sum_r c_r a_r(x_i), i < n (matrix free, any format).
+fsp_status mat_diagonal(MatImpl &M, double *d_diag, hipStream_t s);
 void mat_free(MatImpl &M);
 double mat_bytes_per_apply(const MatImpl &M);
 

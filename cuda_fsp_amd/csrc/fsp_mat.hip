@@ -566,9 +566,51 @@ __global__ void __launch_bounds__(kBlock) k_apply_kron(KronF<V> A, PlainX X, dou
     if (A.with_sink && !transpose) put_dd_partial(partials, sink, red);
 }
 
+// Diagonal of the generator: -sum_r c_r a_r(x_i) over the reactions that
+// change the state (the whole outflow, including outflow leaving the box).
+template <int D, typename V>
+__global__ void __launch_bounds__(kBlock) k_kron_diag(KronF<V> A, double *__restrict__ out,
+                                                      uint32_t tiles, int use_lds)
+{
+    extern __shared__ double smem[];
+    const V *tab = A.tab_g;
+    if (use_lds && A.tab_g) {
+        stage_table((V *)smem, A.tab_g, A.tab_size);
+        tab = (const V *)smem;
+    }
+    for (TileWalk w = tile_walk(tiles); w.t < w.end; w.t += w.step) {
+        const uint32_t i = w.t * kBlock + threadIdx.x;
+        if (i >= A.n) continue;
+        int xs[D];
+        decode<D>(i, A.dm, xs);
+        double s = 0.0;
+        for (int r = 0; r < A.R; ++r) {
+            const RxnDev &rx = A.dm->rx[r];
+            if (rx.zero_nu) continue;
+            const double a = A.prop_table ? A.prop_table[(size_t)r * A.n + i]
+                                          : (double)prop_tab<D>(rx, xs, tab, 0);
+            s += A.coefs[r] * a;
+        }
+        out[i] = -s;
+    }
+}
+
 }  // namespace
 
 static int apply_grid_cap() { return 4096; }
+
+fsp_status mat_diagonal(MatImpl &M, double *d_diag, hipStream_t s)
+{
+    KronF<double> A = kron_view<double>(M, -1, 0.0);
+    const uint32_t tiles = (uint32_t)((M.n + kBlock - 1) / kBlock);
+    const int grid = grid_for(tiles, apply_grid_cap());
+    const size_t tb = sizeof(double) * (size_t)A.tab_size;
+    const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
+    FSP_SWITCH_D(M.hm.dm.d, (k_kron_diag<D_, double><<<grid, kBlock, use_lds ? tb : 0, s>>>(
+                                A, d_diag, tiles, use_lds)));
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
+}
 
 template <typename V>
 static fsp_status launch_kron(const MatImpl &M, const double *x, double *y, int accumulate,

@@ -111,5 +111,5 @@ def test_torch_autograd_bridge():
     ll = L.torch_loglik(prob, theta)
     (2.0 * ll).backward()
     ll_ref, g_ref = prob.value_and_grad(np.log(net.coefs))
-    assert float(ll) == pytest.approx(ll_ref, rel=1e-14)
+    assert float(ll.detach()) == pytest.approx(ll_ref, rel=1e-14)
     assert np.allclose(theta.grad.numpy(), 2.0 * g_ref, rtol=1e-14, atol=0)
