@@ -24,8 +24,6 @@
 
 #include "fsp_formats.h"
 
-extern "C" fsp_status fsp_mat_internal(fsp_mat A, fsp::MatImpl **impl);
-
 namespace fsp {
 namespace {
 

@@ -20,34 +20,10 @@
 #include <string>
 #include <vector>
 
-#include "fsp_batch_kernel.h"
+#include "fsp_batch_impl.h"
 
 namespace fsp {
 
-struct BatchImpl {
-    int count = 0;
-    int m = 30, iop = 0, q_iop = 2, max_reject = 1000;
-    long long max_nstep = 100000;
-    double tol = 1e-8, anorm = 1.0;
-    bool mixed = false;
-    int block = 512, wg_per_cu = 2, cus = 256, grid = 1, dmax = 4;
-    std::vector<HostModel> hms;
-    std::vector<BatchInst> h_inst;
-    std::vector<int64_t> vec_off;
-    DevModel *d_models = nullptr;
-    double *d_tabs = nullptr, *d_coefs = nullptr, *d_vecs = nullptr, *d_p10 = nullptr;
-    BatchInst *d_inst = nullptr;
-    int *d_order = nullptr;
-    unsigned int *d_queue = nullptr;
-    double *ws = nullptr;
-    float *ws32 = nullptr;
-    size_t ws_stride = 0, ws32_stride = 0, ws_slots = 0;
-    int nv_max = 0, tab_lds = 0;
-    hipStream_t cap = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_block = 0, graph_wgcu = 0;
-};
 
 static size_t batch_lds_bytes(const BatchImpl &B, int bs)
 {
@@ -123,24 +99,23 @@ static fsp_status enqueue_batch(BatchImpl &B, int block, int wg_per_cu, hipStrea
 
 // The persistent kernel matches the oracle bit for bit in the CPU (SIMT)
 // build of this file, but one run on a gfx950 device stalled without an
-// identified cause; until that cause is found, launches on a GPU device are
-// refused with an error rather than risked.
-static fsp_status device_launch_allowed()
+// identified cause; until that cause is found it is never launched on a GPU
+// device, and batches there go through the stream scheduler instead.
+static fsp_status persistent_allowed(bool *ok)
 {
     int dev = 0;
     FSP_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     FSP_HIP(hipGetDeviceProperties(&prop, dev));
-    if (std::strncmp(prop.gcnArchName, "gfx", 3) == 0)
-        return fail(FSP_ERR_UNSUPPORTED,
-                    "the batched persistent solver is disabled on GPU devices: a device run "
-                    "stalled and its cause is not yet identified");
+    *ok = std::strncmp(prop.gcnArchName, "gfx", 3) != 0;
     return FSP_OK;
 }
 
 static fsp_status batch_graph(BatchImpl &B, hipStream_t s)
 {
-    FSP_TRY(device_launch_allowed());
+    bool ok = false;
+    FSP_TRY(persistent_allowed(&ok));
+    if (!ok) return fail(FSP_ERR_UNSUPPORTED, "the persistent batch kernel is not launched on GPU devices");
     if (B.gexec && B.graph_block == B.block && B.graph_wgcu == B.wg_per_cu) {
         FSP_HIP(hipGraphLaunch(B.gexec, s));
         return FSP_OK;
@@ -167,6 +142,7 @@ static fsp_status batch_graph(BatchImpl &B, hipStream_t s)
 
 static void batch_free(BatchImpl &B)
 {
+    batch_streams_free(B);
     if (B.gexec) (void)hipGraphExecDestroy(B.gexec);
     if (B.graph) (void)hipGraphDestroy(B.graph);
     if (B.cap) (void)hipStreamDestroy(B.cap);
@@ -264,6 +240,24 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
         int64_t cprev = 0;
         for (int j = 0; j < i; ++j) cprev += B.hms[j].dm.R;
         cvec.insert(cvec.end(), coefs + cprev, coefs + cprev + hm.dm.R);
+        // owned copies for the stream scheduler
+        {
+            const fsp_model &src = models[i];
+            ModelCopy mcp;
+            mcp.d = src.n_species;
+            mcp.R = src.n_reactions;
+            mcp.bounds.assign(src.bounds, src.bounds + mcp.d);
+            mcp.sr.assign(src.stoich_rowptrs, src.stoich_rowptrs + mcp.R + 1);
+            const int snz = src.stoich_rowptrs[mcp.R];
+            mcp.sv.assign(src.stoich_vals, src.stoich_vals + snz);
+            mcp.sc.assign(src.stoich_cols, src.stoich_cols + snz);
+            size_t fw = 0;
+            for (int sp = 0; sp < mcp.d; ++sp) fw += (size_t)src.bounds[sp] + 1;
+            mcp.factors.assign(src.factors, src.factors + fw * (size_t)mcp.R);
+            B.mc.push_back(std::move(mcp));
+            B.coef_off.push_back((int64_t)B.h_coefs.size());
+            B.h_coefs.insert(B.h_coefs.end(), coefs + cprev, coefs + cprev + hm.dm.R);
+        }
         const int sink = with_sink ? (with_sink[i] ? 1 : 0) : 0;
         const int nv = (int)hm.dm.n + sink;
         B.vec_off[i] = vec_total;
@@ -311,6 +305,8 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
     std::vector<int> order((size_t)count);
     for (int i = 0; i < count; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    B.order = order;
+    B.cost = cost;
     FSP_HIP(hipMalloc(&B.d_order, sizeof(int) * (size_t)count));
     FSP_HIP(hipMemcpy(B.d_order, order.data(), sizeof(int) * (size_t)count, hipMemcpyHostToDevice));
     FSP_HIP(hipMalloc(&B.d_queue, sizeof(unsigned int)));
@@ -322,14 +318,21 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
     }
     FSP_HIP(hipMalloc(&B.d_p10, sizeof(double) * p10.size()));
     FSP_HIP(hipMemcpy(B.d_p10, p10.data(), sizeof(double) * p10.size(), hipMemcpyHostToDevice));
-    // workspace for the largest grid any configuration may use (2 per CU)
-    B.ws_slots = (size_t)std::min(count, B.cus * 2);
+    // workspace for the largest grid any configuration may use (2 per CU);
+    // none where the persistent kernel is not launched
+    bool persistent = false;
+    {
+        const fsp_status pst = persistent_allowed(&persistent);
+        if (pst != FSP_OK) return bail(pst);
+    }
+    B.ws_slots = persistent ? (size_t)std::min(count, B.cus * 2) : 0;
     const size_t nvm = (size_t)B.nv_max;
     B.ws_stride = (B.mixed ? 3 : (size_t)(B.m + 3)) * nvm;
     B.ws32_stride = B.mixed ? (size_t)B.m * nvm : 0;
-    hipError_t e = hipMalloc(&B.ws, sizeof(double) * B.ws_stride * B.ws_slots);
+    hipError_t e = hipSuccess;
+    if (B.ws_slots) e = hipMalloc(&B.ws, sizeof(double) * B.ws_stride * B.ws_slots);
     if (e != hipSuccess) return bail(fail(FSP_ERR_NOMEM, "batch workspace"));
-    if (B.mixed) {
+    if (B.mixed && B.ws_slots) {
         e = hipMalloc(&B.ws32, sizeof(float) * B.ws32_stride * B.ws_slots);
         if (e != hipSuccess) return bail(fail(FSP_ERR_NOMEM, "batch workspace"));
     }
@@ -361,6 +364,13 @@ extern "C" fsp_status fsp_batch_solve(fsp_batch b, void *stream)
 {
     using namespace fsp;
     if (!b) return fail(FSP_ERR_INVALID, "NULL batch");
+    bool persistent = false;
+    FSP_TRY(persistent_allowed(&persistent));
+    if (!persistent) {
+        // stream scheduler: synchronous, on its own streams
+        FSP_HIP(hipStreamSynchronize((hipStream_t)stream));
+        return batch_solve_streams(b->impl);
+    }
     return batch_graph(b->impl, (hipStream_t)stream);
 }
 
@@ -396,28 +406,35 @@ extern "C" fsp_status fsp_batch_get_config(fsp_batch b, int *block, int *wg_per_
 {
     using namespace fsp;
     if (!b) return fail(FSP_ERR_INVALID, "NULL batch");
-    if (block) *block = b->impl.block;
-    if (wg_per_cu) *wg_per_cu = b->impl.wg_per_cu;
+    bool persistent = false;
+    FSP_TRY(persistent_allowed(&persistent));
+    // stream scheduler: block 0, wg_per_cu = number of concurrent streams
+    if (block) *block = persistent ? b->impl.block : 0;
+    if (wg_per_cu) *wg_per_cu = persistent ? b->impl.wg_per_cu : b->impl.workers;
     return FSP_OK;
 }
 
-// Autotuner: times each (workgroup size, workgroups per CU) candidate on the
-// batch itself and keeps the fastest; the choice is cached in a JSON file
-// keyed by device, instance count, longest vector, species, m and precision.
+// Autotuner: times each candidate configuration on the batch itself and
+// keeps the fastest: (workgroup size, workgroups per CU) for the persistent
+// kernel, the number of concurrent streams for the stream scheduler.  The
+// choice is cached in a JSON file keyed by device, instance count, longest
+// vector, species, m, precision and path.
 extern "C" fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, int force,
                                          double *best_ms)
 {
     using namespace fsp;
     if (!b) return fail(FSP_ERR_INVALID, "NULL batch");
-    FSP_TRY(device_launch_allowed());
     BatchImpl &B = b->impl;
+    bool persistent = false;
+    FSP_TRY(persistent_allowed(&persistent));
     hipDeviceProp_t prop;
     int dev = 0;
     FSP_HIP(hipGetDevice(&dev));
     FSP_HIP(hipGetDeviceProperties(&prop, dev));
     std::ostringstream key;
     key << prop.gcnArchName << "|cu" << B.cus << "|n" << B.count << "|nv" << B.nv_max << "|d"
-        << B.dmax << "|m" << B.m << "|" << (B.mixed ? "mixed" : "f64");
+        << B.dmax << "|m" << B.m << "|" << (B.mixed ? "mixed" : "f64")
+        << (persistent ? "" : "|streams");
     std::string cache;
     if (cache_path) {
         std::ifstream f(cache_path);
@@ -427,49 +444,61 @@ extern "C" fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, in
         const std::string k = "\"" + key.str() + "\": {";
         const size_t pos = cache.find(k);
         if (pos != std::string::npos && !force) {
-            int blk = 0, wg = 0;
+            int blk = 0, wg = 0, wk = 0;
             double ms = 0;
-            if (std::sscanf(cache.c_str() + pos + k.size(), "\"block\": %d, \"wg_per_cu\": %d, \"ms\": %lf",
-                            &blk, &wg, &ms) == 3) {
-                B.block = blk;
-                B.wg_per_cu = wg;
+            if (std::sscanf(cache.c_str() + pos + k.size(),
+                            "\"block\": %d, \"wg_per_cu\": %d, \"workers\": %d, \"ms\": %lf", &blk,
+                            &wg, &wk, &ms) == 4) {
+                if (persistent) {
+                    B.block = blk;
+                    B.wg_per_cu = wg;
+                } else {
+                    B.workers = wk;
+                }
                 if (best_ms) *best_ms = ms;
                 return FSP_OK;
             }
         }
     }
-    const int blocks[3] = {256, 512, 1024};
     double best = 1e300;
-    int bb = B.block, bw = B.wg_per_cu;
-    hipEvent_t e0, e1;
-    FSP_HIP(hipEventCreate(&e0));
-    FSP_HIP(hipEventCreate(&e1));
-    for (int bs : blocks) {
-        for (int wg = 1; wg <= 2; ++wg) {
-            if (batch_lds_bytes(B, bs) * wg > 160 * 1024) continue;
-            FSP_TRY(enqueue_batch(B, bs, wg, nullptr));  // warm-up
-            FSP_HIP(hipEventRecord(e0, nullptr));
-            FSP_TRY(enqueue_batch(B, bs, wg, nullptr));
-            FSP_HIP(hipEventRecord(e1, nullptr));
-            FSP_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            FSP_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) {
-                best = ms;
-                bb = bs;
-                bw = wg;
+    int bb = 0, bw = 0, bk = 0;
+    if (persistent) {
+        const int blocks[3] = {256, 512, 1024};
+        bb = B.block;
+        bw = B.wg_per_cu;
+        hipEvent_t e0, e1;
+        FSP_HIP(hipEventCreate(&e0));
+        FSP_HIP(hipEventCreate(&e1));
+        for (int bs : blocks) {
+            for (int wg = 1; wg <= 2; ++wg) {
+                if (batch_lds_bytes(B, bs) * wg > 160 * 1024) continue;
+                FSP_TRY(enqueue_batch(B, bs, wg, nullptr));  // warm-up
+                FSP_HIP(hipEventRecord(e0, nullptr));
+                FSP_TRY(enqueue_batch(B, bs, wg, nullptr));
+                FSP_HIP(hipEventRecord(e1, nullptr));
+                FSP_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                FSP_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    bb = bs;
+                    bw = wg;
+                }
             }
         }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        B.block = bb;
+        B.wg_per_cu = bw;
+    } else {
+        FSP_TRY(batch_autotune_streams(B, &best, &bk));
+        B.workers = bk;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    B.block = bb;
-    B.wg_per_cu = bw;
     if (best_ms) *best_ms = best;
     if (cache_path) {
         std::ostringstream ent;
         ent << "\"" << key.str() << "\": {\"block\": " << bb << ", \"wg_per_cu\": " << bw
-            << ", \"ms\": " << best << "}";
+            << ", \"workers\": " << bk << ", \"ms\": " << best << "}";
         std::string body;
         const std::string k = "\"" + key.str() + "\": {";
         const size_t pos = cache.find(k);

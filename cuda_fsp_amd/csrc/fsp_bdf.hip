@@ -633,8 +633,6 @@ fsp_status bdf_advance(BdfImpl &B, double *d_v, double t_out)
 using fsp::BdfImpl;
 using fsp::MatImpl;
 
-extern "C" fsp_status fsp_mat_internal(fsp_mat A, MatImpl **impl);
-
 struct fsp_bdf_s {
     BdfImpl impl;
 };

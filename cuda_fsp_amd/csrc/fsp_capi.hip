@@ -341,7 +341,7 @@ fsp_status fsp_states_to_index(int n_species, const int *bounds, const int *d_st
     return FSP_OK;
 }
 
-fsp_status fsp_mat_internal(fsp_mat A, MatImpl **impl)
+__attribute__((visibility("hidden"))) fsp_status fsp_mat_internal(fsp_mat A, MatImpl **impl)
 {
     if (!A || !impl) return fail(FSP_ERR_INVALID, "NULL matrix");
     *impl = &A->impl;

@@ -7,47 +7,61 @@
 namespace fsp {
 
 constexpr int kP10Lo = -300, kP10Hi = 300;
+constexpr int kElimChunk = 8;  // register chunk of the elimination's row updates
 
 // C = A * B for column-major N x N matrices in LDS.  Each thread computes a
-// 4 x 4 block of C (4 + 4 LDS reads per 16 products); every element is still
-// sum_l A(i, l) B(l, j) accumulated in increasing l, the oracle's order.
-__device__ inline void matmul_lds(int N, const double *A, const double *Bm, double *C)
+// BS x BS block of C from BS + BS LDS reads per BS^2 products; every element
+// is still sum_l A(i, l) B(l, j) accumulated in increasing l, the oracle's
+// order, so the block size changes speed only.
+template <int BS>
+__device__ inline void matmul_lds_bs(int N, const double *A, const double *Bm, double *C)
 {
-    const int nb = (N + 3) / 4;
+    const int nb = (N + BS - 1) / BS;
     for (int b = threadIdx.x; b < nb * nb; b += blockDim.x) {
-        const int i0 = (b % nb) * 4, j0 = (b / nb) * 4;
-        double acc[4][4];
+        const int i0 = (b % nb) * BS, j0 = (b / nb) * BS;
+        double acc[BS][BS];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < BS; ++a)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+            for (int c = 0; c < BS; ++c) acc[a][c] = 0.0;
         for (int l = 0; l < N; ++l) {
-            double av[4], bv[4];
+            double av[BS], bv[BS];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) av[a] = (i0 + a < N) ? A[(i0 + a) + l * N] : 0.0;
+            for (int a = 0; a < BS; ++a) av[a] = (i0 + a < N) ? A[(i0 + a) + l * N] : 0.0;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) bv[c] = (j0 + c < N) ? Bm[l + (j0 + c) * N] : 0.0;
+            for (int c = 0; c < BS; ++c) bv[c] = (j0 + c < N) ? Bm[l + (j0 + c) * N] : 0.0;
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < BS; ++a)
 #pragma unroll
-                for (int c = 0; c < 4; ++c) acc[a][c] += av[a] * bv[c];
+                for (int c = 0; c < BS; ++c) acc[a][c] += av[a] * bv[c];
         }
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < BS; ++a)
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+            for (int c = 0; c < BS; ++c)
                 if (i0 + a < N && j0 + c < N) C[(i0 + a) + (j0 + c) * N] = acc[a][c];
     }
+}
+
+// 2 x 2 blocks when the workgroup has a thread for most of them (more
+// waves in flight), 4 x 4 otherwise (one wave).
+__device__ inline void matmul_lds(int N, const double *A, const double *Bm, double *C)
+{
+    const int nb2 = (N + 1) / 2;
+    if ((int)blockDim.x * 2 >= nb2 * nb2)
+        matmul_lds_bs<2>(N, A, Bm, C);
+    else
+        matmul_lds_bs<4>(N, A, Bm, C);
 }
 
 // --------------------------------------------------------------- dense expm
 // Golub & Van Loan, Matrix Computations, Alg. 11.3.1 (diagonal Pade q = 6,
 // scaling by 2^-j with j = max(0, 1 + floor(log2 |A|_inf)), Gaussian
 // elimination with partial pivoting, j squarings), column-major N x N
-// (N <= 64) in LDS, one workgroup (one wave is enough and fastest).  Same
+// (N <= 64) in LDS, one workgroup of any size.  Same
 // operation order as the oracle's orc_expm, so the two agree bit for bit on
 // equal input.  Result in Nm.
-__device__ void expm_block(int N, double *A, double *X, double *T, double *Nm, double *Dm,
+__device__ inline void expm_block(int N, double *A, double *X, double *T, double *Nm, double *Dm,
                            double *red)
 {
     (void)red;
@@ -100,20 +114,19 @@ __device__ void expm_block(int N, double *A, double *X, double *T, double *Nm, d
     }
     // solve Dm F = Nm by Gaussian elimination with partial pivoting
     for (int k = 0; k < N; ++k) {
-        if (tid < kWave) {
-            // first row of maximal |Dm(i, k)|, i >= k (the oracle's choice)
-            double v = (tid >= k && tid < N) ? fabs(Dm[tid + k * N]) : -1.0;
-            int idx = tid;
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(v, off, kWave);
-                const int oi = __shfl_xor(idx, off, kWave);
-                if (ov > v || (ov == v && oi < idx)) {
-                    v = ov;
-                    idx = oi;
+        if (tid == 0) {
+            // first row of maximal |Dm(i, k)|, i >= k (the oracle's choice);
+            // one thread: N - k independent LDS loads, no shuffle round trips
+            int p = k;
+            double best = fabs(Dm[k + k * N]);
+            for (int i = k + 1; i < N; ++i) {
+                const double v = fabs(Dm[i + k * N]);
+                if (v > best) {
+                    best = v;
+                    p = i;
                 }
             }
-            if (tid == 0) piv_sh = idx;
+            piv_sh = p;
         }
         __syncthreads();
         const int p = piv_sh;
@@ -128,14 +141,36 @@ __device__ void expm_block(int N, double *A, double *X, double *T, double *Nm, d
             }
             __syncthreads();
         }
-        // one thread per row below the pivot: multiplier, row update against
-        // the (unchanged, broadcast) pivot row, multiplier stored in place
+        // Row updates below the pivot by (row, column chunk) pairs: each pair
+        // forms its row's multiplier (the same division in every pair, so the
+        // same value), loads its chunk of the unchanged pivot row and of its
+        // own row, then stores the updated chunk.  Consecutive pairs own
+        // consecutive rows (conflict-free LDS).  The multipliers are not kept:
+        // only the upper triangle is read afterwards.
         const double pv = Dm[k + k * N];
-        for (int i = k + 1 + tid; i < N; i += blockDim.x) {
+        const int rows = N - k - 1;
+        const int chunks = (N + kElimChunk - 1) / kElimChunk;
+        for (int e = tid; e < rows * chunks; e += blockDim.x) {
+            const int i = k + 1 + e % rows;
+            const int j0 = (e / rows) * kElimChunk;
             const double l = Dm[i + k * N] / pv;
-            for (int j = k + 1; j < N; ++j) Dm[i + j * N] -= l * Dm[k + j * N];
-            for (int j = 0; j < N; ++j) Nm[i + j * N] -= l * Nm[k + j * N];
-            Dm[i + k * N] = l;
+            double pd[kElimChunk], pn[kElimChunk], rd[kElimChunk], rn[kElimChunk];
+#pragma unroll
+            for (int q = 0; q < kElimChunk; ++q) {
+                const int j = min(j0 + q, N - 1);
+                pd[q] = Dm[k + j * N];
+                pn[q] = Nm[k + j * N];
+                rd[q] = Dm[i + j * N];
+                rn[q] = Nm[i + j * N];
+            }
+#pragma unroll
+            for (int q = 0; q < kElimChunk; ++q) {
+                const int j = j0 + q;
+                if (j < N) {
+                    if (j > k) Dm[i + j * N] = rd[q] - l * pd[q];
+                    Nm[i + j * N] = rn[q] - l * pn[q];
+                }
+            }
         }
         __syncthreads();
     }

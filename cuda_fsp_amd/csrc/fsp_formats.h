@@ -1,7 +1,8 @@
 // fsp_formats.h -- generator storage formats and their row kernels.
 //
-// Every row kernel works on tiles of kBlock consecutive rows, one thread per
-// row, grid-stride over tiles.  An "x accessor" X supplies input values:
+// Every row kernel works on tiles of consecutive rows (kBlock, or a larger
+// compile-time height for the CSR apply), one thread per row, grid-stride
+// over tiles.  An "x accessor" X supplies input values:
 // x(c) returns the FP64 value of input entry c (a plain load, or a scaled
 // and rounded Krylov vector).  Each tile function returns the row sum for
 // row tile*kBlock + threadIdx.x and adds the thread's share of the sink row
@@ -30,22 +31,22 @@ struct CsrF {
     const int *__restrict__ sink_cols;
     const double *__restrict__ sink_vals;
     uint32_t n;
-    int cap;  // LDS product slots per chunk (>= kBlock)
+    int cap;  // LDS product slots per chunk (>= rows per tile)
 };
 
-constexpr int kCsrPer = 16;  // LDS products per thread per chunk (cap <= kCsrPer * kBlock)
+constexpr int kCsrPer = 16;  // LDS products per thread per chunk (cap <= kCsrPer * tile rows)
 
 // Phase 1 streams the tile's nonzeros (coalesced) into LDS as products: all
 // of a thread's column indices and values of the chunk are loaded first
 // (independent, non-temporal: they are read once), then all its gathers of
 // x, so a chunk costs two memory round trips; phase 2 lets each thread add
 // its row's products in column order.
-template <typename V, class X>
+template <typename V, class X, int TB = kBlock>
 __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
                                   double *prod, dd &sink)
 {
-    const uint32_t r0 = tile * kBlock;
-    const uint32_t r1 = min(r0 + (uint32_t)kBlock, A.n);
+    const uint32_t r0 = tile * TB;
+    const uint32_t r1 = min(r0 + (uint32_t)TB, A.n);
     const uint32_t i = r0 + threadIdx.x;
     const int k0 = A.rp[r0], k1 = A.rp[r1];
     const int kb = (i < A.n) ? A.rp[i] : k1;
@@ -58,7 +59,7 @@ __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
         V vl[kCsrPer];
 #pragma unroll
         for (int q = 0; q < kCsrPer; ++q) {
-            const int k = c0 + (int)threadIdx.x + q * kBlock;
+            const int k = c0 + (int)threadIdx.x + q * TB;
             cl[q] = 0;
             vl[q] = (V)0;
             if (k < c1) {
@@ -68,7 +69,7 @@ __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
         }
 #pragma unroll
         for (int q = 0; q < kCsrPer; ++q) {
-            const int k = c0 + (int)threadIdx.x + q * kBlock;
+            const int k = c0 + (int)threadIdx.x + q * TB;
             if (k < c1) prod[k - c0] = (double)vl[q] * x(cl[q]);
         }
         __syncthreads();
@@ -78,7 +79,7 @@ __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
     if (A.diag && i < A.n) y = A.diag[i] * x(i) + y;
     if (A.sink_off) {
         const int s0 = A.sink_off[r0], s1 = A.sink_off[r1];
-        for (int k = s0 + (int)threadIdx.x; k < s1; k += kBlock)
+        for (int k = s0 + (int)threadIdx.x; k < s1; k += TB)
             sink = dd_add_prod(sink, A.sink_vals[k], x(A.sink_cols[k]));
     }
     return y;
@@ -210,8 +211,9 @@ __device__ inline double kron_row_t(const KronF<V> &A, uint32_t j, const X &x, c
 
 static constexpr size_t kLdsTableMax = 96 * 1024;
 
+// tb: rows per tile (threads per workgroup) of the kernel using the view
 template <typename V>
-inline CsrF<V> csr_view(const MatImpl &M)
+inline CsrF<V> csr_view(const MatImpl &M, int tb = kBlock)
 {
     CsrF<V> A;
     A.rp = M.rp;
@@ -222,9 +224,9 @@ inline CsrF<V> csr_view(const MatImpl &M)
     A.sink_cols = M.sink_cols;
     A.sink_vals = (const double *)M.sink_vals;
     A.n = (uint32_t)M.n;
-    int cap = kBlock * (M.max_row > 1 ? M.max_row : 1);
-    // at most kCsrPer products per thread per chunk (32 KB of LDS)
-    A.cap = cap > kCsrPer * kBlock ? kCsrPer * kBlock : cap;
+    int cap = tb * (M.max_row > 1 ? M.max_row : 1);
+    // at most kCsrPer products per thread per chunk
+    A.cap = cap > kCsrPer * tb ? kCsrPer * tb : cap;
     return A;
 }
 

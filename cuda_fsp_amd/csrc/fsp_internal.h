@@ -124,7 +124,7 @@ struct MatImpl {
     int *sink_cols = nullptr;        // sink row (state indices, ascending)
     void *sink_vals = nullptr;
     int64_t nnz_sink = 0;
-    int *sink_block_ptr = nullptr;   // per kBlock rows: first sink entry
+    int *sink_block_ptr = nullptr;   // first sink entry of rows >= i (n + 1 entries)
     // HYB
     void *diag = nullptr;
     void *slot_vals = nullptr;
@@ -164,3 +164,8 @@ fsp_status scan_counts(const int *d_counts, int *d_out, int64_t n, int64_t *h_to
 fsp_status vec_sum(const double *d_x, int64_t n, double *h_sum, hipStream_t s);
 
 }  // namespace fsp
+
+// Internal view of a matrix handle, shared by the library's sources and not
+// exported from the shared object.
+extern "C" __attribute__((visibility("hidden"))) fsp_status fsp_mat_internal(fsp_mat A,
+                                                                          fsp::MatImpl **impl);

@@ -693,7 +693,7 @@ static fsp_status enqueue_step(ExpvImpl &E, double *d_v)
         FSP_HIP(hipGetLastError());
     }
     const size_t lds = sizeof(double) * 5 * (size_t)M2 * M2;
-    k_expm_control<<<1, kWave, lds, E.s>>>(E.d_st, m, E.max_reject);
+    k_expm_control<<<1, kBlock, lds, E.s>>>(E.d_st, m, E.max_reject);
     k_solution<B><<<E.G, kBlock, 0, E.s>>>(E.d_st, E.V64, (const B *)(E.mixed ? (void *)E.V32 : (void *)E.V64), nv, d_v);
     FSP_HIP(hipGetLastError());
     return FSP_OK;
@@ -761,7 +761,6 @@ struct fsp_expv_s {
 extern "C" fsp_status fsp_expv_create(fsp_mat A, fsp_matvec_cb cb, void *cb_ctx, int64_t n,
                                       double t_final, int m, double tol, int iop, int q_iop,
                                       double anorm, void *stream, fsp_expv *out);
-extern "C" fsp_status fsp_mat_internal(fsp_mat A, fsp::MatImpl **impl);
 
 extern "C" fsp_status fsp_expv_create(fsp_mat A, fsp_matvec_cb cb, void *cb_ctx, int64_t n,
                                       double t_final, int m, double tol, int iop, int q_iop,
@@ -973,7 +972,7 @@ extern "C" fsp_status fsp_expm_dense(int N, const double *d_A, double *d_F, void
     using namespace fsp;
     if (N < 1 || N > kMaxKrylov + 2) return fail(FSP_ERR_UNSUPPORTED, "N must be in 1..52");
     const size_t lds = sizeof(double) * 5 * (size_t)N * N;
-    k_expm_dense<<<1, kWave, lds, (hipStream_t)stream>>>(N, d_A, d_F);
+    k_expm_dense<<<1, kBlock, lds, (hipStream_t)stream>>>(N, d_A, d_F);
     FSP_HIP(hipGetLastError());
     return FSP_OK;
 }

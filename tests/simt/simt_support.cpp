@@ -3,7 +3,7 @@
 // C ABI, so fsp_batch_* report errors through fsp_last_error().
 #include <string>
 
-#include "fsp_internal.h"
+#include "fsp_batch_impl.h"
 
 namespace fsp {
 
@@ -16,6 +16,19 @@ fsp_status fail(fsp_status st, const std::string &msg)
     set_error(msg);
     return st;
 }
+
+// The stream scheduler drives the HIP runtime's streams and the
+// single-instance engine, which this CPU build does not contain; the
+// simulated device always takes the persistent-kernel path.
+fsp_status batch_solve_streams(BatchImpl &)
+{
+    return fail(FSP_ERR_UNSUPPORTED, "no stream scheduler in the CPU build");
+}
+fsp_status batch_autotune_streams(BatchImpl &, double *, int *)
+{
+    return fail(FSP_ERR_UNSUPPORTED, "no stream scheduler in the CPU build");
+}
+void batch_streams_free(BatchImpl &) {}
 
 }  // namespace fsp
 
