@@ -74,11 +74,20 @@ __device__ inline void bput(double *V64, B *Vb, size_t nv, int k, int i, double 
         Vb[(size_t)(k - 1) * nv + i] = (B)v;
 }
 
+// Krylov vector j = in * sig as stored: rounded to B for j >= 1, FP64 for
+// V_0 (the scaled solution vector) with any B.
+template <typename B>
+__device__ inline double bscale(double v, double sig, int j)
+{
+    return j > 0 ? (double)(B)(v * sig) : v * sig;
+}
+
 template <typename B>
 struct BX {
     const double *__restrict__ in;
     double sig;
-    __device__ double operator()(uint32_t c) const { return (double)(B)(in[c] * sig); }
+    int j;
+    __device__ double operator()(uint32_t c) const { return bscale<B>(in[c], sig, j); }
 };
 
 template <int BS, typename B, int NKB, int DMAX>
@@ -155,7 +164,7 @@ __device__ void solve_one(const BatchArgs &a, BatchInst &I, double *sm)
             if (!last && a.iop) istart = (j >= a.q_iop - 1) ? j - a.q_iop + 1 : 0;
             const double *in = (j == 0) ? p : W[j & 1];
             double *out = W[(j + 1) & 1];
-            const BX<B> X{in, sig};
+            const BX<B> X{in, sig, j};
             const int nk = last ? 1 : j - istart + 1;
             dd acc[NKB];
 #pragma unroll
@@ -181,7 +190,7 @@ __device__ void solve_one(const BatchArgs &a, BatchInst &I, double *sm)
             double ys = 0.0, xs = 0.0;
             if (I.with_sink) {
                 ys = dd_value(bsum_dd<BS>(sink, red));
-                xs = (double)(B)(in[n] * sig);
+                xs = bscale<B>(in[n], sig, j);
             }
 #pragma unroll
             for (int kk = 0; kk < NKB; ++kk) {
@@ -216,6 +225,41 @@ __device__ void solve_one(const BatchArgs &a, BatchInst &I, double *sm)
                 const double w = -1.0 * accv + out[i];
                 out[i] = w;
                 nr = dd_add_prod(nr, w, w);
+            }
+            if (!std::is_same<B, double>::value) {
+                // FP32 basis: a second Gram-Schmidt pass.  With one pass the
+                // loss of orthogonality compounds from the FP32 rounding level
+                // by the factor |w| / h(j+1, j) per step and can reach 1
+                // within m steps; the second pass keeps it at that level.
+                // Coefficients h2 in EA (free until the error estimate).
+                dd acc2[NKB];
+#pragma unroll
+                for (int kk = 0; kk < NKB; ++kk) acc2[kk] = dd_zero();
+                for (int i = tid; i < nv; i += BS) {
+                    const double w = out[i];
+#pragma unroll
+                    for (int kk = 0; kk < NKB; ++kk)
+                        if (kk < nk) acc2[kk] = dd_add_prod(acc2[kk], w, bget<B>(V64, Vb, nv, istart + kk, i));
+                }
+#pragma unroll
+                for (int kk = 0; kk < NKB; ++kk) {
+                    if (kk < nk) {
+                        const double h2 = dd_value(bsum_dd<BS>(acc2[kk], red));
+                        if (tid == 0) {
+                            EA[kk] = h2;
+                            H[(istart + kk) + j * M2] += h2;
+                        }
+                    }
+                }
+                __syncthreads();
+                nr = dd_zero();
+                for (int i = tid; i < nv; i += BS) {
+                    double accv = 0.0;
+                    for (int k = istart; k <= j; ++k) accv += bget<B>(V64, Vb, nv, k, i) * EA[k - istart];
+                    const double w = -1.0 * accv + out[i];
+                    out[i] = w;
+                    nr = dd_add_prod(nr, w, w);
+                }
             }
             const double s = sqrt(dd_value(bsum_dd<BS>(nr, red)));
             if (s < btol) {  // happy breakdown

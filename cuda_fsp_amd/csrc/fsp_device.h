@@ -224,6 +224,16 @@ __device__ inline TileWalk tile_walk(uint32_t tiles)
     return TileWalk{start + l, end, cnt};
 }
 
+// Workgroups are dispatched to the XCDs round-robin (block b on XCD
+// b % kXcds).  Renumbers them so that each XCD's blocks cover one contiguous
+// range of [0, G), so neighbouring tiles (and the vector entries they
+// gather) share that XCD's L2.  A bijection on [0, G).
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t G)
+{
+    const uint32_t q = G / kXcds, r = G % kXcds, x = b % kXcds, s = b / kXcds;
+    return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
+}
+
 inline int grid_for(int64_t tiles, int cap)
 {
     int64_t g = tiles < cap ? tiles : cap;

@@ -34,14 +34,14 @@ struct CsrF {
     int cap;  // LDS product slots per chunk (>= rows per tile)
 };
 
-constexpr int kCsrPer = 16;  // LDS products per thread per chunk (cap <= kCsrPer * tile rows)
+constexpr int kCsrPer = 16;  // default LDS products per thread per chunk (cap <= PER * tile rows)
 
 // Phase 1 streams the tile's nonzeros (coalesced) into LDS as products: all
 // of a thread's column indices and values of the chunk are loaded first
 // (independent, non-temporal: they are read once), then all its gathers of
 // x, so a chunk costs two memory round trips; phase 2 lets each thread add
 // its row's products in column order.
-template <typename V, class X, int TB = kBlock>
+template <typename V, class X, int TB = kBlock, int PER = kCsrPer>
 __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
                                   double *prod, dd &sink)
 {
@@ -55,10 +55,10 @@ __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
     for (int c0 = k0; c0 < k1; c0 += A.cap) {
         const int c1 = min(c0 + A.cap, k1);
         __syncthreads();
-        int cl[kCsrPer];
-        V vl[kCsrPer];
+        int cl[PER];
+        V vl[PER];
 #pragma unroll
-        for (int q = 0; q < kCsrPer; ++q) {
+        for (int q = 0; q < PER; ++q) {
             const int k = c0 + (int)threadIdx.x + q * TB;
             cl[q] = 0;
             vl[q] = (V)0;
@@ -68,7 +68,7 @@ __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
             }
         }
 #pragma unroll
-        for (int q = 0; q < kCsrPer; ++q) {
+        for (int q = 0; q < PER; ++q) {
             const int k = c0 + (int)threadIdx.x + q * TB;
             if (k < c1) prod[k - c0] = (double)vl[q] * x(cl[q]);
         }
@@ -211,9 +211,10 @@ __device__ inline double kron_row_t(const KronF<V> &A, uint32_t j, const X &x, c
 
 static constexpr size_t kLdsTableMax = 96 * 1024;
 
-// tb: rows per tile (threads per workgroup) of the kernel using the view
+// tb: rows per tile (threads per workgroup) of the kernel using the view;
+// per: its csr_tile PER
 template <typename V>
-inline CsrF<V> csr_view(const MatImpl &M, int tb = kBlock)
+inline CsrF<V> csr_view(const MatImpl &M, int tb = kBlock, int per = kCsrPer)
 {
     CsrF<V> A;
     A.rp = M.rp;
@@ -225,8 +226,8 @@ inline CsrF<V> csr_view(const MatImpl &M, int tb = kBlock)
     A.sink_vals = (const double *)M.sink_vals;
     A.n = (uint32_t)M.n;
     int cap = tb * (M.max_row > 1 ? M.max_row : 1);
-    // at most kCsrPer products per thread per chunk
-    A.cap = cap > kCsrPer * tb ? kCsrPer * tb : cap;
+    // at most per products per thread per chunk
+    A.cap = cap > per * tb ? per * tb : cap;
     return A;
 }
 

@@ -10,6 +10,11 @@
 //   k_arn_dots   : H(k, j) = sum of partials (fixed order), sink element
 //   k_arn_update : w_{j+1} -= V h (gemv N of :79-81), partial |w|^2
 //   k_arn_norm   : s = |w|, happy breakdown test, H(j+1, j) = s, sigma = 1/s
+// With an FP32 basis (mixed precision) the update is a second Gram-Schmidt
+// pass: k_arn_update_dots (w -= V h, partial dots V_k . w), k_reorth_fin
+// (h2, H(k, j) += h2), k_arn_update with h2.  One pass, as the reference
+// does in FP64, lets the loss of orthogonality of an FP32 basis compound by
+// |w| / h(j+1, j) per step until the basis is degenerate.
 // then k_expm_control (one workgroup: Pade expm of tau*H, error estimate,
 // reject loop, step-size update) and k_solution (v = beta V F0).
 // Reductions are deterministic: per-wave butterflies, per-block partials in
@@ -20,27 +25,13 @@
 #include <cstdlib>
 #include <cstring>
 
-#include "fsp_expm.h"
-#include "fsp_formats.h"
+#include "fsp_krylov_dev.h"
 
 namespace fsp {
 
 
 constexpr long long kTauLogCap = 1 << 16;
 
-struct KState {
-    double beta, btol, t_now, t_new, t_final, tau, xm, err_loc, avnorm, fact;
-    double tol, anorm;
-    double tau_last;
-    double sigma[kMaxKrylov + 2];
-    double F0[kMaxKrylov + 2];
-    double H[(kMaxKrylov + 2) * (kMaxKrylov + 2)];
-    double p10[kP10Hi - kP10Lo + 1];   // correctly rounded 1e-300 .. 1e300
-    double *tau_log;                   // accepted step sizes, by step index
-    long long tau_cap;
-    int mb, k1, broke, active, mx, status, pad0, pad1;
-    long long i_step, n_reject, n_breakdown, n_matvec;
-};
 
 struct ExpvImpl {
     MatImpl *A = nullptr;
@@ -74,34 +65,6 @@ struct ExpvImpl {
 namespace {
 
 // ------------------------------------------------------------- accessors
-template <typename B>
-struct KryX {
-    const double *__restrict__ in;
-    double sig;
-    __device__ double operator()(uint32_t c) const { return (double)(B)(in[c] * sig); }
-};
-
-template <typename B>
-__device__ inline double basis(const double *__restrict__ V64, const B *__restrict__ Vb,
-                               size_t nv, int k, uint32_t i)
-{
-    if (std::is_same<B, double>::value) return V64[(size_t)k * nv + i];
-    return k == 0 ? V64[i] : (double)Vb[(size_t)(k - 1) * nv + i];
-}
-
-template <typename B>
-__device__ inline void store_basis(double *__restrict__ V64, B *__restrict__ Vb, size_t nv,
-                                   int k, uint32_t i, double v)
-{
-    if (std::is_same<B, double>::value) {
-        V64[(size_t)k * nv + i] = v;
-    } else if (k == 0) {
-        V64[i] = v;
-    } else {
-        Vb[(size_t)(k - 1) * nv + i] = (B)v;
-    }
-}
-
 struct ArnArgs {
     KState *st;
     const double *in;     // w_j (or v for j = 0), unscaled
@@ -239,7 +202,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_csr(CsrF<V> A, ArnArgs a)
     double *red = smem;
     double *prod = red + kArnRedLen;
     dd nrm = dd_zero();
-    const KryX<B> X{a.in, a.st->sigma[a.j]};
+    const KryX<B> X{a.in, a.st->sigma[a.j], a.j};
     dd sink = dd_zero();
     for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
         const double y = csr_tile<V>(A, w.t, X, prod, sink);
@@ -256,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
     extern __shared__ double smem[];
     double *red = smem;
     dd nrm = dd_zero();
-    const KryX<B> X{a.in, a.st->sigma[a.j]};
+    const KryX<B> X{a.in, a.st->sigma[a.j], a.j};
     dd sink = dd_zero();
     for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
         const uint32_t i = w.t * kBlock + threadIdx.x;
@@ -279,7 +242,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_kron(KronF<V> A, ArnArgs a, int 
         tab = tabl;
     }
     dd nrm = dd_zero();
-    const KryX<B> X{a.in, a.st->sigma[a.j]};
+    const KryX<B> X{a.in, a.st->sigma[a.j], a.j};
     dd sink = dd_zero();
     for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
         const uint32_t i = w.t * kBlock + threadIdx.x;
@@ -316,7 +279,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_dots(ArnArgs a, int with_sink, i
     double ys = 0.0, xs = 0.0;
     if (with_sink) {
         ys = dd_value(sum_dd_partials(a.partials + 2 * (size_t)nk * a.G, a.G, red));
-        xs = (double)(B)(a.in[a.n] * st->sigma[a.j]);
+        xs = kry_scale<B>(a.in[a.n], st->sigma[a.j], a.j);
     }
     const int k = blockIdx.x;
     dd sd = sum_dd_partials(a.partials + 2 * (size_t)k * a.G, a.G, red);
@@ -345,18 +308,20 @@ __global__ void __launch_bounds__(kBlock) k_arn_dots(ArnArgs a, int with_sink, i
     }
 }
 
-// w_{j+1} -= V[istart..j] H(istart..j, j); partial |w|^2.
+// w_{j+1} -= V[istart..j] h; partial |w|^2.  h = H(istart..j, j), or the
+// second-pass coefficients h2 (pass 1).
 template <typename B>
-__global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2)
+__global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2, int pass)
 {
     if (arn_skip<B>(a)) return;
     __shared__ double red[kRedLen];
     const KState *st = a.st;
+    const double *h = pass ? st->h2 : st->H + (size_t)a.j * M2;
     dd nrm = dd_zero();
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += gridDim.x * kBlock) {
         double acc = 0.0;
         for (int k = a.istart; k <= a.j; ++k)
-            acc += basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) * st->H[k + a.j * M2];
+            acc += basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) * h[k];
         const double w = -1.0 * acc + a.out[i];
         a.out[i] = w;
         nrm = dd_add_prod(nrm, w, w);
@@ -365,6 +330,64 @@ __global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2)
     if (threadIdx.x == 0) {
         a.partials[2 * blockIdx.x] = t.s;
         a.partials[2 * blockIdx.x + 1] = t.c;
+    }
+}
+
+// First pass of the FP32-basis update: w_{j+1} -= V[istart..j] H(., j) over
+// the whole vector (sink element included), then the partial dots
+// V_k . w_{j+1} for the second pass, kDotChunk at a time over the rows this
+// thread updated (same mapping, so its own writes are re-read).  Partials of
+// dot k at 2 (k G + b), as the Arnoldi kernels store them.
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_arn_update_dots(ArnArgs a, int M2)
+{
+    if (arn_skip<B>(a)) return;
+    __shared__ double red[kArnRedLen];
+    const KState *st = a.st;
+    const double *h = st->H + (size_t)a.j * M2;
+    const int nk = a.j - a.istart + 1;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += stride) {
+        double acc = 0.0;
+        for (int k = a.istart; k <= a.j; ++k)
+            acc += basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) * h[k];
+        a.out[i] = -1.0 * acc + a.out[i];
+    }
+    for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
+        dd acc[kDotChunk];
+#pragma unroll
+        for (int q = 0; q < kDotChunk; ++q) acc[q] = dd_zero();
+        for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += stride) {
+            const double w = a.out[i];
+            double vk[kDotChunk];
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q)
+                vk[q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + c0 + q, a.j), i);
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q)
+                if (c0 + q < nk) acc[q] = dd_add_prod(acc[q], w, vk[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < kDotChunk; ++q)
+            if (c0 + q < nk) arn_put_wave(red, c0 + q, acc[q]);
+    }
+    arn_put_wave(red, nk, dd_zero());  // no separate sink slot
+    arn_store_partials(a, nk, red);
+}
+
+// Second-pass coefficients: block k sums the partials of dot k in fixed
+// order; h2(istart + k) = that sum, H(istart + k, j) += it.
+__global__ void __launch_bounds__(kBlock) k_reorth_fin(ArnArgs a, int M2)
+{
+    KState *st = a.st;
+    if (!st->active || st->broke) return;
+    __shared__ double red[kRedLen];
+    const int k = blockIdx.x;
+    const double s = dd_value(sum_dd_partials(a.partials + 2 * (size_t)k * a.G, a.G, red));
+    if (threadIdx.x == 0) {
+        const int kk = a.istart + k;
+        st->h2[kk] = s;
+        st->H[kk + a.j * M2] += s;
     }
 }
 
@@ -404,94 +427,20 @@ __global__ void __launch_bounds__(kBlock) k_norm_partials(const double *__restri
     }
 }
 
-// Step prologue (src/KryExpvFSP.cu:41-65): beta = |v|, first step size,
-// tau = min(t_final - t_now, t_new).  Marks the step inactive when t_final
-// is reached, so steps launched ahead of the host check do nothing.
+// Step prologue (src/KryExpvFSP.cu:41-65) for one state: kry_step_begin.
 __global__ void __launch_bounds__(kBlock) k_step_begin(KState *st, const double *partials, int G,
                                                       int m, int M2)
 {
     __shared__ double red[kRedLen];
-    __shared__ int act;
-    if (threadIdx.x == 0) act = st->t_now < st->t_final;
-    __syncthreads();
-    if (!act) {
-        if (threadIdx.x == 0) st->active = 0;
-        return;
-    }
-    const double s = dd_value(sum_dd_partials(partials, G, red));
-    for (int k = threadIdx.x; k < M2 * M2; k += kBlock) st->H[k] = 0.0;
-    if (threadIdx.x == 0) {
-        st->active = 1;
-        st->beta = sqrt(s);
-        if (st->i_step == 0) {
-            st->xm = 1.0 / (double)m;
-            st->t_new = (1.0 / st->anorm) *
-                        pow((st->fact * st->tol) / (4.0 * st->beta * st->anorm), st->xm);
-            st->btol = st->anorm * st->tol;
-        }
-        st->mb = m;
-        st->tau = fmin(st->t_final - st->t_now, st->t_new);
-        st->broke = 0;
-        st->sigma[0] = 1.0 / st->beta;
-    }
+    // the host enforces max_nstep with its own message
+    kry_step_begin(st, partials, G, m, M2, 0x7fffffffffffffffLL, red);
 }
 
-// Error estimate, reject loop and step update (src/KryExpvFSP.cu:122-174).
+// Error estimate, reject loop and step update for one state: kry_control.
 __global__ void __launch_bounds__(kBlock) k_expm_control(KState *st, int m, int max_reject)
 {
-    if (!st->active) return;
     extern __shared__ double smem[];
-    const int M2 = m + 2;
-    const int NN = M2 * M2;
-    double *A = smem, *X = A + NN, *T = X + NN, *Nm = T + NN, *Dm = Nm + NN;
-    __shared__ double red[kBlock / kWave];
-    const int mb = st->mb, k1 = st->k1;
-    const double beta = st->beta, avnorm = st->avnorm, btol = st->btol, tol = st->tol;
-    double tau = st->tau, xm = st->xm, err = 0.0;
-    int ireject = 0;
-    while (ireject < max_reject) {
-        const int mx = mb + k1;
-        for (int e = threadIdx.x; e < NN; e += blockDim.x) A[e] = tau * st->H[e];
-        __syncthreads();
-        expm_block(M2, A, X, T, Nm, Dm, red);
-        if (k1 == 0) {
-            err = btol;
-            break;
-        }
-        const double phi1 = fabs(beta * Nm[(mx - 2) + 0 * M2]);
-        const double phi2 = fabs(beta * Nm[(mx - 1) + 0 * M2] * avnorm);
-        if (phi1 > phi2 * 10.0) {
-            err = phi2;
-            xm = 1.0 / (double)mx;
-        } else if (phi1 > phi2) {
-            err = (phi1 * phi2) / (phi1 - phi2);
-            xm = 1.0 / (double)mx;
-        } else {
-            err = phi1;
-            xm = 1.0 / (double)(mx - 1);
-        }
-        if (err <= 1.2 * tau * tol) break;
-        tau = 0.9 * tau * pow(tau * tol / err, xm);
-        tau = round_up_2sig(st->p10, tau);
-        ++ireject;
-        __syncthreads();
-    }
-    const int mx = mb + max(0, k1 - 1);
-    for (int k = threadIdx.x; k < M2; k += blockDim.x) st->F0[k] = (k < mx) ? Nm[k] : 0.0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        st->mx = mx;
-        st->n_reject += ireject;
-        if (ireject >= max_reject) st->status = FSP_ERR_NOT_CONVERGED;
-        st->err_loc = err;
-        st->xm = xm;
-        st->tau_last = tau;
-        if (st->tau_log && st->i_step < st->tau_cap) st->tau_log[st->i_step] = tau;
-        st->t_now = st->t_now + tau;
-        double tn = 0.9 * tau * pow(tau * tol / err, xm);
-        st->t_new = round_up_2sig(st->p10, tn);
-        st->i_step += 1;
-    }
+    kry_control(st, m, max_reject, smem);
 }
 
 // v = beta * V[:, 0:mx] F0 (src/KryExpvFSP.cu:160-166).
@@ -517,7 +466,7 @@ __global__ void __launch_bounds__(kBlock) k_cb_scale(ArnArgs a)
     if (!a.st->active || a.st->broke) return;
     const double sig = a.st->sigma[a.j];
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += gridDim.x * kBlock) {
-        store_basis<B>(a.V64, (B *)a.Vb, a.nv, a.j, i, (double)(B)(a.in[i] * sig));
+        store_basis<B>(a.V64, (B *)a.Vb, a.nv, a.j, i, kry_scale<B>(a.in[i], sig, a.j));
         a.out[i] = 0.0;
     }
 }
@@ -688,7 +637,13 @@ static fsp_status enqueue_step(ExpvImpl &E, double *d_v)
         }
         FSP_HIP(hipGetLastError());
         if (last) break;
-        k_arn_update<B><<<E.G, kBlock, 0, E.s>>>(a, M2);
+        if (E.mixed) {
+            k_arn_update_dots<B><<<E.G, kBlock, 0, E.s>>>(a, M2);
+            k_reorth_fin<<<nk, kBlock, 0, E.s>>>(a, M2);
+            k_arn_update<B><<<E.G, kBlock, 0, E.s>>>(a, M2, 1);
+        } else {
+            k_arn_update<B><<<E.G, kBlock, 0, E.s>>>(a, M2, 0);
+        }
         k_arn_norm<<<1, kBlock, 0, E.s>>>(a, M2, E.G);
         FSP_HIP(hipGetLastError());
     }

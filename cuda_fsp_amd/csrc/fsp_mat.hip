@@ -508,15 +508,16 @@ __device__ inline void put_dd_partial(double *partials, dd sink, double *red)
     }
 }
 
-// Merged CSR apply: kCsrApplyRows rows per tile and workgroup (larger tiles
-// amortise the LDS round trip of csr_tile; measured 551 us vs 600 us for
-// 256-row tiles on the 16.7M-state chain), one tile per workgroup.
+// Merged CSR apply: one tile of kCsrApplyRows rows per workgroup (larger
+// tiles amortise the LDS round trip of csr_tile), at most kCsrApplyPer
+// products per thread per chunk, tiles renumbered per XCD (xcd_block).
+// Sink partial of tile t at 2 t, so the finalizer sums them in tile order.
 constexpr int kCsrApplyRows = 512;
+constexpr int kCsrApplyPer = 12;
 
 template <typename V>
 __global__ void __launch_bounds__(kCsrApplyRows) k_apply_csr(CsrF<V> A, PlainX X,
                                                              double *__restrict__ y, int accumulate,
-                                                             uint32_t tiles,
                                                              double *__restrict__ partials)
 {
     constexpr int NW = kCsrApplyRows / kWave;
@@ -524,12 +525,10 @@ __global__ void __launch_bounds__(kCsrApplyRows) k_apply_csr(CsrF<V> A, PlainX X
     double *prod = smem;
     double *red = smem + A.cap;
     dd sink = dd_zero();
-    for (TileWalk w = tile_walk(tiles); w.t < w.end; w.t += w.step) {
-        const uint32_t t = w.t;
-        const double v = csr_tile<V, PlainX, kCsrApplyRows>(A, t, X, prod, sink);
-        const uint32_t i = t * kCsrApplyRows + threadIdx.x;
-        if (i < A.n) y[i] = accumulate ? y[i] + v : v;
-    }
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
+    const double v = csr_tile<V, PlainX, kCsrApplyRows, kCsrApplyPer>(A, t, X, prod, sink);
+    const uint32_t i = t * kCsrApplyRows + threadIdx.x;
+    if (i < A.n) y[i] = accumulate ? y[i] + v : v;
     if (A.sink_off) {
         // block sum of the sink dd over NW waves
         const dd ws = wave_sum_dd(sink);
@@ -542,8 +541,8 @@ __global__ void __launch_bounds__(kCsrApplyRows) k_apply_csr(CsrF<V> A, PlainX X
         if (threadIdx.x == 0) {
             dd s = dd_zero();
             for (int q = 0; q < NW; ++q) s = dd_merge(s, dd{red[2 * q], red[2 * q + 1]});
-            partials[2 * blockIdx.x] = s.s;
-            partials[2 * blockIdx.x + 1] = s.c;
+            partials[2 * t] = s.s;
+            partials[2 * t + 1] = s.c;
         }
     }
 }
@@ -666,20 +665,20 @@ fsp_status mat_apply(MatImpl &M, const double *d_x, double *d_y, int accumulate,
     const bool mixed = M.prec == FSP_MIXED;
     if (M.format == FSP_CSR) {
         // one 512-row tile per workgroup (the partials hold 64 slots per
-        // 256-row tile, so any grid of 512-row tiles fits)
+        // 256-row tile, so one slot per 512-row tile fits)
         const uint32_t ct = (uint32_t)((M.n + kCsrApplyRows - 1) / kCsrApplyRows);
         const int cgrid = grid_for(ct, 1 << 30);
         const size_t lds_red = 2 * kCsrApplyRows / kWave;
         if (mixed) {
-            CsrF<float> A = csr_view<float>(M, kCsrApplyRows);
+            CsrF<float> A = csr_view<float>(M, kCsrApplyRows, kCsrApplyPer);
             const size_t lds = sizeof(double) * ((size_t)A.cap + lds_red);
             k_apply_csr<float><<<cgrid, kCsrApplyRows, lds, s>>>(A, PlainX{d_x}, d_y, accumulate,
-                                                                 ct, M.partials);
+                                                                 M.partials);
         } else {
-            CsrF<double> A = csr_view<double>(M, kCsrApplyRows);
+            CsrF<double> A = csr_view<double>(M, kCsrApplyRows, kCsrApplyPer);
             const size_t lds = sizeof(double) * ((size_t)A.cap + lds_red);
             k_apply_csr<double><<<cgrid, kCsrApplyRows, lds, s>>>(A, PlainX{d_x}, d_y, accumulate,
-                                                                  ct, M.partials);
+                                                                  M.partials);
         }
         FSP_HIP(hipGetLastError());
         if (M.with_sink) {

@@ -121,3 +121,20 @@ def test_simt_lane_order_independent(simlib):
         outs.append(np.load(path))
         os.remove(path)
     assert np.array_equal(outs[0], outs[1])
+
+
+# Rates under which one classical Gram-Schmidt pass over an FP32 basis loses
+# orthogonality completely within 30 Arnoldi steps from a point mass (the loss
+# grows about 2.5x per step from the FP32 rounding level), so the Hessenberg
+# exponential overflows: the mixed form must reorthogonalize.
+TOGGLE_CGS_RATES = (7.838583757547972, 0.5659944527754909, 13.088930136958103, 6.072545924918103)
+
+
+def test_simt_batch_mixed_reorthogonalizes(sim):
+    net = M.toggle_switch((20, 20), rates=TOGGLE_CGS_RATES)
+    B64 = capi.Batch([net], [4.0], precision="f64")
+    B32 = capi.Batch([net], [4.0], precision="mixed")
+    B64.solve()
+    B32.solve()
+    assert B64.stats(0)[2] == 0 and B32.stats(0)[2] == 0
+    assert np.abs(B32.solution(0) - B64.solution(0)).sum() <= 1e-6
