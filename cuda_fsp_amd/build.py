@@ -23,7 +23,7 @@ HIP_LIB = os.path.join(LIBDIR, "libfsp_hip.so")
 CPP_LIB = os.path.join(LIBDIR, "libcufsp.so")
 
 HIP_SOURCES = ["fsp_capi.hip", "fsp_model.hip", "fsp_mat.hip", "fsp_krylov.hip",
-               "fsp_batch.hip", "fsp_batch_streams.hip", "fsp_bdf.hip",
+               "fsp_batch.hip", "fsp_batch_lockstep.hip", "fsp_batch_streams.hip", "fsp_bdf.hip",
                "fsp_adjoint.hip"]
 CPP_SOURCES = ["cuFSP.cpp", "cme.cpp"]
 

@@ -56,7 +56,10 @@ class FspBatchOptions(C.Structure):
     _fields_ = [("m", C.c_int), ("tol", C.c_double), ("anorm", C.c_double),
                 ("precision", C.c_int), ("iop", C.c_int), ("q_iop", C.c_int),
                 ("block", C.c_int), ("wg_per_cu", C.c_int), ("max_reject", C.c_int),
-                ("max_nstep", C.c_int64)]
+                ("max_nstep", C.c_int64), ("path", C.c_int)]
+
+
+BATCH_PATHS = {"auto": 0, "lockstep": 1, "streams": 2, "persistent": 3}
 
 
 MATVEC_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p)
@@ -552,12 +555,15 @@ class Expv:
 
 
 class Batch:
-    """fsp_batch handle: many independent solves v_i <- exp(t_i A_i) v_i in
-    one persistent, graph-captured launch."""
+    """fsp_batch handle: many independent solves v_i <- exp(t_i A_i) v_i.
+    path: "auto", "lockstep" (one graph-captured batch step advances every
+    running instance), "streams" (single-solve engine, up to four instances
+    at a time) or "persistent" (one workgroup per instance; refused on GPU
+    devices)."""
 
     def __init__(self, nets, t_final, with_sink=None, m=30, tol=1e-8, anorm=1.0,
                  precision="f64", iop=False, q_iop=2, block=0, wg_per_cu=0,
-                 max_reject=0, max_nstep=0):
+                 max_reject=0, max_nstep=0, path="auto"):
         self.nets = list(nets)
         count = len(self.nets)
         arr = (FspModel * count)()
@@ -570,7 +576,7 @@ class Batch:
         ws = None if with_sink is None else np.ascontiguousarray([1 if s else 0 for s in with_sink], np.int32)
         self._keep = [coefs, tf, ws]
         opt = FspBatchOptions(m, tol, anorm, PRECISIONS[precision], int(iop), int(q_iop), int(block),
-                              int(wg_per_cu), int(max_reject), int(max_nstep))
+                              int(wg_per_cu), int(max_reject), int(max_nstep), BATCH_PATHS[path])
         h = C.c_void_p()
         check(lib().fsp_batch_create(C.cast(arr, C.c_void_p), coefs.ctypes.data, tf.ctypes.data,
                                      None if ws is None else ws.ctypes.data, count, C.byref(opt),

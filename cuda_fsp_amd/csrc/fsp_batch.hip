@@ -1,15 +1,17 @@
-// fsp_batch.hip -- many uneven FSP solves in one persistent launch.
+// fsp_batch.hip -- many uneven FSP solves: set-up, path dispatch, autotuner
+// and the persistent-kernel path.
 //
-// Each workgroup pulls instances from a work queue (largest estimated cost
-// first) and runs the whole Krylov expv of that instance inside the
-// workgroup: matrix-free Kronecker-sum SpMV from LDS-staged factor tables,
-// fused dot products, classical Gram-Schmidt, the Pade expm of the
-// Hessenberg matrix and the step control of KryExpvFSP::step
-// (src/KryExpvFSP.cu:38-182), with workgroup barriers only.  The launch
-// (queue reset + persistent kernel) is captured once in a HIP graph and
-// replayed per solve.  Workgroup size and workgroups per CU come from an
-// autotuner (fsp_batch_autotune) that times the candidates on the device and
-// caches the choice in a JSON file.
+// Persistent path: each workgroup pulls instances from a work queue
+// (largest estimated cost first) and runs the whole Krylov expv of that
+// instance inside the workgroup: matrix-free Kronecker-sum SpMV from
+// LDS-staged factor tables, fused dot products, classical Gram-Schmidt, the
+// Pade expm of the Hessenberg matrix and the step control of
+// KryExpvFSP::step (src/KryExpvFSP.cu:38-182), with workgroup barriers only.
+// The launch (queue reset + persistent kernel) is captured once in a HIP
+// graph and replayed per solve.  The lockstep path (fsp_batch_lockstep.hip)
+// and the stream scheduler (fsp_batch_streams.hip) are the GPU-device paths.
+// The autotuner (fsp_batch_autotune) times the candidates of the path in use
+// on the device and caches the choice in a JSON file.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -140,9 +142,25 @@ static fsp_status batch_graph(BatchImpl &B, hipStream_t s)
     return FSP_OK;
 }
 
+// Path that fsp_batch_solve takes: the requested one, else the autotuner's
+// choice, else LOCKSTEP on GPU devices and PERSISTENT elsewhere.
+static fsp_status batch_path(const BatchImpl &B, int *path)
+{
+    bool persistent = false;
+    FSP_TRY(persistent_allowed(&persistent));
+    int p = B.path_req;
+    if (p == FSP_BATCH_AUTO) p = B.path_tuned;
+    if (p == FSP_BATCH_AUTO) p = persistent ? FSP_BATCH_PERSISTENT : FSP_BATCH_LOCKSTEP;
+    if (p == FSP_BATCH_PERSISTENT && !persistent)
+        return fail(FSP_ERR_UNSUPPORTED, "the persistent batch kernel is not launched on GPU devices");
+    *path = p;
+    return FSP_OK;
+}
+
 static void batch_free(BatchImpl &B)
 {
     batch_streams_free(B);
+    batch_lockstep_free(B);
     if (B.gexec) (void)hipGraphExecDestroy(B.gexec);
     if (B.graph) (void)hipGraphDestroy(B.graph);
     if (B.cap) (void)hipStreamDestroy(B.cap);
@@ -209,6 +227,9 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
         if (opt->max_nstep > 0) B.max_nstep = opt->max_nstep;
         B.iop = opt->iop ? 1 : 0;
         if (opt->q_iop > 0) B.q_iop = opt->q_iop;
+        if (opt->path < FSP_BATCH_AUTO || opt->path > FSP_BATCH_PERSISTENT)
+            return bail(fail(FSP_ERR_INVALID, "unknown batch path"));
+        B.path_req = opt->path;
     }
     if (B.m < 1 || B.m > kMaxKrylov) return bail(fail(FSP_ERR_UNSUPPORTED, "m must be in 1..50"));
     int ndev = 0;
@@ -364,14 +385,13 @@ extern "C" fsp_status fsp_batch_solve(fsp_batch b, void *stream)
 {
     using namespace fsp;
     if (!b) return fail(FSP_ERR_INVALID, "NULL batch");
-    bool persistent = false;
-    FSP_TRY(persistent_allowed(&persistent));
-    if (!persistent) {
-        // stream scheduler: synchronous, on its own streams
-        FSP_HIP(hipStreamSynchronize((hipStream_t)stream));
-        return batch_solve_streams(b->impl);
-    }
-    return batch_graph(b->impl, (hipStream_t)stream);
+    int path = 0;
+    FSP_TRY(batch_path(b->impl, &path));
+    if (path == FSP_BATCH_PERSISTENT) return batch_graph(b->impl, (hipStream_t)stream);
+    // lockstep and stream scheduler: synchronous, on their own streams
+    FSP_HIP(hipStreamSynchronize((hipStream_t)stream));
+    if (path == FSP_BATCH_LOCKSTEP) return batch_solve_lockstep(b->impl, b->impl.ls_bpc);
+    return batch_solve_streams(b->impl);
 }
 
 extern "C" fsp_status fsp_batch_get_solution(fsp_batch b, int i, double *h_p, int64_t cap)
@@ -406,19 +426,38 @@ extern "C" fsp_status fsp_batch_get_config(fsp_batch b, int *block, int *wg_per_
 {
     using namespace fsp;
     if (!b) return fail(FSP_ERR_INVALID, "NULL batch");
-    bool persistent = false;
-    FSP_TRY(persistent_allowed(&persistent));
-    // stream scheduler: block 0, wg_per_cu = number of concurrent streams
-    if (block) *block = persistent ? b->impl.block : 0;
-    if (wg_per_cu) *wg_per_cu = persistent ? b->impl.wg_per_cu : b->impl.workers;
+    int path = 0;
+    FSP_TRY(batch_path(b->impl, &path));
+    const BatchImpl &B = b->impl;
+    // persistent: (workgroup size, workgroups per CU); lockstep: (-1,
+    // row-kernel workgroups per CU); streams: (0, concurrent streams)
+    const int bs = path == FSP_BATCH_PERSISTENT ? B.block : (path == FSP_BATCH_LOCKSTEP ? -1 : 0);
+    const int wg = path == FSP_BATCH_PERSISTENT ? B.wg_per_cu
+                                                : (path == FSP_BATCH_LOCKSTEP ? B.ls_bpc : B.workers);
+    if (block) *block = bs;
+    if (wg_per_cu) *wg_per_cu = wg;
     return FSP_OK;
 }
 
 // Autotuner: times each candidate configuration on the batch itself and
 // keeps the fastest: (workgroup size, workgroups per CU) for the persistent
-// kernel, the number of concurrent streams for the stream scheduler.  The
-// choice is cached in a JSON file keyed by device, instance count, longest
-// vector, species, m, precision and path.
+// kernel, row-kernel workgroups per CU for the lockstep path, the number of
+// concurrent streams for the stream scheduler.  The choice is cached in a
+// JSON file keyed by device, instance count, longest vector, species, m,
+// precision and path family.
+static void apply_tuned(BatchImpl &B, int path, int blk, int wg, int wk)
+{
+    if (path == FSP_BATCH_PERSISTENT) {
+        B.block = blk;
+        B.wg_per_cu = wg;
+    } else if (path == FSP_BATCH_LOCKSTEP) {
+        B.ls_bpc = wg;
+    } else {
+        B.workers = wk;
+    }
+    if (B.path_req == FSP_BATCH_AUTO) B.path_tuned = path;
+}
+
 extern "C" fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, int force,
                                          double *best_ms)
 {
@@ -427,14 +466,19 @@ extern "C" fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, in
     BatchImpl &B = b->impl;
     bool persistent = false;
     FSP_TRY(persistent_allowed(&persistent));
+    // family of candidates: the requested path, or with AUTO the persistent
+    // kernel where it runs and the lockstep path on GPU devices
+    int fam = B.path_req;
+    if (fam == FSP_BATCH_AUTO) fam = persistent ? FSP_BATCH_PERSISTENT : FSP_BATCH_LOCKSTEP;
+    if (fam == FSP_BATCH_PERSISTENT && !persistent)
+        return fail(FSP_ERR_UNSUPPORTED, "the persistent batch kernel is not launched on GPU devices");
     hipDeviceProp_t prop;
     int dev = 0;
     FSP_HIP(hipGetDevice(&dev));
     FSP_HIP(hipGetDeviceProperties(&prop, dev));
     std::ostringstream key;
     key << prop.gcnArchName << "|cu" << B.cus << "|n" << B.count << "|nv" << B.nv_max << "|d"
-        << B.dmax << "|m" << B.m << "|" << (B.mixed ? "mixed" : "f64")
-        << (persistent ? "" : "|streams");
+        << B.dmax << "|m" << B.m << "|" << (B.mixed ? "mixed" : "f64") << "|path" << fam;
     std::string cache;
     if (cache_path) {
         std::ifstream f(cache_path);
@@ -444,28 +488,21 @@ extern "C" fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, in
         const std::string k = "\"" + key.str() + "\": {";
         const size_t pos = cache.find(k);
         if (pos != std::string::npos && !force) {
-            int blk = 0, wg = 0, wk = 0;
+            int path = 0, blk = 0, wg = 0, wk = 0;
             double ms = 0;
             if (std::sscanf(cache.c_str() + pos + k.size(),
-                            "\"block\": %d, \"wg_per_cu\": %d, \"workers\": %d, \"ms\": %lf", &blk,
-                            &wg, &wk, &ms) == 4) {
-                if (persistent) {
-                    B.block = blk;
-                    B.wg_per_cu = wg;
-                } else {
-                    B.workers = wk;
-                }
+                            "\"path\": %d, \"block\": %d, \"wg_per_cu\": %d, \"workers\": %d, \"ms\": %lf",
+                            &path, &blk, &wg, &wk, &ms) == 5) {
+                apply_tuned(B, path, blk, wg, wk);
                 if (best_ms) *best_ms = ms;
                 return FSP_OK;
             }
         }
     }
     double best = 1e300;
-    int bb = 0, bw = 0, bk = 0;
-    if (persistent) {
+    int bb = B.block, bw = B.wg_per_cu, bk = B.workers;
+    if (fam == FSP_BATCH_PERSISTENT) {
         const int blocks[3] = {256, 512, 1024};
-        bb = B.block;
-        bw = B.wg_per_cu;
         hipEvent_t e0, e1;
         FSP_HIP(hipEventCreate(&e0));
         FSP_HIP(hipEventCreate(&e1));
@@ -488,17 +525,30 @@ extern "C" fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, in
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        B.block = bb;
-        B.wg_per_cu = bw;
+    } else if (fam == FSP_BATCH_LOCKSTEP) {
+        const int bpcs[3] = {0, 4, 16};  // 0: one workgroup per tile
+        for (int bpc : bpcs) {
+            FSP_TRY(batch_lockstep_prepare(B, bpc));
+            const auto t0 = std::chrono::steady_clock::now();
+            FSP_TRY(batch_solve_lockstep(B, bpc));
+            const double ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (ms < best) {
+                best = ms;
+                bw = bpc;
+            }
+        }
+        bb = -1;
     } else {
         FSP_TRY(batch_autotune_streams(B, &best, &bk));
-        B.workers = bk;
+        bb = 0;
     }
+    apply_tuned(B, fam, bb, bw, bk);
     if (best_ms) *best_ms = best;
     if (cache_path) {
         std::ostringstream ent;
-        ent << "\"" << key.str() << "\": {\"block\": " << bb << ", \"wg_per_cu\": " << bw
-            << ", \"workers\": " << bk << ", \"ms\": " << best << "}";
+        ent << "\"" << key.str() << "\": {\"path\": " << fam << ", \"block\": " << bb
+            << ", \"wg_per_cu\": " << bw << ", \"workers\": " << bk << ", \"ms\": " << best << "}";
         std::string body;
         const std::string k = "\"" + key.str() + "\": {";
         const size_t pos = cache.find(k);

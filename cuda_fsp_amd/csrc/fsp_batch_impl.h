@@ -1,6 +1,6 @@
 // fsp_batch_impl.h -- host state of a batch of independent FSP solves,
-// shared by the persistent-kernel path (fsp_batch.hip) and the stream
-// scheduler (fsp_batch_streams.hip).
+// shared by the persistent-kernel path (fsp_batch.hip), the lockstep path
+// (fsp_batch_lockstep.hip) and the stream scheduler (fsp_batch_streams.hip).
 #pragma once
 
 #include <vector>
@@ -30,8 +30,14 @@ struct ModelCopy {
     }
 };
 
+struct LsHost;  // device state of the lockstep path (fsp_batch_lockstep.hip)
+
 struct BatchImpl {
     int count = 0;
+    int path_req = FSP_BATCH_AUTO;     // requested path
+    int path_tuned = FSP_BATCH_AUTO;   // autotuner's choice for AUTO
+    int ls_bpc = 0;                    // lockstep row-kernel workgroups per CU (0: one per tile)
+    LsHost *ls = nullptr;
     int m = 30, iop = 0, q_iop = 2, max_reject = 1000;
     long long max_nstep = 100000;
     double tol = 1e-8, anorm = 1.0;
@@ -71,6 +77,12 @@ struct BatchImpl {
 // single-instance Krylov engine, `workers` instances at a time on their own
 // streams.
 fsp_status batch_solve_streams(BatchImpl &B);
+// Lockstep path (fsp_batch_lockstep.hip): all running instances advanced
+// one Krylov step per replay of one captured batch step.
+fsp_status batch_solve_lockstep(BatchImpl &B, int blocks_per_cu);
+// Allocations and the captured batch step for that grid, ahead of a timed solve.
+fsp_status batch_lockstep_prepare(BatchImpl &B, int blocks_per_cu);
+void batch_lockstep_free(BatchImpl &B);
 fsp_status batch_autotune_streams(BatchImpl &B, double *best_ms, int *best_workers);
 void batch_streams_free(BatchImpl &B);
 

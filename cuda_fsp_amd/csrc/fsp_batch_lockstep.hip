@@ -1,0 +1,858 @@
+// fsp_batch_lockstep.hip -- a batch of uneven FSP solves advanced in lockstep.
+//
+// Every instance runs its own KryExpvFSP::step state machine
+// (src/KryExpvFSP.cu:38-182): own step sizes, reject loop, happy breakdown
+// and final time.  What is shared is the launch sequence: one batch step is
+// the same graph of kernels for all instances (step prologue, m + 1 Arnoldi
+// iterations, Pade control, solution update), and each kernel covers every
+// instance still running.  Row kernels walk a list of 1024-row tiles of all
+// running instances (compacted at each step), so one launch fills the
+// device however small the instances are; per-instance reductions are kept
+// per tile (dot k of tile t at partials[2 (k T + t)]) and summed by one
+// workgroup per (instance, dot).  The operator is each instance's generator
+// in the HYB layout (FP64 diagonal + one ELL slot per unique net change),
+// filled on the device from its model once per batch: a streamed row costs
+// a few loads instead of the Kronecker factor products.  With the FP32
+// basis (mixed) the slot values are FP32 and the update is a second
+// Gram-Schmidt pass, as in the single-solve engine.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fsp_assemble.h"
+#include "fsp_batch_impl.h"
+#include "fsp_krylov_dev.h"
+
+namespace fsp {
+
+struct LsInst {
+    const DevModel *dm;
+    const double *tab;      // FP64 factor tables
+    const double *coefs;
+    double *hdiag;          // HYB: diagonal (n)
+    void *hvals;            // HYB: slot values [U][n], FP64 or FP32 (mixed)
+    int *hcols;             // HYB: slot columns [U][n]
+    double *hsink;          // HYB: outflow to the sink (n), or null
+    double *p;              // solution (nv)
+    const double *p0;       // initial vector (nv)
+    double *V64;            // FP64 basis V_0..V_m, or V_0 alone with an FP32 basis
+    float *V32;             // FP32 basis V_1..V_m (mixed)
+    double *W0, *W1;        // Arnoldi work vectors
+    double t_final;
+    int n, nv, with_sink, tab_size, tile0, ntiles, U, pad;
+};
+
+struct LsArgs {
+    const LsInst *inst;
+    KState *st;
+    const int *tile_inst;   // instance of tile t
+    const int *tile_r0;     // first row (within its instance) of tile t
+    int *atiles;            // tiles of the running instances
+    int *n_atiles;          // their count; [1]: running instances
+    double *partials;       // 2 (m + 2) T doubles
+    int T, count, m, M2;
+    int j, istart, mode, nk; // Arnoldi iteration, first vector orthogonalized against,
+                             // mode (0: dots, 1: avnorm), dots this iteration
+};
+
+namespace {
+
+constexpr int kLsRedLen = 2 * (kMaxKrylov + 2) * (kBlock / kWave);
+constexpr int kLsChunk = 8;
+// Tiles of kLsRows rows: each of the kBlock threads owns rows r0 + tid +
+// u kBlock (u < kLsRpt), so a tile's set-up and its dot reductions are
+// shared by kLsRpt rows per thread.
+constexpr int kLsRpt = 4;
+constexpr int kLsRows = kBlock * kLsRpt;
+
+__device__ inline double *ls_in(const LsInst &I, int j) { return j == 0 ? I.p : ((j & 1) ? I.W1 : I.W0); }
+__device__ inline double *ls_out(const LsInst &I, int j) { return ((j + 1) & 1) ? I.W1 : I.W0; }
+
+template <typename B>
+__device__ inline B *ls_vb(const LsInst &I)
+{
+    return std::is_same<B, double>::value ? (B *)(void *)I.V64 : (B *)(void *)I.V32;
+}
+
+// Wave butterflies of dot k into red (kLsRedLen), then after one barrier the
+// tile's partials of dots 0..nk: dot k of tile t at 2 (k T + t).  Leaves a
+// barrier for the caller's next use of red.
+__device__ inline void ls_put_wave(double *red, int k, dd v)
+{
+    constexpr int NW = kBlock / kWave;
+    const dd t = wave_sum_dd(v);
+    if (threadIdx.x % kWave == 0) {
+        const int w = threadIdx.x / kWave;
+        red[2 * (k * NW + w)] = t.s;
+        red[2 * (k * NW + w) + 1] = t.c;
+    }
+}
+
+__device__ inline void ls_store(const LsArgs &a, int t, int nk, double *red)
+{
+    constexpr int NW = kBlock / kWave;
+    __syncthreads();
+    if ((int)threadIdx.x <= nk) {
+        const int k = threadIdx.x;
+        dd s = dd_zero();
+#pragma unroll
+        for (int q = 0; q < NW; ++q) s = dd_merge(s, dd{red[2 * (k * NW + q)], red[2 * (k * NW + q) + 1]});
+        a.partials[2 * ((size_t)k * a.T + t)] = s.s;
+        a.partials[2 * ((size_t)k * a.T + t) + 1] = s.c;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------- step prologue
+// Tiles of the instances still running (t_now < t_final, no failure), in
+// instance order; n_atiles[1] = their number of instances.  One workgroup.
+__global__ void __launch_bounds__(1024) k_ls_compact(LsArgs a)
+{
+    __shared__ int flag[1024], ntl[1024], off[1024];
+    __shared__ int base, inst_run;
+    if (threadIdx.x == 0) {
+        base = 0;
+        inst_run = 0;
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < a.count; c0 += 1024) {
+        const int i = c0 + threadIdx.x;
+        int f = 0, nt = 0;
+        if (i < a.count) {
+            const KState *st = a.st + i;
+            f = st->t_now < st->t_final && st->status == 0;
+            nt = a.inst[i].ntiles;
+        }
+        flag[threadIdx.x] = f;
+        ntl[threadIdx.x] = nt;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int s = base, r = 0;
+            for (int q = 0; q < 1024 && c0 + q < a.count; ++q) {
+                off[q] = s;
+                if (flag[q]) {
+                    s += ntl[q];
+                    ++r;
+                }
+            }
+            base = s;
+            inst_run += r;
+        }
+        __syncthreads();
+        for (int q = 0; q < 1024 && c0 + q < a.count; ++q) {
+            if (!flag[q]) continue;
+            const int tile0 = a.inst[c0 + q].tile0;
+            for (int u = threadIdx.x; u < ntl[q]; u += blockDim.x) a.atiles[off[q] + u] = tile0 + u;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.n_atiles[0] = base;
+        a.n_atiles[1] = inst_run;
+    }
+}
+
+// |p|^2 per running tile (slot 0).
+__global__ void __launch_bounds__(kBlock) k_ls_norm_partials(LsArgs a)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    const int na = a.n_atiles[0];
+    for (int q = blockIdx.x; q < na; q += gridDim.x) {
+        const int t = a.atiles[q];
+        const LsInst &I = a.inst[a.tile_inst[t]];
+        dd s = dd_zero();
+#pragma unroll
+        for (int u = 0; u < kLsRpt; ++u) {
+            const int r = a.tile_r0[t] + (int)threadIdx.x + u * kBlock;
+            if (r < I.nv) s = dd_add_prod(s, I.p[r], I.p[r]);
+        }
+        s = block_sum_dd(s, red);
+        if (threadIdx.x == 0) {
+            a.partials[2 * (size_t)t] = s.s;
+            a.partials[2 * (size_t)t + 1] = s.c;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ls_step_begin(LsArgs a, long long max_nstep)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    const int i = blockIdx.x;
+    const LsInst &I = a.inst[i];
+    kry_step_begin(a.st + i, a.partials + 2 * (size_t)I.tile0, I.ntiles, a.m, a.M2, max_nstep, red);
+}
+
+// --------------------------------------------------------------- Arnoldi
+// Per running tile: V_j = w_j sigma_j, w_{j+1} = A V_j (Kronecker sum of the
+// instance's model), partial dots V_k . w_{j+1} (k = istart..j) and the sink
+// inflow; mode 1: |A V_m|^2 partials.
+template <typename B>
+__device__ inline double ls_row(const LsInst &I, uint32_t r, const KryX<B> &X, dd &sink)
+{
+    HybF<B> A;
+    A.diag = I.hdiag;
+    A.slot_vals = (const B *)I.hvals;
+    A.slot_cols = I.hcols;
+    A.sink_out = I.with_sink ? I.hsink : nullptr;
+    A.n = (uint32_t)I.n;
+    A.U = I.U;
+    return hyb_row<B>(A, r, X, sink);
+}
+
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_ls_arn(LsArgs a)
+{
+    __shared__ double red[kLsRedLen];
+    const int na = a.n_atiles[0];
+    for (int q = blockIdx.x; q < na; q += gridDim.x) {
+        const int t = a.atiles[q];
+        const int ii = a.tile_inst[t];
+        const KState *st = a.st + ii;
+        if (!st->active || st->broke || (a.mode == 1 && st->k1 == 0)) continue;
+        const LsInst &I = a.inst[ii];
+        const int r0 = a.tile_r0[t] + (int)threadIdx.x;
+        const double *in = ls_in(I, a.j);
+        double *out = ls_out(I, a.j);
+        const KryX<B> X{in, st->sigma[a.j], a.j};
+        B *Vb = ls_vb<B>(I);
+        const int nk = a.nk;
+        dd sink = dd_zero(), nrm = dd_zero();
+        double y[kLsRpt];
+#pragma unroll
+        for (int u = 0; u < kLsRpt; ++u) {
+            const uint32_t r = (uint32_t)(r0 + u * kBlock);
+            y[u] = 0.0;
+            if (r < (uint32_t)I.n) {
+                y[u] = ls_row<B>(I, r, X, sink);
+                store_basis<B>(I.V64, Vb, (size_t)I.nv, a.j, r, X(r));
+                if (a.mode == 0)
+                    out[r] = y[u];
+                else
+                    nrm = dd_add_prod(nrm, y[u], y[u]);
+            }
+        }
+        if (a.mode == 0) {
+            for (int c0 = 0; c0 < nk; c0 += kLsChunk) {
+                dd acc[kLsChunk];
+#pragma unroll
+                for (int v = 0; v < kLsChunk; ++v) acc[v] = dd_zero();
+#pragma unroll
+                for (int u = 0; u < kLsRpt; ++u) {
+                    const uint32_t r = (uint32_t)(r0 + u * kBlock);
+                    if (r >= (uint32_t)I.n) continue;
+                    double vk[kLsChunk];
+#pragma unroll
+                    for (int v = 0; v < kLsChunk; ++v)
+                        vk[v] = basis<B>(I.V64, Vb, (size_t)I.nv, min(a.istart + c0 + v, a.j), r);
+#pragma unroll
+                    for (int v = 0; v < kLsChunk; ++v)
+                        if (c0 + v < nk) acc[v] = dd_add_prod(acc[v], y[u], vk[v]);
+                }
+#pragma unroll
+                for (int v = 0; v < kLsChunk; ++v)
+                    if (c0 + v < nk) ls_put_wave(red, c0 + v, acc[v]);
+            }
+        } else {
+            ls_put_wave(red, 0, nrm);
+        }
+        ls_put_wave(red, nk, sink);
+        ls_store(a, t, nk, red);
+    }
+}
+
+// Block (i, k) of count x nk: dot k of instance i from its tiles' partials
+// in tile order, with the sink element (src/KryExpvFSP.cu:76-78); mode 1:
+// avnorm and H(m+1, m) = 1 (:113-116).
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_ls_dots(LsArgs a)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    const int i = blockIdx.x / a.nk, k = blockIdx.x % a.nk;
+    KState *st = a.st + i;
+    if (!st->active || st->broke || (a.mode == 1 && st->k1 == 0)) return;
+    const LsInst &I = a.inst[i];
+    const double *in = ls_in(I, a.j);
+    double *out = ls_out(I, a.j);
+    B *Vb = ls_vb<B>(I);
+    const size_t nv = (size_t)I.nv;
+    double ys = 0.0, xs = 0.0;
+    if (I.with_sink) {
+        ys = dd_value(sum_dd_partials(a.partials + 2 * ((size_t)a.nk * a.T + I.tile0), I.ntiles, red));
+        xs = kry_scale<B>(in[I.n], st->sigma[a.j], a.j);
+    }
+    dd sd = sum_dd_partials(a.partials + 2 * ((size_t)k * a.T + I.tile0), I.ntiles, red);
+    if (threadIdx.x == 0) {
+        if (a.mode == 0) {
+            const int kk = a.istart + k;
+            if (I.with_sink) {
+                const double vk = (kk == a.j) ? xs : basis<B>(I.V64, Vb, nv, kk, (uint32_t)I.n);
+                sd = dd_add_prod(sd, ys, vk);
+            }
+            st->H[kk + a.j * a.M2] = dd_value(sd);
+            if (k == 0) {
+                st->n_matvec += 1;
+                if (I.with_sink) {
+                    out[I.n] = ys;
+                    store_basis<B>(I.V64, Vb, nv, a.j, (uint32_t)I.n, xs);
+                }
+            }
+        } else {
+            st->avnorm = sqrt(dd_value(dd_add_prod(sd, ys, ys)));
+            st->H[(a.j + 1) + a.j * a.M2] = 1.0;  // src/KryExpvFSP.cu:115
+            st->n_matvec += 1;
+            if (I.with_sink) store_basis<B>(I.V64, Vb, nv, a.j, (uint32_t)I.n, xs);
+        }
+    }
+}
+
+// w_{j+1} -= V[istart..j] h per running tile (all nv entries, the sink
+// element included), |w|^2 partials in slot 0; h = H(., j), or h2 (pass 1).
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_ls_update(LsArgs a, int pass)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    const int na = a.n_atiles[0];
+    for (int q = blockIdx.x; q < na; q += gridDim.x) {
+        const int t = a.atiles[q];
+        const int ii = a.tile_inst[t];
+        const KState *st = a.st + ii;
+        if (!st->active || st->broke) continue;
+        const LsInst &I = a.inst[ii];
+        const int r0 = a.tile_r0[t] + (int)threadIdx.x;
+        const double *h = pass ? st->h2 : st->H + (size_t)a.j * a.M2;
+        double *out = ls_out(I, a.j);
+        const B *Vb = ls_vb<B>(I);
+        dd nrm = dd_zero();
+#pragma unroll
+        for (int u = 0; u < kLsRpt; ++u) {
+            const uint32_t r = (uint32_t)(r0 + u * kBlock);
+            if (r >= (uint32_t)I.nv) continue;
+            double acc = 0.0;
+            for (int k = a.istart; k <= a.j; ++k) acc += basis<B>(I.V64, Vb, (size_t)I.nv, k, r) * h[k];
+            const double w = -1.0 * acc + out[r];
+            out[r] = w;
+            nrm = dd_add_prod(nrm, w, w);
+        }
+        nrm = block_sum_dd(nrm, red);
+        if (threadIdx.x == 0) {
+            a.partials[2 * (size_t)t] = nrm.s;
+            a.partials[2 * (size_t)t + 1] = nrm.c;
+        }
+    }
+}
+
+// FP32 basis: first pass of the update, then partial dots V_k . w_{j+1}
+// (slots 0..nk-1) for the second pass.
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_ls_update_dots(LsArgs a)
+{
+    __shared__ double red[kLsRedLen];
+    const int na = a.n_atiles[0];
+    for (int q = blockIdx.x; q < na; q += gridDim.x) {
+        const int t = a.atiles[q];
+        const int ii = a.tile_inst[t];
+        const KState *st = a.st + ii;
+        if (!st->active || st->broke) continue;
+        const LsInst &I = a.inst[ii];
+        const int r0 = a.tile_r0[t] + (int)threadIdx.x;
+        const double *h = st->H + (size_t)a.j * a.M2;
+        double *out = ls_out(I, a.j);
+        const B *Vb = ls_vb<B>(I);
+        double w[kLsRpt];
+#pragma unroll
+        for (int u = 0; u < kLsRpt; ++u) {
+            const uint32_t r = (uint32_t)(r0 + u * kBlock);
+            w[u] = 0.0;
+            if (r >= (uint32_t)I.nv) continue;
+            double acc = 0.0;
+            for (int k = a.istart; k <= a.j; ++k) acc += basis<B>(I.V64, Vb, (size_t)I.nv, k, r) * h[k];
+            w[u] = -1.0 * acc + out[r];
+            out[r] = w[u];
+        }
+        const int nk = a.nk;
+        for (int c0 = 0; c0 < nk; c0 += kLsChunk) {
+            dd acc[kLsChunk];
+#pragma unroll
+            for (int v = 0; v < kLsChunk; ++v) acc[v] = dd_zero();
+#pragma unroll
+            for (int u = 0; u < kLsRpt; ++u) {
+                const uint32_t r = (uint32_t)(r0 + u * kBlock);
+                if (r >= (uint32_t)I.nv) continue;
+                double vk[kLsChunk];
+#pragma unroll
+                for (int v = 0; v < kLsChunk; ++v)
+                    vk[v] = basis<B>(I.V64, Vb, (size_t)I.nv, min(a.istart + c0 + v, a.j), r);
+#pragma unroll
+                for (int v = 0; v < kLsChunk; ++v)
+                    if (c0 + v < nk) acc[v] = dd_add_prod(acc[v], w[u], vk[v]);
+            }
+#pragma unroll
+            for (int v = 0; v < kLsChunk; ++v)
+                if (c0 + v < nk) ls_put_wave(red, c0 + v, acc[v]);
+        }
+        ls_put_wave(red, nk, dd_zero());
+        ls_store(a, t, nk, red);
+    }
+}
+
+// Block (i, k): h2(istart + k) of instance i, H(istart + k, j) += h2.
+__global__ void __launch_bounds__(kBlock) k_ls_reorth_fin(LsArgs a)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    const int i = blockIdx.x / a.nk, k = blockIdx.x % a.nk;
+    KState *st = a.st + i;
+    if (!st->active || st->broke) return;
+    const LsInst &I = a.inst[i];
+    const double s = dd_value(sum_dd_partials(a.partials + 2 * ((size_t)k * a.T + I.tile0), I.ntiles, red));
+    if (threadIdx.x == 0) {
+        const int kk = a.istart + k;
+        st->h2[kk] = s;
+        st->H[kk + a.j * a.M2] += s;
+    }
+}
+
+// Per instance: s = |w_{j+1}|, happy breakdown (src/KryExpvFSP.cu:90-98) or
+// H(j+1, j) = s and sigma_{j+1} = 1 / s.
+__global__ void __launch_bounds__(kBlock) k_ls_norm(LsArgs a)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    const int i = blockIdx.x;
+    KState *st = a.st + i;
+    if (!st->active || st->broke) return;
+    const LsInst &I = a.inst[i];
+    const double s = sqrt(dd_value(sum_dd_partials(a.partials + 2 * (size_t)I.tile0, I.ntiles, red)));
+    if (threadIdx.x == 0) {
+        if (s < st->btol) {
+            st->k1 = 0;
+            st->mb = a.j + 1;
+            st->tau = st->t_final - st->t_now;
+            st->broke = 1;
+            st->n_breakdown += 1;
+        } else {
+            st->H[(a.j + 1) + a.j * a.M2] = s;
+            st->sigma[a.j + 1] = 1.0 / s;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ls_control(LsArgs a, int max_reject)
+{
+    HIP_DYNAMIC_SHARED(double, smem)
+    kry_control(a.st + blockIdx.x, a.m, max_reject, smem);
+}
+
+// p = beta V[:, 0:mx] F0 per running tile (src/KryExpvFSP.cu:160-166).
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_ls_solution(LsArgs a)
+{
+    const int na = a.n_atiles[0];
+    for (int q = blockIdx.x; q < na; q += gridDim.x) {
+        const int t = a.atiles[q];
+        const int ii = a.tile_inst[t];
+        const KState *st = a.st + ii;
+        if (!st->active) continue;
+        const LsInst &I = a.inst[ii];
+        const int r0 = a.tile_r0[t] + (int)threadIdx.x;
+        const B *Vb = ls_vb<B>(I);
+        const int mx = st->mx;
+        const double beta = st->beta;
+#pragma unroll
+        for (int u = 0; u < kLsRpt; ++u) {
+            const uint32_t r = (uint32_t)(r0 + u * kBlock);
+            if (r >= (uint32_t)I.nv) continue;
+            double acc = 0.0;
+            for (int k = 0; k < mx; ++k) acc += basis<B>(I.V64, Vb, (size_t)I.nv, k, r) * st->F0[k];
+            I.p[r] = beta * acc;
+        }
+    }
+}
+
+// ------------------------------------------------------ set-up and export
+// HYB generator of every instance from its model (the single-solve HYB
+// fill's row values: fsp_assemble.h), one row per thread over all tiles.
+template <int D, typename V>
+__global__ void __launch_bounds__(kBlock) k_ls_fill_hyb(LsArgs a, int mixed)
+{
+    for (int t = blockIdx.x; t < a.T; t += gridDim.x) {
+        const LsInst &I = a.inst[a.tile_inst[t]];
+        const DevModel *dm = I.dm;
+        V *vals = (V *)I.hvals;
+        for (int u = 0; u < kLsRpt; ++u) {
+            const uint32_t i = (uint32_t)(a.tile_r0[t] + (int)threadIdx.x + u * kBlock);
+            if (i >= (uint32_t)I.n) continue;
+            int xs[D];
+            decode<D>(i, dm, xs);
+            double dg, out;
+            int has;
+            diag_and_out<D>(dm, I.tab, nullptr, I.coefs, mixed, i, xs, dg, out, has);
+            I.hdiag[i] = dg;
+            if (I.with_sink) I.hsink[i] = out;
+            for (int s = 0; s < I.U; ++s) {
+                const size_t o = (size_t)s * I.n + i;
+                if (source_in_box<D>(dm->rx[dm->slot_rxn[dm->slot_ptr[s]]], xs, dm)) {
+                    vals[o] = (V)slot_value<D>(dm, I.tab, nullptr, I.coefs, s, i, xs, 1);
+                    I.hcols[o] = (int)((int64_t)i - dm->slot_delta[s]);
+                } else {
+                    vals[o] = (V)0.0;
+                    I.hcols[o] = (int)i;
+                }
+            }
+        }
+    }
+}
+
+// Per instance: p = p0 and a fresh step state (t_now = 0, first step size
+// from the Expokit formula).
+__global__ void __launch_bounds__(kBlock) k_ls_reset(LsArgs a)
+{
+    const int i = blockIdx.x;
+    const LsInst &I = a.inst[i];
+    for (int r = threadIdx.x; r < I.nv; r += blockDim.x) I.p[r] = I.p0[r];
+    if (threadIdx.x == 0) {
+        KState *st = a.st + i;
+        st->t_now = 0.0;
+        st->t_new = 0.0;
+        st->t_final = I.t_final;
+        st->i_step = 0;
+        st->k1 = 2;
+        st->status = 0;
+        st->active = 0;
+        st->broke = 0;
+        st->n_reject = st->n_breakdown = st->n_matvec = 0;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ls_export(LsArgs a, BatchInst *out)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.count; i += gridDim.x * blockDim.x) {
+        const KState *st = a.st + i;
+        out[i].steps = st->i_step;
+        out[i].rejects = st->n_reject;
+        out[i].status = st->status;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ls_running(LsArgs a, int *out)
+{
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int c = 0;
+    for (int i = threadIdx.x; i < a.count; i += blockDim.x) {
+        const KState *st = a.st + i;
+        c += st->t_now < st->t_final && st->status == 0;
+    }
+    atomicAdd(&cnt, c);
+    __syncthreads();
+    if (threadIdx.x == 0) *out = cnt;
+}
+
+}  // namespace
+
+// Device state of the lockstep path (owned by BatchImpl::ls).
+struct LsHost {
+    LsInst *d_inst = nullptr;
+    KState *d_st = nullptr;
+    int *d_tile_inst = nullptr, *d_tile_r0 = nullptr, *d_atiles = nullptr, *d_cnt = nullptr;
+    int *h_cnt = nullptr;
+    double *d_partials = nullptr, *d_basis = nullptr, *d_work = nullptr;
+    float *d_basis32 = nullptr;
+    double *d_hdiag = nullptr, *d_hsink = nullptr;
+    void *d_hvals = nullptr;
+    int *d_hcols = nullptr;
+    int T = 0, grid = 0, graph_grid = 0;
+    hipStream_t s = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+static LsArgs ls_args(const BatchImpl &B, const LsHost &L)
+{
+    LsArgs a;
+    a.inst = L.d_inst;
+    a.st = L.d_st;
+    a.tile_inst = L.d_tile_inst;
+    a.tile_r0 = L.d_tile_r0;
+    a.atiles = L.d_atiles;
+    a.n_atiles = L.d_cnt;
+    a.partials = L.d_partials;
+    a.T = L.T;
+    a.count = B.count;
+    a.m = B.m;
+    a.M2 = B.m + 2;
+    a.j = 0;
+    a.istart = 0;
+    a.mode = 0;
+    a.nk = 1;
+    return a;
+}
+
+void batch_lockstep_free(BatchImpl &B)
+{
+    LsHost *L = B.ls;
+    if (!L) return;
+    if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
+    if (L->graph) (void)hipGraphDestroy(L->graph);
+    if (L->s) (void)hipStreamDestroy(L->s);
+    void *ptrs[] = {L->d_inst, L->d_st, L->d_tile_inst, L->d_tile_r0, L->d_atiles, L->d_cnt,
+                    L->d_partials, L->d_basis, L->d_work, L->d_basis32, L->d_hdiag,
+                    L->d_hsink, L->d_hvals, L->d_hcols};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (L->h_cnt) (void)hipHostFree(L->h_cnt);
+    delete L;
+    B.ls = nullptr;
+}
+
+static fsp_status ls_build(BatchImpl &B)
+{
+    if (B.ls) return FSP_OK;
+    B.ls = new LsHost();
+    LsHost &L = *B.ls;
+    const int count = B.count, m = B.m;
+    // tiles of kLsRows rows over every instance's nv entries
+    std::vector<int> tinst, tr0;
+    std::vector<LsInst> hi((size_t)count);
+    size_t basis_total = 0, basis32_total = 0, work_total = 0, diag_total = 0, slot_total = 0;
+    for (int i = 0; i < count; ++i) {
+        const BatchInst &bi = B.h_inst[i];
+        LsInst &I = hi[i];
+        std::memset(&I, 0, sizeof(I));
+        I.n = bi.n;
+        I.nv = bi.nv;
+        I.with_sink = bi.with_sink;
+        I.tab_size = bi.tab_size;
+        I.t_final = bi.t_final;
+        I.tile0 = (int)tinst.size();
+        I.ntiles = (bi.nv + kLsRows - 1) / kLsRows;
+        for (int u = 0; u < I.ntiles; ++u) {
+            tinst.push_back(i);
+            tr0.push_back(u * kLsRows);
+        }
+        basis_total += (size_t)bi.nv * (B.mixed ? 1 : (size_t)(m + 1));
+        basis32_total += B.mixed ? (size_t)bi.nv * (size_t)m : 0;
+        work_total += 2 * (size_t)bi.nv;
+        I.U = B.hms[i].dm.U;
+        diag_total += (size_t)bi.n;
+        slot_total += (size_t)I.U * (size_t)bi.n;
+    }
+    L.T = (int)tinst.size();
+    FSP_HIP(hipMalloc(&L.d_basis, sizeof(double) * basis_total));
+    if (B.mixed) FSP_HIP(hipMalloc(&L.d_basis32, sizeof(float) * basis32_total));
+    FSP_HIP(hipMalloc(&L.d_work, sizeof(double) * work_total));
+    const size_t vbytes = B.mixed ? sizeof(float) : sizeof(double);
+    FSP_HIP(hipMalloc(&L.d_hdiag, sizeof(double) * diag_total));
+    FSP_HIP(hipMalloc(&L.d_hsink, sizeof(double) * diag_total));
+    FSP_HIP(hipMalloc(&L.d_hvals, vbytes * std::max<size_t>(slot_total, 1)));
+    FSP_HIP(hipMalloc(&L.d_hcols, sizeof(int) * std::max<size_t>(slot_total, 1)));
+    size_t ob = 0, ob32 = 0, ow = 0, od = 0, os = 0;
+    for (int i = 0; i < count; ++i) {
+        const BatchInst &bi = B.h_inst[i];
+        LsInst &I = hi[i];
+        I.dm = bi.dm;
+        I.tab = bi.tab;
+        I.coefs = bi.coefs;
+        I.hdiag = L.d_hdiag + od;
+        I.hsink = L.d_hsink + od;
+        I.hvals = (char *)L.d_hvals + vbytes * os;
+        I.hcols = L.d_hcols + os;
+        I.p = bi.p;
+        I.p0 = bi.p0;
+        I.V64 = L.d_basis + ob;
+        I.V32 = B.mixed ? L.d_basis32 + ob32 : nullptr;
+        I.W0 = L.d_work + ow;
+        I.W1 = I.W0 + bi.nv;
+        ob += (size_t)bi.nv * (B.mixed ? 1 : (size_t)(m + 1));
+        ob32 += B.mixed ? (size_t)bi.nv * (size_t)m : 0;
+        ow += 2 * (size_t)bi.nv;
+        od += (size_t)bi.n;
+        os += (size_t)I.U * (size_t)bi.n;
+    }
+    FSP_HIP(hipMalloc(&L.d_inst, sizeof(LsInst) * (size_t)count));
+    FSP_HIP(hipMemcpy(L.d_inst, hi.data(), sizeof(LsInst) * (size_t)count, hipMemcpyHostToDevice));
+    FSP_HIP(hipMalloc(&L.d_tile_inst, sizeof(int) * (size_t)L.T));
+    FSP_HIP(hipMalloc(&L.d_tile_r0, sizeof(int) * (size_t)L.T));
+    FSP_HIP(hipMalloc(&L.d_atiles, sizeof(int) * (size_t)L.T));
+    FSP_HIP(hipMemcpy(L.d_tile_inst, tinst.data(), sizeof(int) * (size_t)L.T, hipMemcpyHostToDevice));
+    FSP_HIP(hipMemcpy(L.d_tile_r0, tr0.data(), sizeof(int) * (size_t)L.T, hipMemcpyHostToDevice));
+    FSP_HIP(hipMalloc(&L.d_cnt, sizeof(int) * 4));
+    FSP_HIP(hipMemset(L.d_cnt, 0, sizeof(int) * 4));
+    FSP_HIP(hipHostMalloc((void **)&L.h_cnt, sizeof(int) * 4, hipHostMallocDefault));
+    FSP_HIP(hipMalloc(&L.d_partials, sizeof(double) * 2 * (size_t)(m + 2) * (size_t)L.T));
+    // step states: constants once (tolerance, decimal powers), the rest per solve
+    std::vector<double> p10(kP10Hi - kP10Lo + 1);
+    FSP_HIP(hipMemcpy(p10.data(), B.d_p10, sizeof(double) * p10.size(), hipMemcpyDeviceToHost));
+    std::vector<KState> hs((size_t)count);
+    const double fact = std::pow((m + 1) / std::exp(1.0), m + 1) * std::sqrt(2 * (3.1416) * (m + 1));
+    for (int i = 0; i < count; ++i) {
+        KState &k = hs[i];
+        std::memset(&k, 0, sizeof(k));
+        k.fact = fact;
+        k.tol = B.tol;
+        k.anorm = B.anorm;
+        k.t_final = B.h_inst[i].t_final;
+        k.k1 = 2;
+        std::memcpy(k.p10, p10.data(), sizeof(double) * p10.size());
+    }
+    FSP_HIP(hipMalloc(&L.d_st, sizeof(KState) * (size_t)count));
+    FSP_HIP(hipMemcpy(L.d_st, hs.data(), sizeof(KState) * (size_t)count, hipMemcpyHostToDevice));
+    FSP_HIP(hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking));
+    L.grid = 0;
+    // HYB generators
+    const LsArgs a = ls_args(B, L);
+    const int fgrid = std::max(1, std::min(L.T, 65535));
+    if (B.dmax <= 4) {
+        if (B.mixed)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_fill_hyb<4, float>), dim3(fgrid), dim3(kBlock), 0, L.s, a, 1);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_fill_hyb<4, double>), dim3(fgrid), dim3(kBlock), 0, L.s, a, 0);
+    } else {
+        if (B.mixed)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_fill_hyb<8, float>), dim3(fgrid), dim3(kBlock), 0, L.s, a, 1);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_fill_hyb<8, double>), dim3(fgrid), dim3(kBlock), 0, L.s, a, 0);
+    }
+    FSP_HIP(hipGetLastError());
+    FSP_HIP(hipStreamSynchronize(L.s));
+    return FSP_OK;
+}
+
+template <typename Bt>
+static fsp_status ls_enqueue_step_t(BatchImpl &B, LsHost &L, int grid, hipStream_t s)
+{
+    const int m = B.m, M2 = m + 2;
+    LsArgs a = ls_args(B, L);
+    hipLaunchKernelGGL(k_ls_compact, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_ls_norm_partials, dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_ls_step_begin, dim3(B.count), dim3(kBlock), 0, s, a, (long long)B.max_nstep);
+    FSP_HIP(hipGetLastError());
+    int istart = 0;
+    for (int j = 0; j <= m; ++j) {
+        const bool last = (j == m);
+        if (!last && B.iop) istart = (j >= B.q_iop - 1) ? j - B.q_iop + 1 : 0;
+        a.j = j;
+        a.istart = last ? 0 : istart;
+        a.mode = last ? 1 : 0;
+        a.nk = last ? 1 : j - istart + 1;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_arn<Bt>), dim3(grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_dots<Bt>), dim3(B.count * a.nk), dim3(kBlock), 0, s, a);
+        FSP_HIP(hipGetLastError());
+        if (last) break;
+        if (B.mixed) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update_dots<Bt>), dim3(grid), dim3(kBlock), 0, s, a);
+            hipLaunchKernelGGL(k_ls_reorth_fin, dim3(B.count * a.nk), dim3(kBlock), 0, s, a);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update<Bt>), dim3(grid), dim3(kBlock), 0, s, a, 1);
+        } else {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update<Bt>), dim3(grid), dim3(kBlock), 0, s, a, 0);
+        }
+        hipLaunchKernelGGL(k_ls_norm, dim3(B.count), dim3(kBlock), 0, s, a);
+        FSP_HIP(hipGetLastError());
+    }
+    const size_t lds = sizeof(double) * 5 * (size_t)M2 * M2;
+    hipLaunchKernelGGL(k_ls_control, dim3(B.count), dim3(kBlock), lds, s, a, B.max_reject);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_solution<Bt>), dim3(grid), dim3(kBlock), 0, s, a);
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
+}
+
+static fsp_status ls_enqueue_step(BatchImpl &B, LsHost &L, int grid, hipStream_t s)
+{
+    return B.mixed ? ls_enqueue_step_t<float>(B, L, grid, s) : ls_enqueue_step_t<double>(B, L, grid, s);
+}
+
+// Workgroups of the row kernels: one per tile (blocks_per_cu = 0; those
+// past the running tiles return at once), or blocks_per_cu per CU walking
+// the running tiles grid-stride.
+static int ls_grid(const BatchImpl &B, const LsHost &L, int blocks_per_cu)
+{
+    if (blocks_per_cu <= 0) return std::max(1, L.T);
+    return std::max(1, std::min(L.T, B.cus * blocks_per_cu));
+}
+
+// One batch step as a HIP graph (the kernels read all per-step values from
+// device state, so one capture serves every step).
+static fsp_status ls_graph(BatchImpl &B, LsHost &L, int grid)
+{
+    if (L.gexec && L.graph_grid == grid) return FSP_OK;
+    if (L.gexec) (void)hipGraphExecDestroy(L.gexec);
+    if (L.graph) (void)hipGraphDestroy(L.graph);
+    L.gexec = nullptr;
+    L.graph = nullptr;
+    hipStream_t cap = nullptr;
+    FSP_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    FSP_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    fsp_status st = ls_enqueue_step(B, L, grid, cap);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(cap, &g);
+    (void)hipStreamDestroy(cap);
+    FSP_TRY(st);
+    if (ce != hipSuccess) return fail(FSP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+    L.graph = g;
+    FSP_HIP(hipGraphInstantiate(&L.gexec, g, nullptr, nullptr, 0));
+    L.graph_grid = grid;
+    return FSP_OK;
+}
+
+// FSP_BATCH_GRAPH=0 launches each batch step's kernels directly instead of
+// replaying the captured graph (for tools that do not follow graphs).
+static bool ls_use_graph()
+{
+    const char *g = std::getenv("FSP_BATCH_GRAPH");
+    return !(g && std::atoi(g) == 0);
+}
+
+fsp_status batch_lockstep_prepare(BatchImpl &B, int blocks_per_cu)
+{
+    FSP_TRY(ls_build(B));
+    if (!ls_use_graph()) return FSP_OK;
+    return ls_graph(B, *B.ls, ls_grid(B, *B.ls, blocks_per_cu));
+}
+
+fsp_status batch_solve_lockstep(BatchImpl &B, int blocks_per_cu)
+{
+    FSP_TRY(batch_lockstep_prepare(B, blocks_per_cu));
+    LsHost &L = *B.ls;
+    LsArgs a = ls_args(B, L);
+    hipLaunchKernelGGL(k_ls_reset, dim3(B.count), dim3(kBlock), 0, L.s, a);
+    FSP_HIP(hipGetLastError());
+    // steps are launched in groups ahead of the host check; a step with no
+    // running instance does nothing
+    const int ahead = 4;
+    for (long long guard = 0;; guard += ahead) {
+        for (int k = 0; k < ahead; ++k) {
+            if (L.gexec && ls_use_graph())
+                FSP_HIP(hipGraphLaunch(L.gexec, L.s));
+            else
+                FSP_TRY(ls_enqueue_step(B, L, ls_grid(B, L, blocks_per_cu), L.s));
+        }
+        hipLaunchKernelGGL(k_ls_running, dim3(1), dim3(kBlock), 0, L.s, a, L.d_cnt + 2);
+        FSP_HIP(hipGetLastError());
+        FSP_HIP(hipMemcpyAsync(L.h_cnt, L.d_cnt, sizeof(int) * 4, hipMemcpyDeviceToHost, L.s));
+        FSP_HIP(hipStreamSynchronize(L.s));
+        if (L.h_cnt[2] == 0) break;
+        if (guard > B.max_nstep + 8) return fail(FSP_ERR_NOT_CONVERGED, "batch steps exceeded max_nstep");
+    }
+    hipLaunchKernelGGL(k_ls_export, dim3((B.count + kBlock - 1) / kBlock), dim3(kBlock), 0, L.s, a, B.d_inst);
+    FSP_HIP(hipGetLastError());
+    FSP_HIP(hipStreamSynchronize(L.s));
+    // report the first failed instance
+    std::vector<BatchInst> h((size_t)B.count);
+    FSP_HIP(hipMemcpy(h.data(), B.d_inst, sizeof(BatchInst) * (size_t)B.count, hipMemcpyDeviceToHost));
+    for (int i = 0; i < B.count; ++i) {
+        if (h[i].status != 0) {
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "batch instance %d (%d states, t = %f): %s", i, h[i].nv,
+                          h[i].t_final,
+                          h[i].steps >= B.max_nstep ? "step count reached max_nstep"
+                                                    : "Krylov reject loop exhausted max_reject");
+            return fail((fsp_status)h[i].status, buf);
+        }
+    }
+    return FSP_OK;
+}
+
+}  // namespace fsp

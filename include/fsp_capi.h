@@ -213,28 +213,52 @@ typedef struct fsp_batch_options {
     int wg_per_cu;    /* persistent workgroups per CU, 1 or 2 (0: 2)           */
     int max_reject;   /* 0: 1000                                               */
     int64_t max_nstep;/* 0: 100000                                             */
+    int path;         /* FSP_BATCH_AUTO (0) or one of the paths below          */
 } fsp_batch_options;
 
+/* Batch execution paths.  LOCKSTEP: one graph-captured batch step advances
+ * every running instance by one Krylov step (row kernels over the 1024-row
+ * tiles of all running instances; per-instance step control).  STREAMS:
+ * each instance through the single-solve engine, up to four at a time on
+ * their own streams.  PERSISTENT: one workgroup per instance in a
+ * persistent work-queue kernel; refused on GPU devices.  AUTO: the
+ * autotuner's choice, else LOCKSTEP on GPU devices and PERSISTENT elsewhere. */
+#define FSP_BATCH_AUTO 0
+#define FSP_BATCH_LOCKSTEP 1
+#define FSP_BATCH_STREAMS 2
+#define FSP_BATCH_PERSISTENT 3
+
 /* count independent solves v_i <- exp(t_final[i] A_i) v_i (Krylov expv,
- * KryExpvFSP semantics) run by one persistent kernel.  models[i] needs
- * factor tables; coefs holds the R_i coefficients of every instance in
- * order; with_sink may be NULL.  Initial vectors default to delta(state 0). */
+ * KryExpvFSP semantics, each instance with its own step sizes).  models[i]
+ * needs factor tables; coefs holds the R_i coefficients of every instance
+ * in order; with_sink may be NULL.  Initial vectors default to
+ * delta(state 0). */
 fsp_status fsp_batch_create(const fsp_model *models, const double *coefs,
                             const double *t_final, const int *with_sink, int count,
                             const fsp_batch_options *options, void *stream,
                             fsp_batch *out);
 fsp_status fsp_batch_destroy(fsp_batch b);
 fsp_status fsp_batch_set_initial(fsp_batch b, int i, const double *h_p0);
-/* One solve of every instance from its initial vector: a HIP graph (queue
- * reset + persistent kernel) launched on `stream` (asynchronous). */
+/* One solve of every instance from its initial vector.  PERSISTENT: a HIP
+ * graph (queue reset + persistent kernel) launched on `stream`
+ * (asynchronous).  LOCKSTEP and STREAMS wait for `stream`, run on the
+ * batch's own streams and return when every instance is done; an instance
+ * that fails (reject loop or step limit) makes the call return its status
+ * and a message naming it, the others' results stay valid. */
 fsp_status fsp_batch_solve(fsp_batch b, void *stream);
 fsp_status fsp_batch_get_solution(fsp_batch b, int i, double *h_p, int64_t cap);
 fsp_status fsp_batch_get_stats(fsp_batch b, int i, int64_t *steps, int64_t *rejects,
                                int *status);
+/* Configuration of the path in use: PERSISTENT (workgroup size, workgroups
+ * per CU); LOCKSTEP (-1, row-kernel workgroups per CU, 0 = one per 1024-row
+ * tile); STREAMS (0, concurrent streams). */
 fsp_status fsp_batch_get_config(fsp_batch b, int *block, int *wg_per_cu);
-/* Times every (workgroup size, workgroups per CU) candidate on this batch,
- * keeps the fastest and records it in the JSON file cache_path (NULL: no
- * cache); a cached entry for the same device / shape is reused unless force. */
+/* Times every candidate configuration on this batch (PERSISTENT: workgroup
+ * size x workgroups per CU; on GPU devices with path AUTO or LOCKSTEP:
+ * one row-kernel workgroup per tile, 4 or 16 per CU; STREAMS: 1, 2 or 4
+ * streams), keeps the
+ * fastest and records it in the JSON file cache_path (NULL: no cache); a
+ * cached entry for the same device / shape is reused unless force. */
 fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, int force, double *best_ms);
 
 /* ---- time-varying right-hand side and BDF integration ---- */

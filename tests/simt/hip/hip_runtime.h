@@ -367,6 +367,17 @@ inline hipError_t hipFree(void *p)
     std::free(p);
     return hipSuccess;
 }
+enum { hipHostMallocDefault = 0 };
+inline hipError_t hipHostMalloc(void **p, size_t bytes, unsigned)
+{
+    *p = std::malloc(bytes ? bytes : 1);
+    return *p ? hipSuccess : hipErrorInvalidValue;
+}
+inline hipError_t hipHostFree(void *p)
+{
+    std::free(p);
+    return hipSuccess;
+}
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind)
 {
     if (n) std::memcpy(d, s, n);

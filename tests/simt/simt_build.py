@@ -1,5 +1,5 @@
-"""Builds the library's batched solver (fsp_batch.hip + fsp_model.hip) for the
-CPU with g++ against the SIMT stand-in header in this folder, as
+"""Builds the library's batched solver (fsp_batch.hip, fsp_batch_lockstep.hip,
+fsp_model.hip) for the CPU with g++ against the SIMT stand-in header in this folder, as
 tests/simt/_build/libfsp_batch_simt.so.  The sources are compiled unchanged:
 the stand-in supplies the HIP runtime, kernel launches, barriers and shuffles.
 """
@@ -14,11 +14,11 @@ CSRC = os.path.join(ROOT, "cuda_fsp_amd", "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libfsp_batch_simt.so")
 
-SOURCES = [os.path.join(CSRC, "fsp_batch.hip"), os.path.join(CSRC, "fsp_model.hip"),
-           os.path.join(HERE, "simt_support.cpp")]
+SOURCES = [os.path.join(CSRC, "fsp_batch.hip"), os.path.join(CSRC, "fsp_batch_lockstep.hip"),
+           os.path.join(CSRC, "fsp_model.hip"), os.path.join(HERE, "simt_support.cpp")]
 DEPS = SOURCES + [os.path.join(HERE, "hip", "hip_runtime.h")] + [
     os.path.join(CSRC, h) for h in ("fsp_batch_kernel.h", "fsp_batch_impl.h", "fsp_expm.h", "fsp_formats.h",
-                                    "fsp_device.h", "fsp_internal.h")]
+                                    "fsp_device.h", "fsp_internal.h", "fsp_krylov_dev.h")]
 
 
 def build(force: bool = False) -> str:

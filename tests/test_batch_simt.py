@@ -99,7 +99,8 @@ def test_simt_batch_initial_vector_and_autotune(sim, tmp_path):
 
 def test_simt_lane_order_independent(simlib):
     """Reversed lane order between barriers gives the same bits (no result
-    depends on which lane of a phase runs first)."""
+    depends on which lane of a phase runs first), for the persistent and the
+    lockstep path."""
     code = (
         "import sys, numpy as np; sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
         "import ctypes as C, simt_build\n"
@@ -110,7 +111,8 @@ def test_simt_lane_order_independent(simlib):
         "capi._lib = L\n"
         "net = M.toggle_switch((14, 14), rates=M.TOGGLE_RATES)\n"
         "B = capi.Batch([net], [2.0], with_sink=[True], block=256); B.solve()\n"
-        "np.save(sys.argv[1], B.solution(0))\n"
+        "L2 = capi.Batch([net, net], [2.0, 1.0], with_sink=[True, False], path='lockstep'); L2.solve()\n"
+        "np.save(sys.argv[1], np.concatenate([B.solution(0), L2.solution(0), L2.solution(1)]))\n"
     ) % (os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
          os.path.join(os.path.dirname(os.path.abspath(__file__)), "simt"))
     outs = []
@@ -138,3 +140,69 @@ def test_simt_batch_mixed_reorthogonalizes(sim):
     B32.solve()
     assert B64.stats(0)[2] == 0 and B32.stats(0)[2] == 0
     assert np.abs(B32.solution(0) - B64.solution(0)).sum() <= 1e-6
+
+
+# ---------------------------------------------------------------- lockstep
+# The lockstep path (fsp_batch_lockstep.hip) in the same CPU build: every
+# instance's own step sequence, bit-identical to the oracle with the HYB
+# operator, in both lane orders; mixed within 1e-6 of FP64 (including the rates that need the
+# second Gram-Schmidt pass); incomplete orthogonalization; the autotuner.
+
+def _oracle_kw(net, t, sink=False, **kw):
+    # the lockstep path stores each generator in the HYB layout
+    opr = O.Operator(net, "hyb", with_sink=sink)
+    v = M.delta_p0(net, with_sink=sink)
+    rc, out, st = O.expv_outputs(opr, v, [t], **kw)
+    assert rc == 0
+    return out[0], st
+
+
+@pytest.mark.parametrize("iop", [False, True])
+def test_simt_lockstep_f64_bit_exact(sim, iop):
+    nets, tf = _instances(3)
+    sinks = [False, True, False]
+    B = capi.Batch(nets, tf, with_sink=sinks, precision="f64", path="lockstep", iop=iop, q_iop=2)
+    assert B.config()[0] == -1
+    B.solve()
+    for i, net in enumerate(nets):
+        want, st = _oracle_kw(net, tf[i], sinks[i], iop=iop, q_iop=2)
+        steps, rej, status = B.stats(i)
+        assert status == 0
+        assert (steps, rej) == (st.i_step, st.n_reject), (i, steps, rej, st.i_step, st.n_reject)
+        assert np.array_equal(B.solution(i), want), i
+    # a second solve restarts from the initial vectors
+    first = [B.solution(i) for i in range(len(nets))]
+    B.solve()
+    for i in range(len(nets)):
+        assert np.array_equal(B.solution(i), first[i])
+
+
+def test_simt_lockstep_mixed(sim):
+    nets, tf = _instances(2, seed=9)
+    nets = nets + [M.toggle_switch((20, 20), rates=TOGGLE_CGS_RATES)]
+    tf = tf + [4.0]
+    B64 = capi.Batch(nets, tf, precision="f64", path="lockstep")
+    B32 = capi.Batch(nets, tf, precision="mixed", path="lockstep")
+    B64.solve()
+    B32.solve()
+    for i in range(len(nets)):
+        assert B32.stats(i)[2] == 0
+        assert np.abs(B32.solution(i) - B64.solution(i)).sum() <= 1e-6
+
+
+def test_simt_lockstep_initial_vector_and_autotune(sim, tmp_path):
+    net = M.birth_death((30, 20), k=3.0, g=0.8)
+    p0 = np.zeros(net.n_states)
+    p0[5 + 31 * 7] = 1.0
+    B = capi.Batch([net, net], [1.5, 1.5], path="lockstep")
+    B.set_initial(1, p0)
+    path = str(tmp_path / "tune.json")
+    ms = B.autotune(path)
+    assert ms > 0
+    cfg = B.config()
+    entry = list(json.load(open(path)).values())[0]
+    assert cfg[0] == -1 and cfg[1] in (0, 4, 16) and entry["path"] == 1 and entry["wg_per_cu"] == cfg[1]
+    B.solve()
+    opr = O.Operator(net, "hyb")
+    rc, out, st = O.expv_outputs(opr, p0.copy(), [1.5])
+    assert np.array_equal(B.solution(1), out[0])
