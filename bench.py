@@ -16,6 +16,9 @@ Also reported on the same line:
                 solves/s and worst L1 error vs FP64 over the output times
   cpu_baseline  the CPU oracle (restatement of the reference algorithm) on a
                 bounded sample of the same solve, extrapolated to solves/s
+  bdf, adjoint  the time-varying BDF solve and one log-likelihood gradient
+  batch         an uneven batch: sequential plain FP64 solves against the
+                lockstep batch path in FP64 and mixed precision
 """
 from __future__ import annotations
 
@@ -210,6 +213,55 @@ def bench_adjoint(net):
             "gradients_per_s": round(1.0 / (t2 - t1), 4), "gradient": [round(x, 6) for x in g]}
 
 
+def bench_batch(count):
+    """Uneven batch (toggle switches and gene-expression networks drawn
+    50/50, rates within a decade of nominal, n log-uniform in [2,048,
+    262,144], t_final uniform in [1, 100], seed 1): every instance solved
+    sequentially by the plain FP64 engine (merged CSR, one solve at a time),
+    then the whole batch by the lockstep path in FP64 and in mixed
+    precision.  Solve times exclude assembly; worst-instance L1 distances
+    are against the sequential FP64 solutions."""
+    nets, tf = M.batch_instances(count, seed=1)
+    seq = []
+    t_seq = 0.0
+    steps_seq = 0
+    for i, net in enumerate(nets):
+        A = capi.Mat(net, "csr", "f64")
+        v = capi.DeviceBuffer.from_host(M.delta_p0(net))
+        E = capi.Expv(A, A.n_rows, float(tf[i]), m=30, tol=1e-8)
+        capi.check(capi.lib().fsp_stream_sync(None))
+        t0 = time.perf_counter()
+        E.solve(v)
+        capi.check(capi.lib().fsp_stream_sync(None))
+        t_seq += time.perf_counter() - t0
+        steps_seq += int(E.stats().i_step)
+        seq.append(v.to_host())
+        E.close()
+        A.close()
+        v.free()
+    out = {"workload": f"{count} uneven instances (toggle / gene expression 50/50, n in "
+                       f"[2048, 262144], t_final in [1, 100], seed 1), m=30 tol=1e-8",
+           "count": count, "states_total": int(sum(n.n_states for n in nets)),
+           "sequential_f64": {"solve_s": round(t_seq, 3), "solves_per_s": round(count / t_seq, 3),
+                              "krylov_steps": steps_seq,
+                              "engine": "single-solve FP64 merged CSR, one instance at a time"}}
+    for prec in ("f64", "mixed"):
+        B = capi.Batch(nets, tf, precision=prec, path="lockstep")
+        B.solve()  # allocations, HYB fill and graph capture
+        t0 = time.perf_counter()
+        B.solve()
+        dt = time.perf_counter() - t0
+        st = [B.stats(i) for i in range(count)]
+        worst = max(float(np.abs(B.solution(i) - seq[i]).sum()) for i in range(count))
+        out[f"lockstep_{prec}"] = {"solve_s": round(dt, 3), "solves_per_s": round(count / dt, 3),
+                                   "speedup_vs_sequential_f64": round(t_seq / dt, 3),
+                                   "krylov_steps": int(sum(x[0] for x in st)),
+                                   "failed": sum(1 for x in st if x[2]),
+                                   "worst_l1_vs_sequential_f64": worst}
+        B.close()
+    return out
+
+
 def pmc_traffic(kernel="k_apply_csr<double>"):
     """HBM bytes per launch of the SpMV kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (counter passes cannot run inside the
@@ -254,7 +306,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the BDF and adjoint-gradient sections")
+                    help="skip the BDF, adjoint-gradient and batch sections")
+    ap.add_argument("--batch-count", type=int, default=64,
+                    help="instances in the uneven batch section")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the calibration and SpMV kernels (for counter passes)")
     args = ap.parse_args()
@@ -316,6 +370,8 @@ def main():
         extras["bdf"] = bench_bdf(net)
         print("[bench] adjoint gradient", flush=True)
         extras["adjoint"] = bench_adjoint(net)
+        print(f"[bench] uneven batch of {args.batch_count}", flush=True)
+        extras["batch"] = bench_batch(args.batch_count)
 
     line = {
         "metric": METRIC, "value": round(solves_s, 4), "unit": "solves/s",

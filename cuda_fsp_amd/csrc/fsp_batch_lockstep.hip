@@ -53,7 +53,7 @@ struct LsArgs {
     const int *tile_r0;     // first row (within its instance) of tile t
     int *atiles;            // tiles of the running instances
     int *n_atiles;          // their count; [1]: running instances
-    double *partials;       // 2 (m + 2) T doubles
+    double *partials;       // 2 max(m + 2, kLsRpt) T doubles
     int T, count, m, M2;
     int j, istart, mode, nk; // Arnoldi iteration, first vector orthogonalized against,
                              // mode (0: dots, 1: avnorm), dots this iteration
@@ -62,7 +62,7 @@ struct LsArgs {
 namespace {
 
 constexpr int kLsRedLen = 2 * (kMaxKrylov + 2) * (kBlock / kWave);
-constexpr int kLsChunk = 8;
+constexpr int kLsChunk = 4;  // dots per pass over a tile's rows
 // Tiles of kLsRows rows: each of the kBlock threads owns rows r0 + tid +
 // u kBlock (u < kLsRpt), so a tile's set-up and its dot reductions are
 // shared by kLsRpt rows per thread.
@@ -309,28 +309,28 @@ __global__ void __launch_bounds__(kBlock) k_ls_dots(LsArgs a)
     }
 }
 
-// w_{j+1} -= V[istart..j] h per running tile (all nv entries, the sink
-// element included), |w|^2 partials in slot 0; h = H(., j), or h2 (pass 1).
+// w_{j+1} -= V[istart..j] h (all nv entries, the sink element included),
+// one row per thread: kLsRpt workgroups per running tile (quarter u of tile
+// t in workgroup slot kLsRpt t + u), |w|^2 partial of each quarter at
+// 2 (kLsRpt t + u).  h = H(., j), or h2 (pass 1).
 template <typename B>
 __global__ void __launch_bounds__(kBlock) k_ls_update(LsArgs a, int pass)
 {
     __shared__ double red[2 * kBlock / kWave];
-    const int na = a.n_atiles[0];
-    for (int q = blockIdx.x; q < na; q += gridDim.x) {
-        const int t = a.atiles[q];
+    const int nq = a.n_atiles[0] * kLsRpt;
+    for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+        const int t = a.atiles[q / kLsRpt];
+        const int sub = q % kLsRpt;
         const int ii = a.tile_inst[t];
         const KState *st = a.st + ii;
         if (!st->active || st->broke) continue;
         const LsInst &I = a.inst[ii];
-        const int r0 = a.tile_r0[t] + (int)threadIdx.x;
+        const uint32_t r = (uint32_t)(a.tile_r0[t] + sub * kBlock + (int)threadIdx.x);
         const double *h = pass ? st->h2 : st->H + (size_t)a.j * a.M2;
         double *out = ls_out(I, a.j);
         const B *Vb = ls_vb<B>(I);
         dd nrm = dd_zero();
-#pragma unroll
-        for (int u = 0; u < kLsRpt; ++u) {
-            const uint32_t r = (uint32_t)(r0 + u * kBlock);
-            if (r >= (uint32_t)I.nv) continue;
+        if (r < (uint32_t)I.nv) {
             double acc = 0.0;
             for (int k = a.istart; k <= a.j; ++k) acc += basis<B>(I.V64, Vb, (size_t)I.nv, k, r) * h[k];
             const double w = -1.0 * acc + out[r];
@@ -339,8 +339,9 @@ __global__ void __launch_bounds__(kBlock) k_ls_update(LsArgs a, int pass)
         }
         nrm = block_sum_dd(nrm, red);
         if (threadIdx.x == 0) {
-            a.partials[2 * (size_t)t] = nrm.s;
-            a.partials[2 * (size_t)t + 1] = nrm.c;
+            const size_t slot = (size_t)t * kLsRpt + sub;
+            a.partials[2 * slot] = nrm.s;
+            a.partials[2 * slot + 1] = nrm.c;
         }
     }
 }
@@ -415,8 +416,9 @@ __global__ void __launch_bounds__(kBlock) k_ls_reorth_fin(LsArgs a)
     }
 }
 
-// Per instance: s = |w_{j+1}|, happy breakdown (src/KryExpvFSP.cu:90-98) or
-// H(j+1, j) = s and sigma_{j+1} = 1 / s.
+// Per instance: s = |w_{j+1}| from the update's quarter-tile partials,
+// happy breakdown (src/KryExpvFSP.cu:90-98) or H(j+1, j) = s and
+// sigma_{j+1} = 1 / s.
 __global__ void __launch_bounds__(kBlock) k_ls_norm(LsArgs a)
 {
     __shared__ double red[2 * kBlock / kWave];
@@ -424,7 +426,8 @@ __global__ void __launch_bounds__(kBlock) k_ls_norm(LsArgs a)
     KState *st = a.st + i;
     if (!st->active || st->broke) return;
     const LsInst &I = a.inst[i];
-    const double s = sqrt(dd_value(sum_dd_partials(a.partials + 2 * (size_t)I.tile0, I.ntiles, red)));
+    const double s = sqrt(dd_value(
+        sum_dd_partials(a.partials + 2 * (size_t)I.tile0 * kLsRpt, I.ntiles * kLsRpt, red)));
     if (threadIdx.x == 0) {
         if (s < st->btol) {
             st->k1 = 0;
@@ -460,13 +463,22 @@ __global__ void __launch_bounds__(kBlock) k_ls_solution(LsArgs a)
         const B *Vb = ls_vb<B>(I);
         const int mx = st->mx;
         const double beta = st->beta;
+        const uint32_t nv = (uint32_t)I.nv;
+        double acc[kLsRpt];
+#pragma unroll
+        for (int u = 0; u < kLsRpt; ++u) acc[u] = 0.0;
+        for (int k = 0; k < mx; ++k) {
+            const double fk = st->F0[k];
+#pragma unroll
+            for (int u = 0; u < kLsRpt; ++u) {
+                const uint32_t r = (uint32_t)(r0 + u * kBlock);
+                if (r < nv) acc[u] += basis<B>(I.V64, Vb, (size_t)nv, k, r) * fk;
+            }
+        }
 #pragma unroll
         for (int u = 0; u < kLsRpt; ++u) {
             const uint32_t r = (uint32_t)(r0 + u * kBlock);
-            if (r >= (uint32_t)I.nv) continue;
-            double acc = 0.0;
-            for (int k = 0; k < mx; ++k) acc += basis<B>(I.V64, Vb, (size_t)I.nv, k, r) * st->F0[k];
-            I.p[r] = beta * acc;
+            if (r < nv) I.p[r] = beta * acc[u];
         }
     }
 }
@@ -682,7 +694,9 @@ static fsp_status ls_build(BatchImpl &B)
     FSP_HIP(hipMalloc(&L.d_cnt, sizeof(int) * 4));
     FSP_HIP(hipMemset(L.d_cnt, 0, sizeof(int) * 4));
     FSP_HIP(hipHostMalloc((void **)&L.h_cnt, sizeof(int) * 4, hipHostMallocDefault));
-    FSP_HIP(hipMalloc(&L.d_partials, sizeof(double) * 2 * (size_t)(m + 2) * (size_t)L.T));
+    // dot partials: (m + 2) slots of T; the update's quarter-tile norms: kLsRpt T
+    const size_t pslots = (size_t)std::max(m + 2, kLsRpt) * (size_t)L.T;
+    FSP_HIP(hipMalloc(&L.d_partials, sizeof(double) * 2 * pslots));
     // step states: constants once (tolerance, decimal powers), the rest per solve
     std::vector<double> p10(kP10Hi - kP10Lo + 1);
     FSP_HIP(hipMemcpy(p10.data(), B.d_p10, sizeof(double) * p10.size(), hipMemcpyDeviceToHost));
@@ -745,9 +759,9 @@ static fsp_status ls_enqueue_step_t(BatchImpl &B, LsHost &L, int grid, hipStream
         if (B.mixed) {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update_dots<Bt>), dim3(grid), dim3(kBlock), 0, s, a);
             hipLaunchKernelGGL(k_ls_reorth_fin, dim3(B.count * a.nk), dim3(kBlock), 0, s, a);
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update<Bt>), dim3(grid), dim3(kBlock), 0, s, a, 1);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update<Bt>), dim3(grid * kLsRpt), dim3(kBlock), 0, s, a, 1);
         } else {
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update<Bt>), dim3(grid), dim3(kBlock), 0, s, a, 0);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update<Bt>), dim3(grid * kLsRpt), dim3(kBlock), 0, s, a, 0);
         }
         hipLaunchKernelGGL(k_ls_norm, dim3(B.count), dim3(kBlock), 0, s, a);
         FSP_HIP(hipGetLastError());
