@@ -38,9 +38,12 @@ constexpr int kCsrPer = 16;  // default LDS products per thread per chunk (cap <
 
 // Phase 1 streams the tile's nonzeros (coalesced) into LDS as products: all
 // of a thread's column indices and values of the chunk are loaded first
-// (independent, non-temporal: they are read once), then all its gathers of
-// x, so a chunk costs two memory round trips; phase 2 lets each thread add
-// its row's products in column order.
+// (non-temporal: they are read once), then all its gathers of x, so a chunk
+// costs two memory round trips.  Every load is unconditional (slots past
+// the chunk re-read its last entry, a valid index) so that the loads of a
+// phase are issued back to back instead of one wait per slot; only the LDS
+// stores are predicated.  Phase 2 lets each thread add its row's products
+// in column order.
 template <typename V, class X, int TB = kBlock, int PER = kCsrPer>
 __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
                                   double *prod, dd &sink)
@@ -53,24 +56,23 @@ __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
     const int ke = (i < A.n) ? A.rp[i + 1] : k1;
     double y = 0.0;
     for (int c0 = k0; c0 < k1; c0 += A.cap) {
-        const int c1 = min(c0 + A.cap, k1);
+        const int c1 = min(c0 + A.cap, k1);  // c1 > c0
         __syncthreads();
         int cl[PER];
         V vl[PER];
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const int k = c0 + (int)threadIdx.x + q * TB;
-            cl[q] = 0;
-            vl[q] = (V)0;
-            if (k < c1) {
-                cl[q] = __builtin_nontemporal_load(A.cols + k);
-                vl[q] = __builtin_nontemporal_load(A.vals + k);
-            }
+            const int k = min(c0 + (int)threadIdx.x + q * TB, c1 - 1);
+            cl[q] = __builtin_nontemporal_load(A.cols + k);
+            vl[q] = __builtin_nontemporal_load(A.vals + k);
         }
+        double xv[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) xv[q] = x(cl[q]);
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int k = c0 + (int)threadIdx.x + q * TB;
-            if (k < c1) prod[k - c0] = (double)vl[q] * x(cl[q]);
+            if (k < c1) prod[k - c0] = (double)vl[q] * xv[q];
         }
         __syncthreads();
         const int a = max(kb, c0), b = min(ke, c1);
@@ -98,10 +100,51 @@ struct HybF {
     int U;
 };
 
+// Row i with a compile-time slot count: all slot values and columns are
+// loaded first (non-temporal: streamed once per product), then all gathers
+// of x, then the sum in slot order, so a row costs two memory round trips
+// whatever U is.
+template <int NU, typename V, class X>
+__device__ inline double hyb_row_n(const HybF<V> &A, uint32_t i, const X &x, dd &sink)
+{
+    const double xi = x(i);
+    double y = __builtin_nontemporal_load(A.diag + i) * xi;
+    V v[NU];
+    int c[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const size_t o = (size_t)u * A.n + i;
+        v[u] = __builtin_nontemporal_load(A.slot_vals + o);
+        c[u] = __builtin_nontemporal_load(A.slot_cols + o);
+    }
+    double xc[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) xc[u] = x(c[u]);
+#pragma unroll
+    for (int u = 0; u < NU; ++u) y += (double)v[u] * xc[u];
+    if (A.sink_out) sink = dd_add_prod(sink, A.sink_out[i], xi);
+    return y;
+}
+
 template <typename V, class X>
 __device__ inline double hyb_row(const HybF<V> &A, uint32_t i, const X &x, dd &sink)
 {
     if (i >= A.n) return 0.0;
+    switch (A.U) {
+    case 1: return hyb_row_n<1>(A, i, x, sink);
+    case 2: return hyb_row_n<2>(A, i, x, sink);
+    case 3: return hyb_row_n<3>(A, i, x, sink);
+    case 4: return hyb_row_n<4>(A, i, x, sink);
+    case 5: return hyb_row_n<5>(A, i, x, sink);
+    case 6: return hyb_row_n<6>(A, i, x, sink);
+    case 7: return hyb_row_n<7>(A, i, x, sink);
+    case 8: return hyb_row_n<8>(A, i, x, sink);
+    case 9: return hyb_row_n<9>(A, i, x, sink);
+    case 10: return hyb_row_n<10>(A, i, x, sink);
+    case 11: return hyb_row_n<11>(A, i, x, sink);
+    case 12: return hyb_row_n<12>(A, i, x, sink);
+    default: break;
+    }
     const double xi = x(i);
     double y = A.diag[i] * xi;
     for (int u = 0; u < A.U; ++u) {

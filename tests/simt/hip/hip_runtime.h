@@ -288,6 +288,8 @@ inline T atomicAdd(T *p, T v)
     *p = old + v;
     return old;
 }
+// clang's non-temporal load hint: a plain load on the CPU
+#define __builtin_nontemporal_load(p) (*(p))
 using std::isfinite;
 using std::isinf;
 using std::isnan;
