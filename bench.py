@@ -5,13 +5,15 @@
 A "step" is one complete FSP solve of the 3 x 512 x 512 gene-expression
 network (786,432 states): Krylov expv (m = 30, tol = 1e-8) from
 p0 = delta(G=0, n=0, m=0) to t = 60 with 30 equispaced output times, FP64
-merged-CSR generator.  `value` = solves/s with all inputs resident in HBM.
+HYB generator (diagonal + one ELL slot per unique net change).  `value` =
+solves/s with all inputs resident in HBM.
 
 Also reported on the same line:
-  roofline      merged-CSR FP64 SpMV on the 16,777,216-state five-species
-                network (the HBM-scale shape): algorithmic bytes / launch
-                time from HIP events on the launch stream, against 8 TB/s
-  spmv          the same for HYB / KRON / mixed precision
+  roofline      HYB FP64 SpMV on the 16,777,216-state five-species network
+                (the HBM-scale shape): algorithmic bytes / launch time from
+                HIP events on the launch stream, against 8 TB/s
+  spmv          the same for merged CSR / KRON / mixed precision
+  csr_solve     the headline solve with the merged-CSR generator
   mixed         the same solve with the FP32-valued / FP32-basis form:
                 solves/s and worst L1 error vs FP64 over the output times
   cpu_baseline  the CPU oracle (restatement of the reference algorithm) on a
@@ -78,8 +80,8 @@ def solve_outputs(E, v, p0_dev, n, t_out, out_buf):
     capi.check(capi.lib().fsp_stream_sync(None))
 
 
-def bench_solve(net, precision, steps, warmup, t_out):
-    A = capi.Mat(net, "csr", precision)
+def bench_solve(net, precision, steps, warmup, t_out, fmt="hyb"):
+    A = capi.Mat(net, fmt, precision)
     n = A.n_rows
     p0 = M.delta_p0(net)
     p0_dev = capi.DeviceBuffer.from_host(p0)
@@ -262,11 +264,12 @@ def bench_batch(count):
     return out
 
 
-def pmc_traffic(kernel="k_apply_csr<double>"):
+def pmc_traffic(kernel="k_apply_hyb<10, double>"):
     """HBM bytes per launch of the SpMV kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (counter passes cannot run inside the
     timed bench); None when the summary is absent."""
     path = os.path.join(ROOT, "profiles", "chain_16m_spmv_pmc_traffic.md")
+    kernel = kernel.replace(" ", "")
     try:
         for line in open(path):
             if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
@@ -298,6 +301,23 @@ def kernels_only(reps):
     print(f"[kernels] calibration bytes per launch: {8 * n}", flush=True)
 
 
+def batch_only(count, precision):
+    """One lockstep solve of the uneven batch (for kernel-trace runs):
+    prints the solve time, the Krylov step counts and the state-steps."""
+    nets, tf = M.batch_instances(count, seed=1)
+    B = capi.Batch(nets, tf, precision=precision, path="lockstep")
+    t0 = time.perf_counter()
+    B.solve()
+    dt = time.perf_counter() - t0
+    steps = [B.stats(i)[0] for i in range(count)]
+    ns = [n.n_states for n in nets]
+    print(json.dumps({"batch": count, "precision": precision, "solve_s": round(dt, 3),
+                      "krylov_steps": int(sum(steps)), "batch_steps": int(max(steps)),
+                      "state_steps": int(sum(s * n for s, n in zip(steps, ns))),
+                      "states": int(sum(ns))}), flush=True)
+    B.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,9 +331,15 @@ def main():
                     help="instances in the uneven batch section")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the calibration and SpMV kernels (for counter passes)")
+    ap.add_argument("--batch-only", type=int, default=0,
+                    help="only one lockstep solve of an uneven batch of this many instances")
+    ap.add_argument("--batch-precision", default="f64", choices=["f64", "mixed"])
     args = ap.parse_args()
     if args.kernels_only:
         kernels_only(5)
+        return
+    if args.batch_only:
+        batch_only(args.batch_only, args.batch_precision)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path "
@@ -330,6 +356,10 @@ def main():
     solves_s = args.steps / dt
     print(f"[bench] FP64 {solves_s:.3f} solves/s, {st.i_step} Krylov steps/solve", flush=True)
 
+    print("[bench] FP64 solve, merged CSR generator", flush=True)
+    dtc, stc, refc = bench_solve(net, "f64", args.steps, args.warmup, t_out, fmt="csr")
+    l1_csr = float(np.abs(refc - ref).sum(axis=1).max())
+
     print("[bench] mixed solve", flush=True)
     dtm, stm, mixed = bench_solve(net, "mixed", args.steps, args.warmup, t_out)
     l1_mixed = float(np.abs(mixed - ref).sum(axis=1).max())
@@ -338,7 +368,7 @@ def main():
     ev = HipEvents()
     big = M.five_species_chain()
     spmv = []
-    for fmt, prec in (("csr", "f64"), ("hyb", "f64"), ("kron", "f64"), ("csr", "mixed"),
+    for fmt, prec in (("hyb", "f64"), ("csr", "f64"), ("kron", "f64"), ("csr", "mixed"),
                       ("hyb", "mixed"), ("kron", "mixed")):
         print(f"[bench] SpMV {fmt}/{prec} on n={big.n_states}", flush=True)
         spmv.append(bench_spmv(big, fmt, prec, args.spmv_reps, ev))
@@ -348,8 +378,8 @@ def main():
                 "traffic": pmc_traffic(),
                 "traffic_source": "profiles/chain_16m_spmv_pmc_traffic.md (FETCH_SIZE, WRITE_SIZE "
                                   "scaled by a read and a write kernel of known size)",
-                "kernel": "k_apply_csr<double> (merged CSR SpMV, FP64)",
-                "workload": f"five_species_chain n={big.n_states} nnz={head['nnz']}",
+                "kernel": "k_apply_hyb<10, double> (HYB SpMV, FP64, 10 slots)",
+                "workload": f"five_species_chain n={big.n_states} slots={head['nnz']}",
                 "algorithmic_bytes": head["algorithmic_bytes"],
                 "us_per_launch": head["us_per_apply"]}
 
@@ -379,12 +409,14 @@ def main():
         "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "gene_expression 3x512x512 (786,432 states): Krylov expv "
-                               "m=30 tol=1e-8 to t=60 at 30 output times, FP64 merged CSR",
+                               "m=30 tol=1e-8 to t=60 at 30 output times, FP64 HYB generator",
                    "krylov_steps_per_solve": int(st.i_step),
                    "matvecs_per_solve": int(st.n_matvec)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "spmv": spmv,
+        "csr_solve": {"solves_per_s": round(args.steps / dtc, 4),
+                      "krylov_steps_per_solve": int(stc.i_step), "worst_l1_vs_hyb": l1_csr},
         "mixed": {"solves_per_s": round(args.steps / dtm, 4),
                   "speedup_vs_f64": round(dt / dtm, 3), "worst_l1_vs_f64": l1_mixed,
                   "krylov_steps_per_solve": int(stm.i_step)},

@@ -483,7 +483,9 @@ __global__ void __launch_bounds__(kCsrApplyRows) k_apply_csr(CsrF<V> A, PlainX X
     }
 }
 
-template <typename V>
+// NU > 0: the slot count as a compile-time constant (registers sized for
+// it); NU = 0: any slot count.
+template <int NU, typename V>
 __global__ void __launch_bounds__(kBlock) k_apply_hyb(HybF<V> A, PlainX X, double *__restrict__ y,
                                                       int accumulate, uint32_t tiles,
                                                       double *__restrict__ partials)
@@ -493,10 +495,34 @@ __global__ void __launch_bounds__(kBlock) k_apply_hyb(HybF<V> A, PlainX X, doubl
     for (TileWalk w = tile_walk(tiles); w.t < w.end; w.t += w.step) {
         const uint32_t t = w.t;
         const uint32_t i = t * kBlock + threadIdx.x;
-        const double v = hyb_row<V>(A, i, X, sink);
+        double v = 0.0;
+        if (NU > 0) {
+            if (i < A.n) v = hyb_row_n<(NU > 0 ? NU : 1)>(A, i, X, sink);
+        } else {
+            v = hyb_row<V>(A, i, X, sink);
+        }
         if (i < A.n) y[i] = accumulate ? y[i] + v : v;
     }
     if (A.sink_out) put_dd_partial(partials, sink, red);
+}
+
+template <typename V>
+static void launch_hyb(const HybF<V> &A, const double *d_x, double *d_y, int accumulate,
+                       uint32_t tiles, double *partials, int grid, hipStream_t s)
+{
+#define FSP_HYB_CASE(K)                                                                   \
+    case K:                                                                               \
+        k_apply_hyb<K, V><<<grid, kBlock, 0, s>>>(A, PlainX{d_x}, d_y, accumulate, tiles, \
+                                                  partials);                              \
+        return;
+    switch (A.U) {
+        FSP_HYB_CASE(1) FSP_HYB_CASE(2) FSP_HYB_CASE(3) FSP_HYB_CASE(4)
+        FSP_HYB_CASE(5) FSP_HYB_CASE(6) FSP_HYB_CASE(7) FSP_HYB_CASE(8)
+        FSP_HYB_CASE(9) FSP_HYB_CASE(10) FSP_HYB_CASE(11) FSP_HYB_CASE(12)
+    default:
+        k_apply_hyb<0, V><<<grid, kBlock, 0, s>>>(A, PlainX{d_x}, d_y, accumulate, tiles, partials);
+    }
+#undef FSP_HYB_CASE
 }
 
 template <int D, typename V>
@@ -623,12 +649,16 @@ fsp_status mat_apply(MatImpl &M, const double *d_x, double *d_y, int accumulate,
         }
         return FSP_OK;
     } else if (M.format == FSP_HYB) {
+        // one tile per workgroup
+        const int hgrid = grid_for(tiles, 1 << 30);
         if (mixed)
-            k_apply_hyb<float><<<grid, kBlock, 0, s>>>(hyb_view<float>(M), PlainX{d_x}, d_y,
-                                                      accumulate, tiles, M.partials);
+            launch_hyb<float>(hyb_view<float>(M), d_x, d_y, accumulate, tiles, M.partials, hgrid, s);
         else
-            k_apply_hyb<double><<<grid, kBlock, 0, s>>>(hyb_view<double>(M), PlainX{d_x}, d_y,
-                                                       accumulate, tiles, M.partials);
+            launch_hyb<double>(hyb_view<double>(M), d_x, d_y, accumulate, tiles, M.partials, hgrid, s);
+        FSP_HIP(hipGetLastError());
+        if (M.with_sink) k_finalize_sink<<<1, kBlock, 0, s>>>(M.partials, hgrid, d_y + M.n, accumulate);
+        FSP_HIP(hipGetLastError());
+        return FSP_OK;
     } else {
         return (mixed && M.hm.has_factors)
                    ? launch_kron<float>(M, d_x, d_y, accumulate, s, 0, -1, 0.0)
