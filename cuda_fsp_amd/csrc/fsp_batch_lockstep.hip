@@ -13,8 +13,8 @@
 // in the HYB layout (FP64 diagonal + one ELL slot per unique net change),
 // filled on the device from its model once per batch: a streamed row costs
 // a few loads instead of the Kronecker factor products.  With the FP32
-// basis (mixed) the slot values are FP32 and the update is a second
-// Gram-Schmidt pass, as in the single-solve engine.
+// basis (mixed) the slot values are FP32 and the basis is projected out with
+// its Gram matrix (one pass over the basis), as in the single-solve engine.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -53,7 +53,7 @@ struct LsArgs {
     const int *tile_r0;     // first row (within its instance) of tile t
     int *atiles;            // tiles of the running instances
     int *n_atiles;          // their count; [1]: running instances
-    double *partials;       // 2 max(m + 2, kLsRpt) T doubles
+    double *partials;       // 2 max(2m + 4, kLsRpt) T doubles
     int T, count, m, M2;
     int j, istart, mode, nk; // Arnoldi iteration, first vector orthogonalized against,
                              // mode (0: dots, 1: avnorm), dots this iteration
@@ -61,7 +61,10 @@ struct LsArgs {
 
 namespace {
 
-constexpr int kLsRedLen = 2 * (kMaxKrylov + 2) * (kBlock / kWave);
+// Slots per tile: dots 0..nk-1, the sink at nk, with an FP32 basis the
+// Gram column at nk+1..2nk.
+constexpr int kLsSlots = 2 * kMaxKrylov + 4;
+constexpr int kLsRedLen = 2 * kLsSlots * (kBlock / kWave);
 constexpr int kLsChunk = 4;  // dots per pass over a tile's rows
 // Tiles of kLsRows rows: each of the kBlock threads owns rows r0 + tid +
 // u kBlock (u < kLsRpt), so a tile's set-up and its dot reductions are
@@ -78,9 +81,9 @@ __device__ inline B *ls_vb(const LsInst &I)
     return std::is_same<B, double>::value ? (B *)(void *)I.V64 : (B *)(void *)I.V32;
 }
 
-// Wave butterflies of dot k into red (kLsRedLen), then after one barrier the
-// tile's partials of dots 0..nk: dot k of tile t at 2 (k T + t).  Leaves a
-// barrier for the caller's next use of red.
+// Wave butterflies of slot k into red (kLsRedLen); ls_store then writes,
+// after one barrier, the tile's partials of slots 0..last: slot k of tile t
+// at 2 (k T + t), and leaves a barrier for the caller's next use of red.
 __device__ inline void ls_put_wave(double *red, int k, dd v)
 {
     constexpr int NW = kBlock / kWave;
@@ -92,11 +95,11 @@ __device__ inline void ls_put_wave(double *red, int k, dd v)
     }
 }
 
-__device__ inline void ls_store(const LsArgs &a, int t, int nk, double *red)
+__device__ inline void ls_store(const LsArgs &a, int t, int last, double *red)
 {
     constexpr int NW = kBlock / kWave;
     __syncthreads();
-    if ((int)threadIdx.x <= nk) {
+    if ((int)threadIdx.x <= last) {
         const int k = threadIdx.x;
         dd s = dd_zero();
 #pragma unroll
@@ -235,32 +238,42 @@ __global__ void __launch_bounds__(kBlock) k_ls_arn(LsArgs a)
                     nrm = dd_add_prod(nrm, y[u], y[u]);
             }
         }
+        constexpr bool GRAM = gram_basis<B>();
         if (a.mode == 0) {
             for (int c0 = 0; c0 < nk; c0 += kLsChunk) {
-                dd acc[kLsChunk];
+                dd acc[kLsChunk], accg[kLsChunk];
 #pragma unroll
-                for (int v = 0; v < kLsChunk; ++v) acc[v] = dd_zero();
+                for (int v = 0; v < kLsChunk; ++v) acc[v] = accg[v] = dd_zero();
 #pragma unroll
                 for (int u = 0; u < kLsRpt; ++u) {
                     const uint32_t r = (uint32_t)(r0 + u * kBlock);
                     if (r >= (uint32_t)I.n) continue;
+                    const double xj = GRAM ? basis<B>(I.V64, Vb, (size_t)I.nv, a.j, r) : 0.0;
                     double vk[kLsChunk];
 #pragma unroll
                     for (int v = 0; v < kLsChunk; ++v)
                         vk[v] = basis<B>(I.V64, Vb, (size_t)I.nv, min(a.istart + c0 + v, a.j), r);
 #pragma unroll
-                    for (int v = 0; v < kLsChunk; ++v)
-                        if (c0 + v < nk) acc[v] = dd_add_prod(acc[v], y[u], vk[v]);
+                    for (int v = 0; v < kLsChunk; ++v) {
+                        if (c0 + v < nk) {
+                            acc[v] = dd_add_prod(acc[v], y[u], vk[v]);
+                            if (GRAM) accg[v] = dd_add_prod(accg[v], xj, vk[v]);
+                        }
+                    }
                 }
 #pragma unroll
-                for (int v = 0; v < kLsChunk; ++v)
-                    if (c0 + v < nk) ls_put_wave(red, c0 + v, acc[v]);
+                for (int v = 0; v < kLsChunk; ++v) {
+                    if (c0 + v < nk) {
+                        ls_put_wave(red, c0 + v, acc[v]);
+                        if (GRAM) ls_put_wave(red, nk + 1 + c0 + v, accg[v]);
+                    }
+                }
             }
         } else {
             ls_put_wave(red, 0, nrm);
         }
         ls_put_wave(red, nk, sink);
-        ls_store(a, t, nk, red);
+        ls_store(a, t, (GRAM && a.mode == 0) ? 2 * nk : nk, red);
     }
 }
 
@@ -285,14 +298,23 @@ __global__ void __launch_bounds__(kBlock) k_ls_dots(LsArgs a)
         xs = kry_scale<B>(in[I.n], st->sigma[a.j], a.j);
     }
     dd sd = sum_dd_partials(a.partials + 2 * ((size_t)k * a.T + I.tile0), I.ntiles, red);
+    dd sg = dd_zero();
+    if (gram_basis<B>() && a.mode == 0)
+        sg = sum_dd_partials(a.partials + 2 * ((size_t)(a.nk + 1 + k) * a.T + I.tile0), I.ntiles, red);
     if (threadIdx.x == 0) {
         if (a.mode == 0) {
             const int kk = a.istart + k;
             if (I.with_sink) {
                 const double vk = (kk == a.j) ? xs : basis<B>(I.V64, Vb, nv, kk, (uint32_t)I.n);
                 sd = dd_add_prod(sd, ys, vk);
+                if (gram_basis<B>()) sg = dd_add_prod(sg, xs, vk);
             }
             st->H[kk + a.j * a.M2] = dd_value(sd);
+            if (gram_basis<B>()) {
+                const double g = dd_value(sg);
+                st->G[kk + a.j * kGramLd] = g;
+                st->G[a.j + kk * kGramLd] = g;
+            }
             if (k == 0) {
                 st->n_matvec += 1;
                 if (I.with_sink) {
@@ -309,12 +331,21 @@ __global__ void __launch_bounds__(kBlock) k_ls_dots(LsArgs a)
     }
 }
 
-// w_{j+1} -= V[istart..j] h (all nv entries, the sink element included),
-// one row per thread: kLsRpt workgroups per running tile (quarter u of tile
-// t in workgroup slot kLsRpt t + u), |w|^2 partial of each quarter at
-// 2 (kLsRpt t + u).  h = H(., j), or h2 (pass 1).
+// FP32 basis: per instance, H(., j) <- 2h - G h (gram_coefs).
+__global__ void __launch_bounds__(kBlock) k_ls_gram_coef(LsArgs a)
+{
+    KState *st = a.st + blockIdx.x;
+    if (!st->active || st->broke) return;
+    HIP_DYNAMIC_SHARED(double, lds)
+    gram_coefs(st, a.istart, a.j, a.M2, lds);
+}
+
+// w_{j+1} -= V[istart..j] H(., j) (all nv entries, the sink element
+// included), one row per thread: kLsRpt workgroups per running tile
+// (quarter u of tile t in workgroup slot kLsRpt t + u), |w|^2 partial of
+// each quarter at 2 (kLsRpt t + u).
 template <typename B>
-__global__ void __launch_bounds__(kBlock) k_ls_update(LsArgs a, int pass)
+__global__ void __launch_bounds__(kBlock) k_ls_update(LsArgs a)
 {
     __shared__ double red[2 * kBlock / kWave];
     const int nq = a.n_atiles[0] * kLsRpt;
@@ -326,7 +357,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_update(LsArgs a, int pass)
         if (!st->active || st->broke) continue;
         const LsInst &I = a.inst[ii];
         const uint32_t r = (uint32_t)(a.tile_r0[t] + sub * kBlock + (int)threadIdx.x);
-        const double *h = pass ? st->h2 : st->H + (size_t)a.j * a.M2;
+        const double *h = st->H + (size_t)a.j * a.M2;
         double *out = ls_out(I, a.j);
         const B *Vb = ls_vb<B>(I);
         dd nrm = dd_zero();
@@ -343,76 +374,6 @@ __global__ void __launch_bounds__(kBlock) k_ls_update(LsArgs a, int pass)
             a.partials[2 * slot] = nrm.s;
             a.partials[2 * slot + 1] = nrm.c;
         }
-    }
-}
-
-// FP32 basis: first pass of the update, then partial dots V_k . w_{j+1}
-// (slots 0..nk-1) for the second pass.
-template <typename B>
-__global__ void __launch_bounds__(kBlock) k_ls_update_dots(LsArgs a)
-{
-    __shared__ double red[kLsRedLen];
-    const int na = a.n_atiles[0];
-    for (int q = blockIdx.x; q < na; q += gridDim.x) {
-        const int t = a.atiles[q];
-        const int ii = a.tile_inst[t];
-        const KState *st = a.st + ii;
-        if (!st->active || st->broke) continue;
-        const LsInst &I = a.inst[ii];
-        const int r0 = a.tile_r0[t] + (int)threadIdx.x;
-        const double *h = st->H + (size_t)a.j * a.M2;
-        double *out = ls_out(I, a.j);
-        const B *Vb = ls_vb<B>(I);
-        double w[kLsRpt];
-#pragma unroll
-        for (int u = 0; u < kLsRpt; ++u) {
-            const uint32_t r = (uint32_t)(r0 + u * kBlock);
-            w[u] = 0.0;
-            if (r >= (uint32_t)I.nv) continue;
-            double acc = 0.0;
-            for (int k = a.istart; k <= a.j; ++k) acc += basis<B>(I.V64, Vb, (size_t)I.nv, k, r) * h[k];
-            w[u] = -1.0 * acc + out[r];
-            out[r] = w[u];
-        }
-        const int nk = a.nk;
-        for (int c0 = 0; c0 < nk; c0 += kLsChunk) {
-            dd acc[kLsChunk];
-#pragma unroll
-            for (int v = 0; v < kLsChunk; ++v) acc[v] = dd_zero();
-#pragma unroll
-            for (int u = 0; u < kLsRpt; ++u) {
-                const uint32_t r = (uint32_t)(r0 + u * kBlock);
-                if (r >= (uint32_t)I.nv) continue;
-                double vk[kLsChunk];
-#pragma unroll
-                for (int v = 0; v < kLsChunk; ++v)
-                    vk[v] = basis<B>(I.V64, Vb, (size_t)I.nv, min(a.istart + c0 + v, a.j), r);
-#pragma unroll
-                for (int v = 0; v < kLsChunk; ++v)
-                    if (c0 + v < nk) acc[v] = dd_add_prod(acc[v], w[u], vk[v]);
-            }
-#pragma unroll
-            for (int v = 0; v < kLsChunk; ++v)
-                if (c0 + v < nk) ls_put_wave(red, c0 + v, acc[v]);
-        }
-        ls_put_wave(red, nk, dd_zero());
-        ls_store(a, t, nk, red);
-    }
-}
-
-// Block (i, k): h2(istart + k) of instance i, H(istart + k, j) += h2.
-__global__ void __launch_bounds__(kBlock) k_ls_reorth_fin(LsArgs a)
-{
-    __shared__ double red[2 * kBlock / kWave];
-    const int i = blockIdx.x / a.nk, k = blockIdx.x % a.nk;
-    KState *st = a.st + i;
-    if (!st->active || st->broke) return;
-    const LsInst &I = a.inst[i];
-    const double s = dd_value(sum_dd_partials(a.partials + 2 * ((size_t)k * a.T + I.tile0), I.ntiles, red));
-    if (threadIdx.x == 0) {
-        const int kk = a.istart + k;
-        st->h2[kk] = s;
-        st->H[kk + a.j * a.M2] += s;
     }
 }
 
@@ -694,8 +655,9 @@ static fsp_status ls_build(BatchImpl &B)
     FSP_HIP(hipMalloc(&L.d_cnt, sizeof(int) * 4));
     FSP_HIP(hipMemset(L.d_cnt, 0, sizeof(int) * 4));
     FSP_HIP(hipHostMalloc((void **)&L.h_cnt, sizeof(int) * 4, hipHostMallocDefault));
-    // dot partials: (m + 2) slots of T; the update's quarter-tile norms: kLsRpt T
-    const size_t pslots = (size_t)std::max(m + 2, kLsRpt) * (size_t)L.T;
+    // dot partials: 2m + 4 slots of T (dots, sink, Gram column); the
+    // update's quarter-tile norms: kLsRpt T
+    const size_t pslots = (size_t)std::max(2 * m + 4, kLsRpt) * (size_t)L.T;
     FSP_HIP(hipMalloc(&L.d_partials, sizeof(double) * 2 * pslots));
     // step states: constants once (tolerance, decimal powers), the rest per solve
     std::vector<double> p10(kP10Hi - kP10Lo + 1);
@@ -756,13 +718,9 @@ static fsp_status ls_enqueue_step_t(BatchImpl &B, LsHost &L, int grid, hipStream
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_dots<Bt>), dim3(B.count * a.nk), dim3(kBlock), 0, s, a);
         FSP_HIP(hipGetLastError());
         if (last) break;
-        if (B.mixed) {
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update_dots<Bt>), dim3(grid), dim3(kBlock), 0, s, a);
-            hipLaunchKernelGGL(k_ls_reorth_fin, dim3(B.count * a.nk), dim3(kBlock), 0, s, a);
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update<Bt>), dim3(grid * kLsRpt), dim3(kBlock), 0, s, a, 1);
-        } else {
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update<Bt>), dim3(grid * kLsRpt), dim3(kBlock), 0, s, a, 0);
-        }
+        if (B.mixed)
+            hipLaunchKernelGGL(k_ls_gram_coef, dim3(B.count), dim3(kBlock), kGramLdsBytes, s, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update<Bt>), dim3(grid * kLsRpt), dim3(kBlock), 0, s, a);
         hipLaunchKernelGGL(k_ls_norm, dim3(B.count), dim3(kBlock), 0, s, a);
         FSP_HIP(hipGetLastError());
     }

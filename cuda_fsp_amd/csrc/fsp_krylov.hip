@@ -10,11 +10,14 @@
 //   k_arn_dots   : H(k, j) = sum of partials (fixed order), sink element
 //   k_arn_update : w_{j+1} -= V h (gemv N of :79-81), partial |w|^2
 //   k_arn_norm   : s = |w|, happy breakdown test, H(j+1, j) = s, sigma = 1/s
-// With an FP32 basis (mixed precision) the update is a second Gram-Schmidt
-// pass: k_arn_update_dots (w -= V h, partial dots V_k . w), k_reorth_fin
-// (h2, H(k, j) += h2), k_arn_update with h2.  One pass, as the reference
-// does in FP64, lets the loss of orthogonality of an FP32 basis compound by
-// |w| / h(j+1, j) per step until the basis is degenerate.
+// With an FP32 basis (mixed precision) one classical Gram-Schmidt pass, as
+// the reference does in FP64, lets the loss of orthogonality compound by
+// |w| / h(j+1, j) per step until the basis is degenerate.  The mixed path
+// therefore projects with the Gram matrix G = V^T V of the stored basis:
+// k_arn_* also returns the dots V_k . V_j (column j of G), k_gram_coef turns
+// the Hessenberg column h into c = 2h - G h (the second Gram-Schmidt pass
+// h2 = V^T (w - V h) = h - G h folded into the first), and k_arn_update
+// subtracts V c in one sweep over the basis.
 // then k_expm_control (one workgroup: Pade expm of tau*H, error estimate,
 // reject loop, step-size update) and k_solution (v = beta V F0).
 // Reductions are deterministic: per-wave butterflies, per-block partials in
@@ -82,7 +85,10 @@ struct ArnArgs {
 constexpr int kRedLen = 2 * kBlock / kWave;
 // LDS of the Arnoldi kernels' end reduction: one dd per (dot, wave), for up
 // to m + 1 dots plus the sink sum.
-constexpr int kArnRedLen = 2 * (kMaxKrylov + 2) * (kBlock / kWave);
+// Partial slots per Arnoldi kernel: dots 0..nk-1, the sink sum at nk and,
+// with an FP32 basis, the Gram column at nk+1..2nk.
+constexpr int kArnSlots = 2 * kMaxKrylov + 4;
+constexpr int kArnRedLen = 2 * kArnSlots * (kBlock / kWave);
 constexpr int kDotChunk = 8;
 
 template <typename B>
@@ -107,14 +113,14 @@ __device__ inline void arn_epi(const ArnArgs &a, uint32_t i, double y, double xi
         nrm = dd_add_prod(nrm, y, y);
 }
 
-// Merges the per-wave dd sums stored in red (dot k of wave w at 2 (k NW + w))
-// in wave order and writes the block's partials: dot k of block b at
-// 2 (k G + b), the sink sum after the nk dots.
-__device__ inline void arn_store_partials(const ArnArgs &a, int nk, const double *red)
+// Merges the per-wave dd sums stored in red (slot k of wave w at
+// 2 (k NW + w)) in wave order and writes the block's partials of slots
+// 0..last: slot k of block b at 2 (k G + b).
+__device__ inline void arn_store_partials(const ArnArgs &a, int last, const double *red)
 {
     constexpr int NW = kBlock / kWave;
     __syncthreads();
-    if ((int)threadIdx.x <= nk) {
+    if ((int)threadIdx.x <= last) {
         const int kk = threadIdx.x;
         dd s = dd_zero();
 #pragma unroll
@@ -142,36 +148,49 @@ __device__ inline void arn_put_wave(double *red, int k, dd v)
 // valid), so few registers are live and a chunk costs one memory round trip.
 // Every wave reduces each dot by butterflies into red (kArnRedLen doubles);
 // one barrier, then the block partials.
+// With an FP32 basis the same pass also forms the Gram dots V_k . V_j
+// (slots nk+1..2nk) from the same basis loads.
 template <typename B>
 __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red)
 {
+    constexpr bool GRAM = gram_basis<B>();
+    constexpr int CH = GRAM ? kDotChunk / 2 : kDotChunk;
     const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
     if (a.mode == 0) {
-        for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
-            dd acc[kDotChunk];
+        for (int c0 = 0; c0 < nk; c0 += CH) {
+            dd acc[CH], accg[CH];
 #pragma unroll
-            for (int q = 0; q < kDotChunk; ++q) acc[q] = dd_zero();
+            for (int q = 0; q < CH; ++q) acc[q] = accg[q] = dd_zero();
             for (TileWalk t = tile_walk(a.tiles); t.t < t.end; t.t += t.step) {
                 const uint32_t i = t.t * kBlock + threadIdx.x;
                 if (i >= a.n) continue;
                 const double y = a.out[i];
-                double vk[kDotChunk];
+                const double xj = GRAM ? basis<B>(a.V64, (const B *)a.Vb, a.nv, a.j, i) : 0.0;
+                double vk[CH];
 #pragma unroll
-                for (int q = 0; q < kDotChunk; ++q)
+                for (int q = 0; q < CH; ++q)
                     vk[q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + c0 + q, a.j), i);
 #pragma unroll
-                for (int q = 0; q < kDotChunk; ++q)
-                    if (c0 + q < nk) acc[q] = dd_add_prod(acc[q], y, vk[q]);
+                for (int q = 0; q < CH; ++q) {
+                    if (c0 + q < nk) {
+                        acc[q] = dd_add_prod(acc[q], y, vk[q]);
+                        if (GRAM) accg[q] = dd_add_prod(accg[q], xj, vk[q]);
+                    }
+                }
             }
 #pragma unroll
-            for (int q = 0; q < kDotChunk; ++q)
-                if (c0 + q < nk) arn_put_wave(red, c0 + q, acc[q]);
+            for (int q = 0; q < CH; ++q) {
+                if (c0 + q < nk) {
+                    arn_put_wave(red, c0 + q, acc[q]);
+                    if (GRAM) arn_put_wave(red, nk + 1 + c0 + q, accg[q]);
+                }
+            }
         }
     } else {
         arn_put_wave(red, 0, nrm);
     }
     arn_put_wave(red, nk, sink);
-    arn_store_partials(a, nk, red);
+    arn_store_partials(a, (GRAM && a.mode == 0) ? 2 * nk : nk, red);
 }
 
 // Register-accumulator variant for the callback path (k_cb_dots), which
@@ -283,15 +302,24 @@ __global__ void __launch_bounds__(kBlock) k_arn_dots(ArnArgs a, int with_sink, i
     }
     const int k = blockIdx.x;
     dd sd = sum_dd_partials(a.partials + 2 * (size_t)k * a.G, a.G, red);
+    dd sg = dd_zero();
+    if (gram_basis<B>() && a.mode == 0)
+        sg = sum_dd_partials(a.partials + 2 * (size_t)(nk + 1 + k) * a.G, a.G, red);
     if (threadIdx.x == 0) {
         if (a.mode == 0) {
             const int kk = a.istart + k;
             if (with_sink) {
                 const double vk = (kk == a.j) ? xs : basis<B>(a.V64, (const B *)a.Vb, a.nv, kk, a.n);
                 sd = dd_add_prod(sd, ys, vk);
+                if (gram_basis<B>()) sg = dd_add_prod(sg, xs, vk);
             }
             const double s = dd_value(sd);
             st->H[kk + a.j * M2] = s;
+            if (gram_basis<B>()) {
+                const double g = dd_value(sg);
+                st->G[kk + a.j * kGramLd] = g;
+                st->G[a.j + kk * kGramLd] = g;
+            }
             if (k == 0) {
                 st->n_matvec += 1;
                 if (with_sink) {
@@ -308,15 +336,23 @@ __global__ void __launch_bounds__(kBlock) k_arn_dots(ArnArgs a, int with_sink, i
     }
 }
 
-// w_{j+1} -= V[istart..j] h; partial |w|^2.  h = H(istart..j, j), or the
-// second-pass coefficients h2 (pass 1).
+// FP32 basis: H(., j) <- 2h - G h (gram_coefs), one workgroup.
+__global__ void __launch_bounds__(kBlock) k_gram_coef(ArnArgs a, int M2)
+{
+    KState *st = a.st;
+    if (!st->active || st->broke) return;
+    HIP_DYNAMIC_SHARED(double, lds)
+    gram_coefs(st, a.istart, a.j, M2, lds);
+}
+
+// w_{j+1} -= V[istart..j] H(istart..j, j); partial |w|^2.
 template <typename B>
-__global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2, int pass)
+__global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2)
 {
     if (arn_skip<B>(a)) return;
     __shared__ double red[kRedLen];
     const KState *st = a.st;
-    const double *h = pass ? st->h2 : st->H + (size_t)a.j * M2;
+    const double *h = st->H + (size_t)a.j * M2;
     dd nrm = dd_zero();
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += gridDim.x * kBlock) {
         double acc = 0.0;
@@ -330,64 +366,6 @@ __global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2, int pa
     if (threadIdx.x == 0) {
         a.partials[2 * blockIdx.x] = t.s;
         a.partials[2 * blockIdx.x + 1] = t.c;
-    }
-}
-
-// First pass of the FP32-basis update: w_{j+1} -= V[istart..j] H(., j) over
-// the whole vector (sink element included), then the partial dots
-// V_k . w_{j+1} for the second pass, kDotChunk at a time over the rows this
-// thread updated (same mapping, so its own writes are re-read).  Partials of
-// dot k at 2 (k G + b), as the Arnoldi kernels store them.
-template <typename B>
-__global__ void __launch_bounds__(kBlock) k_arn_update_dots(ArnArgs a, int M2)
-{
-    if (arn_skip<B>(a)) return;
-    __shared__ double red[kArnRedLen];
-    const KState *st = a.st;
-    const double *h = st->H + (size_t)a.j * M2;
-    const int nk = a.j - a.istart + 1;
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += stride) {
-        double acc = 0.0;
-        for (int k = a.istart; k <= a.j; ++k)
-            acc += basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) * h[k];
-        a.out[i] = -1.0 * acc + a.out[i];
-    }
-    for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
-        dd acc[kDotChunk];
-#pragma unroll
-        for (int q = 0; q < kDotChunk; ++q) acc[q] = dd_zero();
-        for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += stride) {
-            const double w = a.out[i];
-            double vk[kDotChunk];
-#pragma unroll
-            for (int q = 0; q < kDotChunk; ++q)
-                vk[q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + c0 + q, a.j), i);
-#pragma unroll
-            for (int q = 0; q < kDotChunk; ++q)
-                if (c0 + q < nk) acc[q] = dd_add_prod(acc[q], w, vk[q]);
-        }
-#pragma unroll
-        for (int q = 0; q < kDotChunk; ++q)
-            if (c0 + q < nk) arn_put_wave(red, c0 + q, acc[q]);
-    }
-    arn_put_wave(red, nk, dd_zero());  // no separate sink slot
-    arn_store_partials(a, nk, red);
-}
-
-// Second-pass coefficients: block k sums the partials of dot k in fixed
-// order; h2(istart + k) = that sum, H(istart + k, j) += it.
-__global__ void __launch_bounds__(kBlock) k_reorth_fin(ArnArgs a, int M2)
-{
-    KState *st = a.st;
-    if (!st->active || st->broke) return;
-    __shared__ double red[kRedLen];
-    const int k = blockIdx.x;
-    const double s = dd_value(sum_dd_partials(a.partials + 2 * (size_t)k * a.G, a.G, red));
-    if (threadIdx.x == 0) {
-        const int kk = a.istart + k;
-        st->h2[kk] = s;
-        st->H[kk + a.j * M2] += s;
     }
 }
 
@@ -518,7 +496,7 @@ static fsp_status expv_alloc(ExpvImpl &E)
     int cap = 1024;
     if (const char *g = std::getenv("FSP_ARN_GRID")) cap = std::max(8, std::atoi(g));
     E.G = grid_for(E.tiles, cap);
-    FSP_HIP(hipMalloc(&E.partials, 2 * sizeof(double) * (size_t)E.G * (kMaxKrylov + 4)));
+    FSP_HIP(hipMalloc(&E.partials, 2 * sizeof(double) * (size_t)E.G * kArnSlots));
     KState &h = *E.h_st;
     h.t_final = 0.0;
     h.t_now = 0.0;
@@ -637,13 +615,8 @@ static fsp_status enqueue_step(ExpvImpl &E, double *d_v)
         }
         FSP_HIP(hipGetLastError());
         if (last) break;
-        if (E.mixed) {
-            k_arn_update_dots<B><<<E.G, kBlock, 0, E.s>>>(a, M2);
-            k_reorth_fin<<<nk, kBlock, 0, E.s>>>(a, M2);
-            k_arn_update<B><<<E.G, kBlock, 0, E.s>>>(a, M2, 1);
-        } else {
-            k_arn_update<B><<<E.G, kBlock, 0, E.s>>>(a, M2, 0);
-        }
+        if (E.mixed) k_gram_coef<<<1, kBlock, kGramLdsBytes, E.s>>>(a, M2);
+        k_arn_update<B><<<E.G, kBlock, 0, E.s>>>(a, M2);
         k_arn_norm<<<1, kBlock, 0, E.s>>>(a, M2, E.G);
         FSP_HIP(hipGetLastError());
     }

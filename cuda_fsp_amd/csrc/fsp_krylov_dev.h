@@ -12,8 +12,9 @@ namespace fsp {
 
 // One solve's step state: scalars of KryExpvFSP::step, the (m+2)^2
 // Hessenberg matrix (column-major), the scaling of each Arnoldi vector
-// (sigma_j = 1 / h(j, j-1)), the first column of exp(tau H) (F0) and the
-// decimal powers used by the step-size rounding.
+// (sigma_j = 1 / h(j, j-1)), the first column of exp(tau H) (F0), the Gram
+// matrix of an FP32 basis and the decimal powers used by the step-size
+// rounding.
 struct KState {
     double beta, btol, t_now, t_new, t_final, tau, xm, err_loc, avnorm, fact;
     double tol, anorm;
@@ -21,13 +22,45 @@ struct KState {
     double sigma[kMaxKrylov + 2];
     double F0[kMaxKrylov + 2];
     double H[(kMaxKrylov + 2) * (kMaxKrylov + 2)];
-    double h2[kMaxKrylov + 2];         // second Gram-Schmidt pass (FP32 basis)
+    // FP32 basis: Gram matrix V_k . V_l of the stored basis, leading
+    // dimension kMaxKrylov + 1
+    double G[(kMaxKrylov + 1) * (kMaxKrylov + 1)];
     double p10[kP10Hi - kP10Lo + 1];   // correctly rounded 1e-300 .. 1e300
     double *tau_log;                   // accepted step sizes, by step index
     long long tau_cap;
     int mb, k1, broke, active, mx, status, pad0, pad1;
     long long i_step, n_reject, n_breakdown, n_matvec;
 };
+
+constexpr int kGramLd = kMaxKrylov + 1;  // leading dimension of KState::G
+
+// An FP32 basis (B = float) is projected out with its Gram matrix.
+template <typename B>
+constexpr bool gram_basis() { return !std::is_same<B, double>::value; }
+
+// Gram-corrected Gram-Schmidt coefficients for column j over the window
+// istart..j, one workgroup: c_k = 2 h_k - sum_l G(k, l) h_l (l ascending),
+// with h = H(., j) = V^T w on entry and c written back as H(., j).  This
+// folds the second classical Gram-Schmidt pass, whose coefficients are
+// V^T (w - V h) = h - G h, into the first, so the update sweeps the basis
+// once.  lds: (kMaxKrylov + 1)^2 + (kMaxKrylov + 2) doubles.
+__device__ inline void gram_coefs(KState *st, int istart, int j, int M2, double *lds)
+{
+    const int w = j - istart + 1;
+    double *gw = lds, *hw = lds + (kMaxKrylov + 1) * (kMaxKrylov + 1);
+    double *h = st->H + (size_t)j * M2;
+    for (int e = threadIdx.x; e < w * w; e += blockDim.x)
+        gw[e] = st->G[(istart + e % w) + (istart + e / w) * kGramLd];
+    for (int e = threadIdx.x; e < w; e += blockDim.x) hw[e] = h[istart + e];
+    __syncthreads();
+    for (int k = threadIdx.x; k < w; k += blockDim.x) {
+        double s = 0.0;
+        for (int l = 0; l < w; ++l) s += gw[k + l * w] * hw[l];
+        h[istart + k] = 2.0 * hw[k] - s;
+    }
+}
+
+constexpr size_t kGramLdsBytes = sizeof(double) * ((kMaxKrylov + 1) * (kMaxKrylov + 1) + kMaxKrylov + 2);
 
 // Krylov vector j = in * sigma_j as the basis stores it: rounded to B for
 // j >= 1; V_0 (the scaled solution vector) is kept in FP64 with any B, so
@@ -47,13 +80,17 @@ struct KryX {
 };
 
 // Basis vector k, entry i: FP64 V_0..V_m (B = double), or V_0 in FP64 and
-// V_1..V_m in B.
+// V_1..V_m in B.  Both candidate loads are issued (each address is valid
+// for any k) and one is selected, so a loop over k has no branch between
+// its loads.
 template <typename B>
 __device__ inline double basis(const double *__restrict__ V64, const B *__restrict__ Vb,
                                size_t nv, int k, uint32_t i)
 {
     if (std::is_same<B, double>::value) return V64[(size_t)k * nv + i];
-    return k == 0 ? V64[i] : (double)Vb[(size_t)(k - 1) * nv + i];
+    const double v0 = V64[i];
+    const double vb = (double)Vb[(size_t)(k > 0 ? k - 1 : 0) * nv + i];
+    return k == 0 ? v0 : vb;
 }
 
 template <typename B>
