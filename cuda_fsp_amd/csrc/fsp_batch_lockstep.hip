@@ -95,6 +95,19 @@ __device__ inline void ls_put_wave(double *red, int k, dd v)
     }
 }
 
+// FP64 sum (no compensation) into the same slot layout: used by the FP32
+// basis, whose dots need no bit-exact match with the oracle.
+__device__ inline void ls_put_wave_f(double *red, int k, double v)
+{
+    constexpr int NW = kBlock / kWave;
+    const double t = wave_sum(v);
+    if (threadIdx.x % kWave == 0) {
+        const int w = threadIdx.x / kWave;
+        red[2 * (k * NW + w)] = t;
+        red[2 * (k * NW + w) + 1] = 0.0;
+    }
+}
+
 __device__ inline void ls_store(const LsArgs &a, int t, int last, double *red)
 {
     constexpr int NW = kBlock / kWave;
@@ -239,33 +252,56 @@ __global__ void __launch_bounds__(kBlock) k_ls_arn(LsArgs a)
             }
         }
         constexpr bool GRAM = gram_basis<B>();
-        if (a.mode == 0) {
+        if (a.mode == 0 && !GRAM) {
             for (int c0 = 0; c0 < nk; c0 += kLsChunk) {
-                dd acc[kLsChunk], accg[kLsChunk];
+                dd acc[kLsChunk];
 #pragma unroll
-                for (int v = 0; v < kLsChunk; ++v) acc[v] = accg[v] = dd_zero();
+                for (int v = 0; v < kLsChunk; ++v) acc[v] = dd_zero();
 #pragma unroll
                 for (int u = 0; u < kLsRpt; ++u) {
                     const uint32_t r = (uint32_t)(r0 + u * kBlock);
                     if (r >= (uint32_t)I.n) continue;
-                    const double xj = GRAM ? basis<B>(I.V64, Vb, (size_t)I.nv, a.j, r) : 0.0;
                     double vk[kLsChunk];
 #pragma unroll
                     for (int v = 0; v < kLsChunk; ++v)
                         vk[v] = basis<B>(I.V64, Vb, (size_t)I.nv, min(a.istart + c0 + v, a.j), r);
 #pragma unroll
-                    for (int v = 0; v < kLsChunk; ++v) {
-                        if (c0 + v < nk) {
-                            acc[v] = dd_add_prod(acc[v], y[u], vk[v]);
-                            if (GRAM) accg[v] = dd_add_prod(accg[v], xj, vk[v]);
-                        }
+                    for (int v = 0; v < kLsChunk; ++v)
+                        if (c0 + v < nk) acc[v] = dd_add_prod(acc[v], y[u], vk[v]);
+                }
+#pragma unroll
+                for (int v = 0; v < kLsChunk; ++v)
+                    if (c0 + v < nk) ls_put_wave(red, c0 + v, acc[v]);
+            }
+        } else if (a.mode == 0) {
+            // FP32 basis: the dots and the Gram column V_k . V_j from the same
+            // loads, in plain FP64 (a fixed summation tree, so still
+            // deterministic)
+            constexpr int CH = kLsChunk;
+            for (int c0 = 0; c0 < nk; c0 += CH) {
+                double acc[CH], accg[CH];
+#pragma unroll
+                for (int v = 0; v < CH; ++v) acc[v] = accg[v] = 0.0;
+#pragma unroll
+                for (int u = 0; u < kLsRpt; ++u) {
+                    const uint32_t r = (uint32_t)(r0 + u * kBlock);
+                    if (r >= (uint32_t)I.n) continue;
+                    const double xj = basis<B>(I.V64, Vb, (size_t)I.nv, a.j, r);
+                    double vk[CH];
+#pragma unroll
+                    for (int v = 0; v < CH; ++v)
+                        vk[v] = basis<B>(I.V64, Vb, (size_t)I.nv, min(a.istart + c0 + v, a.j), r);
+#pragma unroll
+                    for (int v = 0; v < CH; ++v) {
+                        acc[v] = fma(y[u], vk[v], acc[v]);
+                        accg[v] = fma(xj, vk[v], accg[v]);
                     }
                 }
 #pragma unroll
-                for (int v = 0; v < kLsChunk; ++v) {
+                for (int v = 0; v < CH; ++v) {
                     if (c0 + v < nk) {
-                        ls_put_wave(red, c0 + v, acc[v]);
-                        if (GRAM) ls_put_wave(red, nk + 1 + c0 + v, accg[v]);
+                        ls_put_wave_f(red, c0 + v, acc[v]);
+                        ls_put_wave_f(red, nk + 1 + c0 + v, accg[v]);
                     }
                 }
             }

@@ -149,40 +149,71 @@ __device__ inline void arn_put_wave(double *red, int k, dd v)
 // Every wave reduces each dot by butterflies into red (kArnRedLen doubles);
 // one barrier, then the block partials.
 // With an FP32 basis the same pass also forms the Gram dots V_k . V_j
-// (slots nk+1..2nk) from the same basis loads.
+// (slots nk+1..2nk) from the same basis loads, both in plain FP64 (a fixed
+// summation tree, so still deterministic; no bit-exact oracle match is
+// asked of the FP32 path).
+__device__ inline void arn_put_wave_f(double *red, int k, double v)
+{
+    constexpr int NW = kBlock / kWave;
+    const double t = wave_sum(v);
+    if (threadIdx.x % kWave == 0) {
+        const int w = threadIdx.x / kWave;
+        red[2 * (k * NW + w)] = t;
+        red[2 * (k * NW + w) + 1] = 0.0;
+    }
+}
+
 template <typename B>
 __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red)
 {
     constexpr bool GRAM = gram_basis<B>();
-    constexpr int CH = GRAM ? kDotChunk / 2 : kDotChunk;
     const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
-    if (a.mode == 0) {
-        for (int c0 = 0; c0 < nk; c0 += CH) {
-            dd acc[CH], accg[CH];
+    if (a.mode == 0 && !GRAM) {
+        for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
+            dd acc[kDotChunk];
 #pragma unroll
-            for (int q = 0; q < CH; ++q) acc[q] = accg[q] = dd_zero();
+            for (int q = 0; q < kDotChunk; ++q) acc[q] = dd_zero();
             for (TileWalk t = tile_walk(a.tiles); t.t < t.end; t.t += t.step) {
                 const uint32_t i = t.t * kBlock + threadIdx.x;
                 if (i >= a.n) continue;
                 const double y = a.out[i];
-                const double xj = GRAM ? basis<B>(a.V64, (const B *)a.Vb, a.nv, a.j, i) : 0.0;
-                double vk[CH];
+                double vk[kDotChunk];
 #pragma unroll
-                for (int q = 0; q < CH; ++q)
+                for (int q = 0; q < kDotChunk; ++q)
                     vk[q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + c0 + q, a.j), i);
 #pragma unroll
-                for (int q = 0; q < CH; ++q) {
-                    if (c0 + q < nk) {
-                        acc[q] = dd_add_prod(acc[q], y, vk[q]);
-                        if (GRAM) accg[q] = dd_add_prod(accg[q], xj, vk[q]);
-                    }
+                for (int q = 0; q < kDotChunk; ++q)
+                    if (c0 + q < nk) acc[q] = dd_add_prod(acc[q], y, vk[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q)
+                if (c0 + q < nk) arn_put_wave(red, c0 + q, acc[q]);
+        }
+    } else if (a.mode == 0) {
+        for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
+            double acc[kDotChunk], accg[kDotChunk];
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q) acc[q] = accg[q] = 0.0;
+            for (TileWalk t = tile_walk(a.tiles); t.t < t.end; t.t += t.step) {
+                const uint32_t i = t.t * kBlock + threadIdx.x;
+                if (i >= a.n) continue;
+                const double y = a.out[i];
+                const double xj = basis<B>(a.V64, (const B *)a.Vb, a.nv, a.j, i);
+                double vk[kDotChunk];
+#pragma unroll
+                for (int q = 0; q < kDotChunk; ++q)
+                    vk[q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + c0 + q, a.j), i);
+#pragma unroll
+                for (int q = 0; q < kDotChunk; ++q) {
+                    acc[q] = fma(y, vk[q], acc[q]);
+                    accg[q] = fma(xj, vk[q], accg[q]);
                 }
             }
 #pragma unroll
-            for (int q = 0; q < CH; ++q) {
+            for (int q = 0; q < kDotChunk; ++q) {
                 if (c0 + q < nk) {
-                    arn_put_wave(red, c0 + q, acc[q]);
-                    if (GRAM) arn_put_wave(red, nk + 1 + c0 + q, accg[q]);
+                    arn_put_wave_f(red, c0 + q, acc[q]);
+                    arn_put_wave_f(red, nk + 1 + c0 + q, accg[q]);
                 }
             }
         }
