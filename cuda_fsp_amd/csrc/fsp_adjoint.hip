@@ -11,10 +11,11 @@
 //     dLL/dtheta_r = c_r * integral_0^{t_K} lambda(s)^T A_r p(s) ds,
 //     lambda(s) = sum_{k : t_k > s} exp((t_k - s) A^T) g_k,
 //     g_k[i] = N_k[b(i)] / P_k[b(i)] where P_k > floor (else 0).
-// lambda is integrated backwards with the transposed generator (Krylov expv
-// on A^T, matrix free), p forwards; the integral uses Gauss-Legendre panels
-// graded geometrically towards both ends of every observation interval (the
-// integrand has transients of width ~1/|A| there).  Checkpointing: the
+// lambda is integrated backwards with the transposed generator (the fused
+// Krylov expv on A^T, assembled in HYB: row j holds the outflow rates of
+// state j at its targets' columns), p forwards; the integral uses
+// Gauss-Legendre panels graded geometrically towards both ends of every
+// observation interval (the integrand has transients of width ~1/|A| there).  Checkpointing: the
 // forward pass keeps p only at the observation times; the backward pass
 // recomputes p at one interval's nodes from its checkpoint.
 #include <algorithm>
@@ -22,6 +23,7 @@
 #include <cstring>
 #include <vector>
 
+#include "fsp_assemble.h"
 #include "fsp_formats.h"
 
 namespace fsp {
@@ -70,21 +72,18 @@ void launch_seed(const DevModel *dm, const int *keep, const double *P, const dou
                        floor_val, g, n);
 }
 
-__global__ void k_axpy_inplace(double a, const double *x, double *y, int64_t n)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        y[i] += a * x[i];
-}
-
-// partial dd sums of x . y per block
-__global__ void k_dot_partials(const double *x, const double *y, int64_t n, double *partials)
+// y = a y + x, with per-workgroup dd partials of |y|^2
+__global__ void k_axpby_sumsq(double a, const double *__restrict__ x, double *__restrict__ y,
+                              int64_t n, double *__restrict__ partials)
 {
     __shared__ double red[2 * kBlock / kWave];
     dd s = dd_zero();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        s = dd_add_prod(s, x[i], y[i]);
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = a * y[i] + x[i];
+        y[i] = v;
+        s = dd_add_prod(s, v, v);
+    }
     s = block_sum_dd(s, red);
     if (threadIdx.x == 0) {
         partials[2 * blockIdx.x] = s.s;
@@ -97,6 +96,89 @@ __global__ void k_reduce_one(const double *partials, int G, double *out)
     __shared__ double red[2 * kBlock / kWave];
     const dd s = sum_dd_partials(partials, G, red);
     if (threadIdx.x == 0) out[0] = dd_value(s);
+}
+
+__global__ void k_scale(double a, double *y, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        y[i] *= a;
+}
+
+constexpr int kGradChunk = 16;  // reactions per workgroup of k_grad_terms
+
+// Per-workgroup dd partials of lambda^T A_r p for the reactions
+// r = 16 blockIdx.y + k, all of them in one pass over p and lambda, in the
+// outflow (column) form of A_r p:
+//     lambda^T A_r p = sum_j a_r(x_j) p_j (lambda_{j + delta_r} - lambda_j),
+// where lambda_{j + delta_r} is 0 when the target of state j leaves the box
+// (the outflow stays on the diagonal).  Partials: [r][block][2].
+template <int D>
+__global__ void k_grad_terms(const DevModel *__restrict__ dm, const double *__restrict__ tab_g,
+                             int use_lds, const double *__restrict__ table,
+                             const double *__restrict__ p, const double *__restrict__ lam,
+                             uint32_t n, double *__restrict__ partials)
+{
+    extern __shared__ double smem[];
+    __shared__ double red[2 * kBlock / kWave];
+    const double *tab = tab_g;
+    if (tab_g && use_lds) {
+        stage_table(smem, tab_g, dm->compact_size);
+        tab = smem;
+    }
+    const int r0 = (int)blockIdx.y * kGradChunk;
+    const int R = dm->R;
+    dd acc[kGradChunk];
+#pragma unroll
+    for (int k = 0; k < kGradChunk; ++k) acc[k] = dd_zero();
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        int xs[D];
+        decode<D>(i, dm, xs);
+        const double pi = p[i], li = lam[i];
+#pragma unroll
+        for (int k = 0; k < kGradChunk; ++k) {
+            const int r = r0 + k;
+            if (r < R && !dm->rx[r].zero_nu) {
+                const RxnDev &rx = dm->rx[r];
+                const double lt = target_leaves<D>(rx, xs, dm) ? 0.0 : lam[(int64_t)i + rx.delta];
+                const double a = prop_at<D>(dm, tab, table, r, i, xs, 0);
+                acc[k] = dd_add_prod(acc[k], a * pi, lt - li);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kGradChunk; ++k) {
+        const int r = r0 + k;
+        if (r >= R) break;
+        const dd sk = block_sum_dd(acc[k], red);
+        if (threadIdx.x == 0) {
+            partials[2 * ((size_t)r * gridDim.x + blockIdx.x)] = sk.s;
+            partials[2 * ((size_t)r * gridDim.x + blockIdx.x) + 1] = sk.c;
+        }
+    }
+}
+
+// grad[r] += w * (sum of reaction r's partials), one workgroup per reaction
+__global__ void k_grad_accum(const double *__restrict__ partials, int G, double w,
+                             double *__restrict__ grad)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    const int r = blockIdx.x;
+    const dd s = sum_dd_partials(partials + 2 * (size_t)r * G, G, red);
+    if (threadIdx.x == 0) grad[r] += w * dd_value(s);
+}
+
+template <int D>
+void launch_grad_terms(const MatImpl &M, const double *p, const double *lam, int G,
+                       double *partials, hipStream_t s)
+{
+    const DevModel &dm = M.hm.dm;
+    const size_t tb = sizeof(double) * M.hm.compact.size();
+    const int use_lds = (M.d_compact && !M.hm.compact.empty() && tb <= kLdsTableMax) ? 1 : 0;
+    const dim3 grid((unsigned)G, (unsigned)((dm.R + kGradChunk - 1) / kGradChunk));
+    hipLaunchKernelGGL(k_grad_terms<D>, grid, dim3(kBlock), use_lds ? tb : 0, s, M.d_model,
+                       M.d_compact, use_lds, M.hm.prop_table_dev, p, lam, (uint32_t)M.n,
+                       partials);
 }
 
 // Gauss-Legendre nodes and weights on [-1, 1] (Newton on P_q).
@@ -146,19 +228,6 @@ void graded_rule(double a, double b, int q, int L, std::vector<double> &s, std::
     }
 }
 
-struct TransposeCtx {
-    MatImpl *M;
-    hipStream_t s;
-    fsp_status st;
-};
-
-void transpose_cb(void *ctx, const double *x, double *y)
-{
-    auto *c = (TransposeCtx *)ctx;
-    const fsp_status st = mat_apply_transpose(*c->M, x, y, 1, c->s);
-    if (st != FSP_OK) c->st = st;
-}
-
 struct Work {
     std::vector<double *> bufs;
     double *alloc(int64_t n)
@@ -176,6 +245,7 @@ struct Work {
 
 struct LikRun {
     fsp_mat A = nullptr;
+    fsp_mat AT = nullptr;  // transposed generator of the backward pass
     MatImpl *M = nullptr;
     fsp_expv E = nullptr;
     fsp_expv Eb = nullptr;
@@ -183,20 +253,10 @@ struct LikRun {
     {
         if (Eb) fsp_expv_destroy(Eb);
         if (E) fsp_expv_destroy(E);
+        if (AT) fsp_mat_destroy(AT);
         if (A) fsp_mat_destroy(A);
     }
 };
-
-fsp_status dot_dev(const double *x, const double *y, int64_t n, double *partials, double *dout,
-                   double *host, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_dot_partials, dim3(vgrid(n)), dim3(kBlock), 0, s, x, y, n, partials);
-    hipLaunchKernelGGL(k_reduce_one, dim3(1), dim3(kBlock), 0, s, partials, vgrid(n), dout);
-    FSP_HIP(hipGetLastError());
-    FSP_HIP(hipMemcpyAsync(host, dout, sizeof(double), hipMemcpyDeviceToHost, s));
-    FSP_HIP(hipStreamSynchronize(s));
-    return FSP_OK;
-}
 
 fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const double *d_p0,
                        const fsp_lik_data *data, const fsp_grad_options *opt, double *h_ll,
@@ -241,21 +301,14 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
     Work W;
     double *p = W.alloc(n), *tmp = W.alloc(n), *lam = W.alloc(n), *g = W.alloc(n);
     double *P = W.alloc(nb), *cnt = W.alloc(nb * K);
-    double *partials = W.alloc(2 * kGridMax + 2), *dout = W.alloc(1);
     std::vector<double *> ckpt((size_t)K + 1, nullptr);
-    if (!p || !tmp || !lam || !g || !P || !cnt || !partials || !dout)
+    if (!p || !tmp || !lam || !g || !P || !cnt)
         return fail(FSP_ERR_NOMEM, "likelihood workspace");
     int *d_keep = nullptr;
     FSP_HIP(hipMalloc(&d_keep, sizeof(int) * (size_t)d));
     W.bufs.push_back((double *)d_keep);
     FSP_HIP(hipMemcpy(d_keep, keep.data(), sizeof(int) * (size_t)d, hipMemcpyHostToDevice));
     FSP_HIP(hipMemcpy(cnt, data->counts, sizeof(double) * (size_t)(nb * K), hipMemcpyHostToDevice));
-    double *h_pin = nullptr;
-    FSP_HIP(hipHostMalloc((void **)&h_pin, sizeof(double), hipHostMallocDefault));
-    struct PinFree {
-        double *p;
-        ~PinFree() { (void)hipHostFree(p); }
-    } pin_guard{h_pin};
 
     // forward pass: p at the observation times (checkpoints), LL
     FSP_HIP(hipMemcpyAsync(p, d_p0, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
@@ -281,12 +334,22 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
     *h_ll = ll;
     if (!h_grad) return FSP_OK;
 
-    // backward pass over the intervals (t_{k-1}, t_k], k = K..1
-    std::vector<double> grad((size_t)R, 0.0);
-    TransposeCtx tctx{run.M, s, FSP_OK};
-    FSP_TRY(fsp_expv_create(nullptr, &transpose_cb, &tctx, n, 1.0, o.m, o.tol, 0, 2, 1.0, s, &run.Eb));
+    // backward pass over the intervals (t_{k-1}, t_k], k = K..1: lambda by
+    // the fused Krylov engine on A^T stored in HYB; the sensitivity
+    // integrals accumulate on the device
+    FSP_TRY(fsp_mat_create_transposed(model, c.data(), s, &run.AT));
+    FSP_TRY(fsp_expv_create(run.AT, nullptr, nullptr, n, 1.0, o.m, o.tol, 0, 2, 1.0, s, &run.Eb));
     FSP_TRY(fsp_expv_set_limits(run.Eb, 1000, 1000000));
     FSP_HIP(hipMemsetAsync(lam, 0, sizeof(double) * (size_t)n, s));
+    const int Gg = vgrid(n);
+    const int Rc = (R + kGradChunk - 1) / kGradChunk * kGradChunk;
+    double *gpart = W.alloc(2 * (int64_t)Rc * Gg), *d_grad = W.alloc(R + 1);
+    if (!gpart || !d_grad) return fail(FSP_ERR_NOMEM, "likelihood gradient workspace");
+    FSP_HIP(hipMemsetAsync(d_grad, 0, sizeof(double) * (size_t)(R + 1), s));
+    // The Krylov error test is absolute in |v| (src/KryExpvFSP.cu:122-146),
+    // and |lambda| reaches N/P ~ 1e6 or more, so lambda is integrated as
+    // sig * lam with |lam|_2 = 1 (tol relative to |lambda|).
+    double sig = 1.0;
     const int nodes_per = 2 * o.grading_levels * o.quad_nodes;
     std::vector<double *> node_p((size_t)nodes_per, nullptr);
     for (int q = 0; q < nodes_per; ++q) {
@@ -295,13 +358,25 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
     }
     const DevModel *dm = M.d_model;
     for (int k = K - 1; k >= 0; --k) {
-        // lambda(t_k) += g_k
+        // lambda(t_k) += g_k, renormalised: lambda = sig * lam with |lam| = 1
         FSP_TRY(fsp_marginal(ckpt[k + 1], d, model->bounds, keep.data(), P, s));
         FSP_SWITCH_D(d, launch_seed<D_>(dm, d_keep, P, cnt + (int64_t)k * nb, data->floor_val, g,
                                         (uint32_t)n, s));
+        hipLaunchKernelGGL(k_axpby_sumsq, dim3(Gg), dim3(kBlock), 0, s, sig, g, lam, n, gpart);
+        hipLaunchKernelGGL(k_reduce_one, dim3(1), dim3(kBlock), 0, s, gpart, Gg, d_grad + R);
         FSP_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_axpy_inplace, dim3(vgrid(n)), dim3(kBlock), 0, s, 1.0, g, lam, n);
+        double ss = 0.0;
+        FSP_HIP(hipMemcpyAsync(&ss, d_grad + R, sizeof(double), hipMemcpyDeviceToHost, s));
+        FSP_HIP(hipStreamSynchronize(s));
+        sig = std::sqrt(ss);
         const double a = k ? data->times[k - 1] : 0.0, b = data->times[k];
+        if (!(sig > 0.0)) {
+            // lambda vanishes on this interval: no contribution, and it stays 0
+            sig = 1.0;
+            continue;
+        }
+        if (!std::isfinite(sig)) return fail(FSP_ERR_NOT_CONVERGED, "adjoint vector is not finite");
+        hipLaunchKernelGGL(k_scale, dim3(Gg), dim3(kBlock), 0, s, 1.0 / sig, lam, n);
         std::vector<double> sn, wn;
         graded_rule(a, b, o.quad_nodes, o.grading_levels, sn, wn);
         // p at the nodes from the checkpoint at a
@@ -317,20 +392,18 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
         for (int q = nodes_per - 1; q >= 0; --q) {
             FSP_TRY(fsp_expv_set_final_time(run.Eb, b - sn[q]));
             FSP_TRY(fsp_expv_solve(run.Eb, lam));
-            if (tctx.st != FSP_OK) return tctx.st;
-            for (int r = 0; r < R; ++r) {
-                FSP_TRY(mat_apply_term(M, r, 1.0, node_p[q], tmp, 0, s));
-                double dv = 0.0;
-                FSP_TRY(dot_dev(lam, tmp, n, partials, dout, h_pin, s));
-                dv = *h_pin;
-                grad[r] += wn[q] * dv;
-            }
+            FSP_SWITCH_D(d, launch_grad_terms<D_>(M, node_p[q], lam, Gg, gpart, s));
+            hipLaunchKernelGGL(k_grad_accum, dim3(R), dim3(kBlock), 0, s, gpart, Gg, wn[q] * sig,
+                               d_grad);
+            FSP_HIP(hipGetLastError());
         }
         // lambda down to the interval start
         FSP_TRY(fsp_expv_set_final_time(run.Eb, b - a));
         FSP_TRY(fsp_expv_solve(run.Eb, lam));
-        if (tctx.st != FSP_OK) return tctx.st;
     }
+    std::vector<double> grad((size_t)R, 0.0);
+    FSP_HIP(hipMemcpyAsync(grad.data(), d_grad, sizeof(double) * (size_t)R, hipMemcpyDeviceToHost, s));
+    FSP_HIP(hipStreamSynchronize(s));
     for (int r = 0; r < R; ++r) h_grad[r] = c[r] * grad[r];
     return FSP_OK;
 }

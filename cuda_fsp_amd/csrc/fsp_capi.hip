@@ -348,8 +348,9 @@ __attribute__((visibility("hidden"))) fsp_status fsp_mat_internal(fsp_mat A, Mat
     return FSP_OK;
 }
 
-fsp_status fsp_mat_create(const fsp_model *model, fsp_format format, fsp_precision precision,
-                          int with_sink, const double *coefs, void *stream, fsp_mat *out)
+static fsp_status mat_create(const fsp_model *model, fsp_format format, fsp_precision precision,
+                             int with_sink, const double *coefs, void *stream, int transposed,
+                             fsp_mat *out)
 {
     if (!out) return fail(FSP_ERR_INVALID, "out is NULL");
     *out = nullptr;
@@ -373,6 +374,7 @@ fsp_status fsp_mat_create(const fsp_model *model, fsp_format format, fsp_precisi
     M.format = format;
     M.prec = precision;
     M.with_sink = with_sink ? 1 : 0;
+    M.transposed = transposed;
     M.coefs.assign(coefs, coefs + M.hm.dm.R);
     st = mat_assemble(M, (hipStream_t)stream);
     if (st != FSP_OK) {
@@ -382,6 +384,18 @@ fsp_status fsp_mat_create(const fsp_model *model, fsp_format format, fsp_precisi
     }
     *out = h;
     return FSP_OK;
+}
+
+fsp_status fsp_mat_create(const fsp_model *model, fsp_format format, fsp_precision precision,
+                          int with_sink, const double *coefs, void *stream, fsp_mat *out)
+{
+    return mat_create(model, format, precision, with_sink, coefs, stream, 0, out);
+}
+
+__attribute__((visibility("hidden"))) fsp_status fsp_mat_create_transposed(
+    const fsp_model *model, const double *coefs, void *stream, fsp_mat *out)
+{
+    return mat_create(model, FSP_HYB, FSP_F64, 0, coefs, stream, 1, out);
 }
 
 fsp_status fsp_mat_destroy(fsp_mat A)

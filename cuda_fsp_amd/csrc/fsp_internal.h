@@ -108,6 +108,9 @@ struct MatImpl {
     fsp_format format;
     fsp_precision prec;
     int with_sink;
+    // HYB F64 without a sink only: the arrays hold the transpose A^T (row j:
+    // outflow rates of state j at the columns of its targets)
+    int transposed = 0;
     int64_t n, n_rows;
     std::vector<double> coefs;       // coefficients the stored values hold
     // device copies
@@ -169,3 +172,7 @@ fsp_status vec_sum(const double *d_x, int64_t n, double *h_sum, hipStream_t s);
 // exported from the shared object.
 extern "C" __attribute__((visibility("hidden"))) fsp_status fsp_mat_internal(fsp_mat A,
                                                                           fsp::MatImpl **impl);
+// Transposed generator A^T of a model (HYB, F64, no sink), for solves with
+// A^T such as the adjoint pass of the likelihood gradient; not exported.
+extern "C" __attribute__((visibility("hidden"))) fsp_status fsp_mat_create_transposed(
+    const fsp_model *model, const double *coefs, void *stream, fsp_mat *out);

@@ -196,7 +196,8 @@ __global__ void k_fill_hyb(const DevModel *__restrict__ dm, const double *__rest
                            int use_lds, const double *__restrict__ table,
                            const double *__restrict__ coefs, uint32_t n, int mixed,
                            double *__restrict__ diag, V *__restrict__ slot_vals,
-                           int *__restrict__ slot_cols, double *__restrict__ sink_out)
+                           int *__restrict__ slot_cols, double *__restrict__ sink_out,
+                           int transposed)
 {
     extern __shared__ double smem[];
     const double *tab = table_ptr(dm, tab_g, smem, use_lds);
@@ -211,7 +212,15 @@ __global__ void k_fill_hyb(const DevModel *__restrict__ dm, const double *__rest
     if (sink_out) sink_out[i] = out;
     for (int u = 0; u < dm->U; ++u) {
         const size_t o = (size_t)u * n + i;
-        if (source_in_box<D>(dm->rx[dm->slot_rxn[dm->slot_ptr[u]]], xs, dm)) {
+        const RxnDev &rx = dm->rx[dm->slot_rxn[dm->slot_ptr[u]]];
+        if (transposed) {
+            // row i of A^T: the outflow rate of state i towards its target
+            const bool in = !target_leaves<D>(rx, xs, dm);
+            slot_vals[o] = (V)(in ? slot_value<D>(dm, tab, table, coefs, u, i, xs, 0) : 0.0);
+            slot_cols[o] = in ? (int)((int64_t)i + dm->slot_delta[u]) : (int)i;
+            continue;
+        }
+        if (source_in_box<D>(rx, xs, dm)) {
             slot_vals[o] = (V)slot_value<D>(dm, tab, table, coefs, u, i, xs, 1);
             slot_cols[o] = (int)((int64_t)i - dm->slot_delta[u]);
         } else {
@@ -317,12 +326,12 @@ static fsp_status fill_values(MatImpl &M, hipStream_t s)
             FSP_SWITCH_D(dm.d, (k_fill_hyb<D_, float><<<blocks, kBlock, lds, s>>>(
                                    M.d_model, tab, use_lds, table, M.d_coefs, n, 1,
                                    (double *)M.diag, (float *)M.slot_vals, M.slot_cols,
-                                   (double *)M.sink_out)));
+                                   (double *)M.sink_out, 0)));
         } else {
             FSP_SWITCH_D(dm.d, (k_fill_hyb<D_, double><<<blocks, kBlock, lds, s>>>(
                                    M.d_model, tab, use_lds, table, M.d_coefs, n, 0,
                                    (double *)M.diag, (double *)M.slot_vals, M.slot_cols,
-                                   (double *)M.sink_out)));
+                                   (double *)M.sink_out, M.transposed)));
         }
     }
     FSP_HIP(hipGetLastError());
@@ -674,6 +683,7 @@ fsp_status mat_apply_transpose(MatImpl &M, const double *d_x, double *d_y, int a
                                hipStream_t s)
 {
     if (M.with_sink) return fail(FSP_ERR_UNSUPPORTED, "transpose apply with a sink state");
+    if (M.transposed) return fail(FSP_ERR_UNSUPPORTED, "transpose apply of a transposed generator");
     // The transpose is matrix-free from the model for every format: the
     // generator entries are the model's propensities, so no transposed copy
     // of the CSR / HYB arrays is kept.
@@ -685,6 +695,7 @@ fsp_status mat_apply_term(MatImpl &M, int r, double c, const double *d_x, double
 {
     if (r < 0 || r >= M.hm.dm.R) return fail(FSP_ERR_INVALID, "reaction index out of range");
     if (M.with_sink) return fail(FSP_ERR_UNSUPPORTED, "term apply with a sink state");
+    if (M.transposed) return fail(FSP_ERR_UNSUPPORTED, "term apply of a transposed generator");
     return launch_kron<double>(M, d_x, d_y, accumulate, s, 0, r, c);
 }
 
