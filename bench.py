@@ -18,7 +18,9 @@ Also reported on the same line:
                 solves/s and worst L1 error vs FP64 over the output times
   cpu_baseline  the CPU oracle (restatement of the reference algorithm) on a
                 bounded sample of the same solve, extrapolated to solves/s
-  bdf, adjoint  the time-varying BDF solve and one log-likelihood gradient
+  bdf, adjoint  the time-varying BDF solve; one log-likelihood gradient, and
+                gradients at several parameter points one at a time and
+                batched on 4 streams
   batch         an uneven batch: sequential plain FP64 solves against the
                 lockstep batch path in FP64 and mixed precision
 """
@@ -194,25 +196,53 @@ def bench_bdf(net):
             "mass_in_box_t60": mass}
 
 
-def bench_adjoint(net):
+def bench_adjoint(net, points):
     """One log-likelihood gradient by the adjoint on the headline network:
     joint (n, m) histograms of 2,000 cells at t = 5, 10, 20, 30, 45, 60
-    (multinomial from the FSP marginal, seed 2), 9 log rates."""
+    (multinomial from the FSP marginal, seed 2), 9 log rates, HYB generator.
+    Then `points` parameter points (log rates within +-0.5 of nominal,
+    seed 3) one at a time and batched on 4 streams."""
     from cuda_fsp_amd import likelihood as _L
     times = [5.0, 10.0, 20.0, 30.0, 45.0, 60.0]
     keep = [0, 1, 1]
     counts = _L.sample_histograms(net, times, keep, 2000, seed=2, tol=1e-10)
-    prob = _L.LikelihoodProblem(net, times, keep, counts, tol=1e-10, grading_levels=6)
+    prob = _L.LikelihoodProblem(net, times, keep, counts, tol=1e-10, grading_levels=6, fmt="hyb")
     th = np.log(net.coefs)
     t0 = time.perf_counter()
     ll = prob.value(th)
     t1 = time.perf_counter()
+    print(f"[bench] log-likelihood {t1 - t0:.2f} s", flush=True)
     ll2, g = prob.value_and_grad(th)
     t2 = time.perf_counter()
-    return {"workload": "gene_expression 3x512x512, 6 times x 2,000 cells, 9 log rates, "
-                        "Krylov tol 1e-10, 2x6 graded Gauss-Legendre panels of 8 nodes per interval",
-            "loglik": ll, "loglik_s": round(t1 - t0, 3), "gradient_s": round(t2 - t1, 3),
-            "gradients_per_s": round(1.0 / (t2 - t1), 4), "gradient": [round(x, 6) for x in g]}
+    print(f"[bench] gradient {t2 - t1:.2f} s", flush=True)
+    out = {"workload": "gene_expression 3x512x512, 6 times x 2,000 cells, 9 log rates, "
+                       "Krylov tol 1e-10, 2x6 graded Gauss-Legendre panels of 8 nodes per "
+                       "interval, HYB generator",
+           "loglik": ll, "loglik_s": round(t1 - t0, 3), "gradient_s": round(t2 - t1, 3),
+           "gradients_per_s": round(1.0 / (t2 - t1), 4), "gradient": [round(x, 6) for x in g]}
+    if points > 0:
+        rng = np.random.default_rng(3)
+        thetas = th[None, :] + rng.uniform(-0.5, 0.5, (points, len(th)))
+        t3 = time.perf_counter()
+        ll_s, g_s = prob.value_and_grad_batch(thetas, workers=1)
+        t4 = time.perf_counter()
+        print(f"[bench] {points} gradients one at a time {t4 - t3:.2f} s", flush=True)
+        ll_b, g_b = prob.value_and_grad_batch(thetas, workers=4)
+        t5 = time.perf_counter()
+        print(f"[bench] {points} gradients on 4 streams {t5 - t4:.2f} s", flush=True)
+        out["batch"] = {"points": points, "sequential_s": round(t4 - t3, 3),
+                        "batched_s": round(t5 - t4, 3), "workers": 4,
+                        "speedup": round((t4 - t3) / (t5 - t4), 3),
+                        "gradients_per_s": round(points / (t5 - t4), 4),
+                        "identical_to_sequential": bool(np.array_equal(g_s, g_b) and
+                                                        np.array_equal(ll_s, ll_b))}
+    return out
+
+
+def grad_only(points):
+    """Only the adjoint section, with `points` batched parameter points."""
+    net = M.gene_expression()
+    print(json.dumps(bench_adjoint(net, points)), flush=True)
 
 
 def bench_batch(count):
@@ -334,12 +364,19 @@ def main():
     ap.add_argument("--batch-only", type=int, default=0,
                     help="only one lockstep solve of an uneven batch of this many instances")
     ap.add_argument("--batch-precision", default="f64", choices=["f64", "mixed"])
+    ap.add_argument("--grad-points", type=int, default=4,
+                    help="parameter points in the batched-gradient section")
+    ap.add_argument("--grad-only", action="store_true",
+                    help="only the adjoint-gradient section")
     args = ap.parse_args()
     if args.kernels_only:
         kernels_only(5)
         return
     if args.batch_only:
         batch_only(args.batch_only, args.batch_precision)
+        return
+    if args.grad_only:
+        grad_only(args.grad_points)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path "
@@ -399,7 +436,7 @@ def main():
         print("[bench] BDF time-varying solve", flush=True)
         extras["bdf"] = bench_bdf(net)
         print("[bench] adjoint gradient", flush=True)
-        extras["adjoint"] = bench_adjoint(net)
+        extras["adjoint"] = bench_adjoint(net, args.grad_points)
         print(f"[bench] uneven batch of {args.batch_count}", flush=True)
         extras["batch"] = bench_batch(args.batch_count)
 

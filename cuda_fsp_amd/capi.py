@@ -143,6 +143,7 @@ _SIGS = {
     "fsp_rhs_apply": (C.c_int, [_vp, _d, _vp, _vp, _vp, _vp, _vp]),
     "fsp_loglik": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fsp_loglik_grad": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fsp_loglik_grad_batch": (C.c_int, [_vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
     "fsp_bdf_create": (C.c_int, [_vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "fsp_bdf_destroy": (C.c_int, [_vp]),
     "fsp_bdf_advance": (C.c_int, [_vp, _vp, _d]),
@@ -421,6 +422,33 @@ def loglik(net, log_coefs, times, keep_mask, counts, floor_val=1e-12, p0=None, f
         check(lib().fsp_loglik(model.ref, th.ctypes.data, ptr(pbuf), C.byref(data), C.byref(opt),
                                C.byref(ll), stream))
         return ll.value
+    finally:
+        if own:
+            pbuf.free()
+
+
+def loglik_grad_batch(net, log_coefs, times, keep_mask, counts, floor_val=1e-12, p0=None,
+                      fmt="csr", m=30, tol=1e-10, quad_nodes=8, grading_levels=10, workers=0):
+    """LL and dLL/dlog_coefs at every row of log_coefs [points, R]
+    (fsp_loglik_grad_batch): returns (ll [points], grad [points, R])."""
+    model = Model(net)
+    th = np.ascontiguousarray(log_coefs, np.float64)
+    if th.ndim != 2 or th.shape[1] != net.n_reactions:
+        raise ValueError("log_coefs must be [points, n_reactions]")
+    tt = np.ascontiguousarray(times, np.float64)
+    km = np.ascontiguousarray([1 if k else 0 for k in keep_mask], np.int32)
+    cn = np.ascontiguousarray(counts, np.float64)
+    data = FspLikData(len(tt), tt.ctypes.data, km.ctypes.data, cn.ctypes.data, float(floor_val))
+    opt = FspGradOptions(FORMATS[fmt], int(m), float(tol), int(quad_nodes), int(grading_levels))
+    own = p0 is None
+    pbuf = DeviceBuffer.from_host(_delta0(net.n_states)) if own else p0
+    ll = np.zeros(th.shape[0], np.float64)
+    g = np.zeros(th.shape, np.float64)
+    try:
+        check(lib().fsp_loglik_grad_batch(model.ref, th.shape[0], th.ctypes.data, ptr(pbuf),
+                                          C.byref(data), C.byref(opt), int(workers),
+                                          ll.ctypes.data, g.ctypes.data))
+        return ll, g
     finally:
         if own:
             pbuf.free()

@@ -19,8 +19,11 @@
 // forward pass keeps p only at the observation times; the backward pass
 // recomputes p at one interval's nodes from its checkpoint.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "fsp_assemble.h"
@@ -307,8 +310,8 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
     int *d_keep = nullptr;
     FSP_HIP(hipMalloc(&d_keep, sizeof(int) * (size_t)d));
     W.bufs.push_back((double *)d_keep);
-    FSP_HIP(hipMemcpy(d_keep, keep.data(), sizeof(int) * (size_t)d, hipMemcpyHostToDevice));
-    FSP_HIP(hipMemcpy(cnt, data->counts, sizeof(double) * (size_t)(nb * K), hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(d_keep, keep.data(), sizeof(int) * (size_t)d, hipMemcpyHostToDevice, s));
+    FSP_HIP(copy_sync(cnt, data->counts, sizeof(double) * (size_t)(nb * K), hipMemcpyHostToDevice, s));
 
     // forward pass: p at the observation times (checkpoints), LL
     FSP_HIP(hipMemcpyAsync(p, d_p0, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
@@ -408,6 +411,80 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
     return FSP_OK;
 }
 
+// Batched gradients: points claimed in order from a shared counter by
+// `workers` host threads, each with its own stream (a process has 4 hardware
+// queues), so one point's host synchronisations and small kernels overlap
+// another's row kernels.  Each point runs exactly the single-point code.
+struct GradBatch {
+    const fsp_model *model;
+    int n_points, R;
+    const double *log_coefs, *d_p0;
+    const fsp_lik_data *data;
+    const fsp_grad_options *opt;
+    double *h_ll, *h_grad;
+    std::atomic<int> next{0};
+    std::atomic<int> failed{0};
+};
+
+void grad_worker(GradBatch *B, fsp_status *st, std::string *err)
+{
+    *st = FSP_OK;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        *st = fail(FSP_ERR_HIP, "batched gradient: stream creation");
+        *err = fsp_last_error();
+        B->failed = 1;
+        return;
+    }
+    for (;;) {
+        if (B->failed.load()) break;
+        const int i = B->next.fetch_add(1);
+        if (i >= B->n_points) break;
+        const fsp_status r = loglik_impl(B->model, B->log_coefs + (size_t)i * B->R, B->d_p0, B->data,
+                                         B->opt, B->h_ll + i, B->h_grad + (size_t)i * B->R, s);
+        if (r != FSP_OK) {
+            *st = r;
+            *err = "point " + std::to_string(i) + ": " + fsp_last_error();
+            B->failed = 1;
+            break;
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+}
+
+fsp_status loglik_grad_batch(const fsp_model *model, int n_points, const double *log_coefs,
+                             const double *d_p0, const fsp_lik_data *data,
+                             const fsp_grad_options *opt, int workers, double *h_ll,
+                             double *h_grad)
+{
+    if (!model || !log_coefs || !d_p0 || !data || !h_ll || !h_grad)
+        return fail(FSP_ERR_INVALID, "NULL argument");
+    if (n_points < 0) return fail(FSP_ERR_INVALID, "n_points < 0");
+    if (n_points == 0) return FSP_OK;
+    if (workers <= 0) workers = 4;
+    workers = std::min(workers, n_points);
+    GradBatch B;
+    B.model = model;
+    B.n_points = n_points;
+    B.R = model->n_reactions;
+    B.log_coefs = log_coefs;
+    B.d_p0 = d_p0;
+    B.data = data;
+    B.opt = opt;
+    B.h_ll = h_ll;
+    B.h_grad = h_grad;
+    std::vector<fsp_status> st((size_t)workers, FSP_OK);
+    std::vector<std::string> err((size_t)workers);
+    std::vector<std::thread> th;
+    for (int w = 1; w < workers; ++w) th.emplace_back(grad_worker, &B, &st[w], &err[w]);
+    grad_worker(&B, &st[0], &err[0]);
+    for (std::thread &t : th) t.join();
+    for (int w = 0; w < workers; ++w)
+        if (st[w] != FSP_OK) return fail(st[w], "batched gradient, " + err[w]);
+    return FSP_OK;
+}
+
 }  // namespace
 }  // namespace fsp
 
@@ -425,4 +502,14 @@ extern "C" fsp_status fsp_loglik_grad(const fsp_model *model, const double *log_
 {
     if (!h_grad) return fsp::fail(FSP_ERR_INVALID, "h_grad is NULL");
     return fsp::loglik_impl(model, log_coefs, d_p0, data, opt, h_ll, h_grad, (hipStream_t)stream);
+}
+
+extern "C" fsp_status fsp_loglik_grad_batch(const fsp_model *model, int n_points,
+                                            const double *log_coefs, const double *d_p0,
+                                            const fsp_lik_data *data,
+                                            const fsp_grad_options *opt, int workers,
+                                            double *h_ll, double *h_grad)
+{
+    return fsp::loglik_grad_batch(model, n_points, log_coefs, d_p0, data, opt, workers, h_ll,
+                                  h_grad);
 }

@@ -298,19 +298,19 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
     B.tab_lds = (base + sizeof(double) * max_tab <= 150 * 1024) ? max_tab : 0;
     FSP_HIP(hipMalloc(&B.d_models, sizeof(DevModel) * (size_t)count));
     for (int i = 0; i < count; ++i)
-        FSP_HIP(hipMemcpy(B.d_models + i, &B.hms[i].dm, sizeof(DevModel), hipMemcpyHostToDevice));
+        FSP_HIP(copy_sync(B.d_models + i, &B.hms[i].dm, sizeof(DevModel), hipMemcpyHostToDevice, nullptr));
     FSP_HIP(hipMalloc(&B.d_tabs, sizeof(double) * std::max<size_t>(tabs.size(), 1)));
     if (!tabs.empty())
-        FSP_HIP(hipMemcpy(B.d_tabs, tabs.data(), sizeof(double) * tabs.size(), hipMemcpyHostToDevice));
+        FSP_HIP(copy_sync(B.d_tabs, tabs.data(), sizeof(double) * tabs.size(), hipMemcpyHostToDevice, nullptr));
     FSP_HIP(hipMalloc(&B.d_coefs, sizeof(double) * cvec.size()));
-    FSP_HIP(hipMemcpy(B.d_coefs, cvec.data(), sizeof(double) * cvec.size(), hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(B.d_coefs, cvec.data(), sizeof(double) * cvec.size(), hipMemcpyHostToDevice, nullptr));
     FSP_HIP(hipMalloc(&B.d_vecs, sizeof(double) * (size_t)vec_total));
-    FSP_HIP(hipMemset(B.d_vecs, 0, sizeof(double) * (size_t)vec_total));
+    FSP_HIP(fill_sync(B.d_vecs, 0, sizeof(double) * (size_t)vec_total, nullptr));
     // default initial vectors: delta at state 0
     for (int i = 0; i < count; ++i) {
         const double one = 1.0;
-        FSP_HIP(hipMemcpy(B.d_vecs + B.vec_off[i] + B.h_inst[i].nv, &one, sizeof(double),
-                          hipMemcpyHostToDevice));
+        FSP_HIP(copy_sync(B.d_vecs + B.vec_off[i] + B.h_inst[i].nv, &one, sizeof(double),
+                          hipMemcpyHostToDevice, nullptr));
     }
     for (int i = 0; i < count; ++i) {
         BatchInst &I = B.h_inst[i];
@@ -321,15 +321,15 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
         I.p0 = I.p + I.nv;
     }
     FSP_HIP(hipMalloc(&B.d_inst, sizeof(BatchInst) * (size_t)count));
-    FSP_HIP(hipMemcpy(B.d_inst, B.h_inst.data(), sizeof(BatchInst) * (size_t)count,
-                      hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(B.d_inst, B.h_inst.data(), sizeof(BatchInst) * (size_t)count,
+                      hipMemcpyHostToDevice, nullptr));
     std::vector<int> order((size_t)count);
     for (int i = 0; i < count; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
     B.order = order;
     B.cost = cost;
     FSP_HIP(hipMalloc(&B.d_order, sizeof(int) * (size_t)count));
-    FSP_HIP(hipMemcpy(B.d_order, order.data(), sizeof(int) * (size_t)count, hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(B.d_order, order.data(), sizeof(int) * (size_t)count, hipMemcpyHostToDevice, nullptr));
     FSP_HIP(hipMalloc(&B.d_queue, sizeof(unsigned int)));
     std::vector<double> p10(kP10Hi - kP10Lo + 1);
     for (int k = kP10Lo; k <= kP10Hi; ++k) {
@@ -338,7 +338,7 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
         p10[k - kP10Lo] = std::strtod(buf, nullptr);
     }
     FSP_HIP(hipMalloc(&B.d_p10, sizeof(double) * p10.size()));
-    FSP_HIP(hipMemcpy(B.d_p10, p10.data(), sizeof(double) * p10.size(), hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(B.d_p10, p10.data(), sizeof(double) * p10.size(), hipMemcpyHostToDevice, nullptr));
     // workspace for the largest grid any configuration may use (2 per CU);
     // none where the persistent kernel is not launched
     bool persistent = false;
@@ -376,8 +376,8 @@ extern "C" fsp_status fsp_batch_set_initial(fsp_batch b, int i, const double *h_
     using namespace fsp;
     if (!b || !h_p0 || i < 0 || i >= b->impl.count) return fail(FSP_ERR_INVALID, "bad argument");
     BatchImpl &B = b->impl;
-    FSP_HIP(hipMemcpy(B.d_vecs + B.vec_off[i] + B.h_inst[i].nv, h_p0,
-                      sizeof(double) * (size_t)B.h_inst[i].nv, hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(B.d_vecs + B.vec_off[i] + B.h_inst[i].nv, h_p0,
+                      sizeof(double) * (size_t)B.h_inst[i].nv, hipMemcpyHostToDevice, nullptr));
     return FSP_OK;
 }
 
@@ -387,6 +387,7 @@ extern "C" fsp_status fsp_batch_solve(fsp_batch b, void *stream)
     if (!b) return fail(FSP_ERR_INVALID, "NULL batch");
     int path = 0;
     FSP_TRY(batch_path(b->impl, &path));
+    b->impl.solve_stream = (hipStream_t)stream;
     if (path == FSP_BATCH_PERSISTENT) return batch_graph(b->impl, (hipStream_t)stream);
     // lockstep and stream scheduler: synchronous, on their own streams
     FSP_HIP(hipStreamSynchronize((hipStream_t)stream));
@@ -401,9 +402,9 @@ extern "C" fsp_status fsp_batch_get_solution(fsp_batch b, int i, double *h_p, in
     BatchImpl &B = b->impl;
     const int64_t nv = B.h_inst[i].nv;
     if (cap < nv) return fail(FSP_ERR_INVALID, "buffer too small");
-    FSP_HIP(hipDeviceSynchronize());
-    FSP_HIP(hipMemcpy(h_p, B.d_vecs + B.vec_off[i], sizeof(double) * (size_t)nv,
-                      hipMemcpyDeviceToHost));
+    FSP_HIP(hipStreamSynchronize(B.solve_stream));
+    FSP_HIP(copy_sync(h_p, B.d_vecs + B.vec_off[i], sizeof(double) * (size_t)nv,
+                      hipMemcpyDeviceToHost, nullptr));
     return FSP_OK;
 }
 
@@ -414,8 +415,8 @@ extern "C" fsp_status fsp_batch_get_stats(fsp_batch b, int i, int64_t *steps, in
     if (!b || i < 0 || i >= b->impl.count) return fail(FSP_ERR_INVALID, "bad argument");
     BatchImpl &B = b->impl;
     fsp::BatchInst I;
-    FSP_HIP(hipDeviceSynchronize());
-    FSP_HIP(hipMemcpy(&I, B.d_inst + i, sizeof(I), hipMemcpyDeviceToHost));
+    FSP_HIP(hipStreamSynchronize(B.solve_stream));
+    FSP_HIP(copy_sync(&I, B.d_inst + i, sizeof(I), hipMemcpyDeviceToHost, nullptr));
     if (steps) *steps = I.steps;
     if (rejects) *rejects = I.rejects;
     if (status) *status = I.status;

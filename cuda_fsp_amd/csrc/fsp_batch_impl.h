@@ -38,6 +38,7 @@ struct BatchImpl {
     int path_tuned = FSP_BATCH_AUTO;   // autotuner's choice for AUTO
     int ls_bpc = 0;                    // lockstep row-kernel workgroups per CU (0: one per tile)
     LsHost *ls = nullptr;
+    hipStream_t solve_stream = nullptr;  // stream of the last solve (persistent path)
     int m = 30, iop = 0, q_iop = 2, max_reject = 1000;
     long long max_nstep = 100000;
     double tol = 1e-8, anorm = 1.0;

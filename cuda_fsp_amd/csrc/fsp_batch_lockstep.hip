@@ -682,14 +682,14 @@ static fsp_status ls_build(BatchImpl &B)
         os += (size_t)I.U * (size_t)bi.n;
     }
     FSP_HIP(hipMalloc(&L.d_inst, sizeof(LsInst) * (size_t)count));
-    FSP_HIP(hipMemcpy(L.d_inst, hi.data(), sizeof(LsInst) * (size_t)count, hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(L.d_inst, hi.data(), sizeof(LsInst) * (size_t)count, hipMemcpyHostToDevice, nullptr));
     FSP_HIP(hipMalloc(&L.d_tile_inst, sizeof(int) * (size_t)L.T));
     FSP_HIP(hipMalloc(&L.d_tile_r0, sizeof(int) * (size_t)L.T));
     FSP_HIP(hipMalloc(&L.d_atiles, sizeof(int) * (size_t)L.T));
-    FSP_HIP(hipMemcpy(L.d_tile_inst, tinst.data(), sizeof(int) * (size_t)L.T, hipMemcpyHostToDevice));
-    FSP_HIP(hipMemcpy(L.d_tile_r0, tr0.data(), sizeof(int) * (size_t)L.T, hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(L.d_tile_inst, tinst.data(), sizeof(int) * (size_t)L.T, hipMemcpyHostToDevice, nullptr));
+    FSP_HIP(copy_sync(L.d_tile_r0, tr0.data(), sizeof(int) * (size_t)L.T, hipMemcpyHostToDevice, nullptr));
     FSP_HIP(hipMalloc(&L.d_cnt, sizeof(int) * 4));
-    FSP_HIP(hipMemset(L.d_cnt, 0, sizeof(int) * 4));
+    FSP_HIP(fill_sync(L.d_cnt, 0, sizeof(int) * 4, nullptr));
     FSP_HIP(hipHostMalloc((void **)&L.h_cnt, sizeof(int) * 4, hipHostMallocDefault));
     // dot partials: 2m + 4 slots of T (dots, sink, Gram column); the
     // update's quarter-tile norms: kLsRpt T
@@ -697,7 +697,7 @@ static fsp_status ls_build(BatchImpl &B)
     FSP_HIP(hipMalloc(&L.d_partials, sizeof(double) * 2 * pslots));
     // step states: constants once (tolerance, decimal powers), the rest per solve
     std::vector<double> p10(kP10Hi - kP10Lo + 1);
-    FSP_HIP(hipMemcpy(p10.data(), B.d_p10, sizeof(double) * p10.size(), hipMemcpyDeviceToHost));
+    FSP_HIP(copy_sync(p10.data(), B.d_p10, sizeof(double) * p10.size(), hipMemcpyDeviceToHost, nullptr));
     std::vector<KState> hs((size_t)count);
     const double fact = std::pow((m + 1) / std::exp(1.0), m + 1) * std::sqrt(2 * (3.1416) * (m + 1));
     for (int i = 0; i < count; ++i) {
@@ -711,7 +711,7 @@ static fsp_status ls_build(BatchImpl &B)
         std::memcpy(k.p10, p10.data(), sizeof(double) * p10.size());
     }
     FSP_HIP(hipMalloc(&L.d_st, sizeof(KState) * (size_t)count));
-    FSP_HIP(hipMemcpy(L.d_st, hs.data(), sizeof(KState) * (size_t)count, hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(L.d_st, hs.data(), sizeof(KState) * (size_t)count, hipMemcpyHostToDevice, nullptr));
     FSP_HIP(hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking));
     L.grid = 0;
     // HYB generators
@@ -849,7 +849,7 @@ fsp_status batch_solve_lockstep(BatchImpl &B, int blocks_per_cu)
     FSP_HIP(hipStreamSynchronize(L.s));
     // report the first failed instance
     std::vector<BatchInst> h((size_t)B.count);
-    FSP_HIP(hipMemcpy(h.data(), B.d_inst, sizeof(BatchInst) * (size_t)B.count, hipMemcpyDeviceToHost));
+    FSP_HIP(copy_sync(h.data(), B.d_inst, sizeof(BatchInst) * (size_t)B.count, hipMemcpyDeviceToHost, nullptr));
     for (int i = 0; i < B.count; ++i) {
         if (h[i].status != 0) {
             char buf[160];

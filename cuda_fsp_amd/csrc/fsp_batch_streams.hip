@@ -103,8 +103,8 @@ fsp_status batch_solve_streams(BatchImpl &B)
     for (std::thread &t : th) t.join();
     for (hipStream_t s : B.wstreams) FSP_HIP(hipStreamSynchronize(s));
     // per-instance statistics into the device records read by fsp_batch_get_stats
-    FSP_HIP(hipMemcpy(B.d_inst, B.h_inst.data(), sizeof(BatchInst) * (size_t)B.count,
-                      hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(B.d_inst, B.h_inst.data(), sizeof(BatchInst) * (size_t)B.count,
+                      hipMemcpyHostToDevice, nullptr));
     for (int w = 0; w < workers; ++w)
         if (st[w] != FSP_OK) return fail(st[w], "batch " + err[w]);
     return FSP_OK;

@@ -233,13 +233,13 @@ fsp_status fsp_free(void *d_ptr)
 
 fsp_status fsp_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes)
 {
-    FSP_HIP(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(d_dst, h_src, bytes, hipMemcpyHostToDevice, nullptr));
     return FSP_OK;
 }
 
 fsp_status fsp_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes)
 {
-    FSP_HIP(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+    FSP_HIP(copy_sync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, nullptr));
     return FSP_OK;
 }
 
@@ -310,7 +310,7 @@ fsp_status fsp_states_enumerate(int n_species, const int *bounds, int *d_states,
     if (!d_states) return fail(FSP_ERR_INVALID, "d_states is NULL");
     DevModel *dm = nullptr;
     FSP_HIP(hipMalloc(&dm, sizeof(DevModel)));
-    FSP_HIP(hipMemcpy(dm, &hm.dm, sizeof(DevModel), hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(dm, &hm.dm, sizeof(DevModel), hipMemcpyHostToDevice, (hipStream_t)stream));
     const uint32_t n = (uint32_t)hm.dm.n;
     const unsigned blocks = (n + kBlock - 1) / kBlock;
     hipStream_t s = (hipStream_t)stream;
@@ -331,7 +331,7 @@ fsp_status fsp_states_to_index(int n_species, const int *bounds, const int *d_st
     if (count == 0) return FSP_OK;
     DevModel *dm = nullptr;
     FSP_HIP(hipMalloc(&dm, sizeof(DevModel)));
-    FSP_HIP(hipMemcpy(dm, &hm.dm, sizeof(DevModel), hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(dm, &hm.dm, sizeof(DevModel), hipMemcpyHostToDevice, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     k_to_index<<<(unsigned)((count + kBlock - 1) / kBlock), kBlock, 0, s>>>(dm, d_states, count,
                                                                            d_index);
@@ -466,7 +466,7 @@ fsp_status fsp_mat_export_csr(fsp_mat A, int *d_row_ptrs, int *d_cols, double *d
         if (M.with_sink) {
             const int end = (int)(M.nnz_rows + M.nnz_sink);
             FSP_HIP(hipStreamSynchronize(s));
-            FSP_HIP(hipMemcpy(d_row_ptrs + M.n + 1, &end, sizeof(int), hipMemcpyHostToDevice));
+            FSP_HIP(copy_sync(d_row_ptrs + M.n + 1, &end, sizeof(int), hipMemcpyHostToDevice, s));
         }
     }
     if (d_cols) {

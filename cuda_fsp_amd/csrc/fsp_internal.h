@@ -102,6 +102,26 @@ struct HostModel {
 
 fsp_status build_host_model(const fsp_model *m, HostModel &hm);
 
+// Blocking copy and fill ordered on stream s, used in place of hipMemcpy /
+// hipMemset: those run on the legacy default stream, and HIP refuses a
+// legacy-stream operation while another thread of the process is capturing
+// a graph.  s = NULL: the calling thread's own default stream, which still
+// orders after earlier work on the legacy stream.
+inline hipError_t copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind,
+                            hipStream_t s)
+{
+    if (!s) s = hipStreamPerThread;
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
+inline hipError_t fill_sync(void *dst, int value, size_t bytes, hipStream_t s)
+{
+    if (!s) s = hipStreamPerThread;
+    const hipError_t e = hipMemsetAsync(dst, value, bytes, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
 // ------------------------------------------------------------ matrices
 struct MatImpl {
     HostModel hm;

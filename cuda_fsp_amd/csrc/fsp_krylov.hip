@@ -546,7 +546,7 @@ static fsp_status expv_alloc(ExpvImpl &E)
     FSP_HIP(hipMalloc(&E.tau_log, sizeof(double) * (size_t)kTauLogCap));
     h.tau_log = E.tau_log;
     h.tau_cap = kTauLogCap;
-    FSP_HIP(hipMemcpy(E.d_st, &h, sizeof(KState), hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(E.d_st, &h, sizeof(KState), hipMemcpyHostToDevice, E.s));
     return FSP_OK;
 }
 
@@ -878,8 +878,8 @@ extern "C" fsp_status fsp_expv_get_tau_history(fsp_expv e, double *h_taus, int64
     const int64_t n = std::min<int64_t>(e->impl.h_st->i_step, kTauLogCap);
     *count = n;
     if (h_taus && cap > 0 && n > 0)
-        FSP_HIP(hipMemcpy(h_taus, e->impl.tau_log, sizeof(double) * (size_t)std::min(n, cap),
-                          hipMemcpyDeviceToHost));
+        FSP_HIP(copy_sync(h_taus, e->impl.tau_log, sizeof(double) * (size_t)std::min(n, cap),
+                          hipMemcpyDeviceToHost, e->impl.s));
     return FSP_OK;
 }
 

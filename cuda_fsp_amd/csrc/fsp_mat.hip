@@ -273,18 +273,18 @@ __global__ void k_term_fill(const DevModel *__restrict__ dm, const double *__res
 
 }  // namespace
 
-static fsp_status upload_model(MatImpl &M)
+static fsp_status upload_model(MatImpl &M, hipStream_t s)
 {
     FSP_HIP(hipMalloc(&M.d_model, sizeof(DevModel)));
-    FSP_HIP(hipMemcpy(M.d_model, &M.hm.dm, sizeof(DevModel), hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(M.d_model, &M.hm.dm, sizeof(DevModel), hipMemcpyHostToDevice, s));
     if (!M.hm.compact.empty()) {
         FSP_HIP(hipMalloc(&M.d_compact, sizeof(double) * M.hm.compact.size()));
-        FSP_HIP(hipMemcpy(M.d_compact, M.hm.compact.data(), sizeof(double) * M.hm.compact.size(),
-                          hipMemcpyHostToDevice));
+        FSP_HIP(copy_sync(M.d_compact, M.hm.compact.data(), sizeof(double) * M.hm.compact.size(),
+                          hipMemcpyHostToDevice, s));
     }
     FSP_HIP(hipMalloc(&M.d_coefs, sizeof(double) * (size_t)M.hm.dm.R));
-    FSP_HIP(hipMemcpy(M.d_coefs, M.coefs.data(), sizeof(double) * (size_t)M.hm.dm.R,
-                      hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(M.d_coefs, M.coefs.data(), sizeof(double) * (size_t)M.hm.dm.R,
+                      hipMemcpyHostToDevice, s));
     return FSP_OK;
 }
 
@@ -346,12 +346,12 @@ fsp_status mat_assemble(MatImpl &M, hipStream_t s)
     const uint32_t n = (uint32_t)M.n;
     const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
     const int mixed = M.prec == FSP_MIXED;
-    FSP_TRY(upload_model(M));
+    FSP_TRY(upload_model(M, s));
     const int64_t tiles = (M.n + kBlock - 1) / kBlock;
     M.n_partials = std::max<int64_t>(tiles, 1) * 64;
     FSP_HIP(hipMalloc(&M.partials, sizeof(double) * (size_t)M.n_partials));
     FSP_HIP(hipMalloc(&M.ticket, sizeof(unsigned int) * 4));
-    FSP_HIP(hipMemset(M.ticket, 0, sizeof(unsigned int) * 4));
+    FSP_HIP(fill_sync(M.ticket, 0, sizeof(unsigned int) * 4, s));
     if (M.format == FSP_CSR) {
         int *cnt = nullptr, *scnt = nullptr;
         FSP_HIP(hipMalloc(&cnt, sizeof(int) * (size_t)(M.n + 1)));
@@ -376,7 +376,7 @@ fsp_status mat_assemble(MatImpl &M, hipStream_t s)
         // longest row (for the LDS product buffer); counts are still in cnt
         {
             std::vector<int> h((size_t)M.n);
-            FSP_HIP(hipMemcpy(h.data(), cnt, sizeof(int) * (size_t)M.n, hipMemcpyDeviceToHost));
+            FSP_HIP(copy_sync(h.data(), cnt, sizeof(int) * (size_t)M.n, hipMemcpyDeviceToHost, s));
             M.max_row = M.n ? *std::max_element(h.begin(), h.end()) : 0;
         }
         hipFree(cnt);
@@ -399,8 +399,8 @@ fsp_status mat_assemble(MatImpl &M, hipStream_t s)
         if (mixed && !M.hm.compact.empty()) {
             std::vector<float> f(M.hm.compact.begin(), M.hm.compact.end());
             FSP_HIP(hipMalloc(&M.d_compact_f, sizeof(float) * f.size()));
-            FSP_HIP(hipMemcpy(M.d_compact_f, f.data(), sizeof(float) * f.size(),
-                              hipMemcpyHostToDevice));
+            FSP_HIP(copy_sync(M.d_compact_f, f.data(), sizeof(float) * f.size(),
+                              hipMemcpyHostToDevice, s));
         }
         M.nnz_rows = 0;
     }
@@ -739,11 +739,11 @@ fsp_status term_csr(const HostModel &hm, int r, int *d_rp, int *d_cols, double *
     double *d_tab = nullptr;
     int *cnt = nullptr, *rp = d_rp;
     FSP_HIP(hipMalloc(&d_model, sizeof(DevModel)));
-    FSP_HIP(hipMemcpy(d_model, &dm, sizeof(DevModel), hipMemcpyHostToDevice));
+    FSP_HIP(copy_sync(d_model, &dm, sizeof(DevModel), hipMemcpyHostToDevice, s));
     if (!hm.compact.empty()) {
         FSP_HIP(hipMalloc(&d_tab, sizeof(double) * hm.compact.size()));
-        FSP_HIP(hipMemcpy(d_tab, hm.compact.data(), sizeof(double) * hm.compact.size(),
-                          hipMemcpyHostToDevice));
+        FSP_HIP(copy_sync(d_tab, hm.compact.data(), sizeof(double) * hm.compact.size(),
+                          hipMemcpyHostToDevice, s));
     }
     FSP_HIP(hipMalloc(&cnt, sizeof(int) * (size_t)(n + 1)));
     if (!rp) FSP_HIP(hipMalloc(&rp, sizeof(int) * (size_t)(n + 1)));

@@ -71,6 +71,12 @@ class LikelihoodProblem:
         return capi.loglik(self.net, log_coefs, self.times, self.keep_mask, self.counts,
                            grad=True, **self.kw)
 
+    def value_and_grad_batch(self, log_coefs, workers=0):
+        """LL and gradient at every row of log_coefs [points, R], several
+        points at a time on their own streams."""
+        return capi.loglik_grad_batch(self.net, log_coefs, self.times, self.keep_mask,
+                                      self.counts, workers=workers, **self.kw)
+
 
 def torch_loglik(problem: LikelihoodProblem, theta):
     """LL(theta) as a differentiable torch scalar (theta: CPU tensor of log

@@ -347,6 +347,17 @@ fsp_status fsp_loglik_grad(const fsp_model *model, const double *log_coefs,
                            const double *d_p0, const fsp_lik_data *data,
                            const fsp_grad_options *options, double *h_ll, double *h_grad,
                            void *stream);
+/* LL and gradient at n_points parameter points: log_coefs and h_grad are
+ * n_points x n_reactions (row-major), h_ll has n_points entries.  Points
+ * run on `workers` host threads with a stream each (<= 0: 4, the hardware
+ * queues of a process); every point gives exactly fsp_loglik_grad's result.
+ * The error names the first failed point.  The reference has no gradient;
+ * its batched counterpart would be a loop over cme::Likelihood::evaluate
+ * (src/Likelihood.cpp:8-33). */
+fsp_status fsp_loglik_grad_batch(const fsp_model *model, int n_points, const double *log_coefs,
+                                 const double *d_p0, const fsp_lik_data *data,
+                                 const fsp_grad_options *options, int workers, double *h_ll,
+                                 double *h_grad);
 /* Dense matrix exponential on the device (one workgroup, FP64, Pade q=6
  * with scaling and squaring; stands in for arma::expmat at
  * src/KryExpvFSP.cu:125).  Column-major N x N, N <= 52. */

@@ -395,6 +395,13 @@ inline hipError_t hipMemset(void *d, int v, size_t n)
     std::memset(d, v, n);
     return hipSuccess;
 }
+inline hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t st)
+{
+    simt::submit(st, [=]() { hipMemset(d, v, n); });
+    return hipSuccess;
+}
+// the calling thread's default stream: a handle that is never captured
+#define hipStreamPerThread ((hipStream_t)(uintptr_t)0x2)
 inline hipError_t hipDeviceSynchronize() { return simt_status(); }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return simt_status(); }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned)

@@ -86,6 +86,23 @@ def test_expv_argument_errors():
     assert L.fsp_expv_create(None, good, None, 0, 1.0, 30, 1e-8, 0, 2, 1.0, None, C.byref(h)) == 1
 
 
+def test_batched_gradient_argument_errors():
+    L = capi.lib()
+    net = M.toggle_switch((7, 7))
+    model = capi.Model(net)
+    th = np.zeros((2, net.n_reactions))
+    ll = np.zeros(2)
+    g = np.zeros((2, net.n_reactions))
+    # NULL data / negative point count are refused before any device work
+    assert L.fsp_loglik_grad_batch(model.ref, 2, th.ctypes.data, None, None, None, 0,
+                                   ll.ctypes.data, g.ctypes.data) == 1
+    fake = C.c_void_p(16)
+    assert L.fsp_loglik_grad_batch(model.ref, -1, th.ctypes.data, fake, fake, None, 0,
+                                   ll.ctypes.data, g.ctypes.data) == 1
+    assert L.fsp_loglik_grad_batch(model.ref, 0, th.ctypes.data, fake, fake, None, 0,
+                                   ll.ctypes.data, g.ctypes.data) == 0
+
+
 def test_fastdiv_exact():
     # q / d == (q * mul) >> shift for every q < 2^31 (Granlund-Montgomery);
     # checked for every radix the models use and edge quotients

@@ -100,6 +100,21 @@ def test_adjoint_gradient_matches_central_differences():
     assert err <= 1e-5
 
 
+def test_batched_gradients_equal_single_points():
+    net = M.gene_expression((2, 10, 9))
+    times = [1.0, 2.5]
+    counts = _data(net, times, 150, seed=6)
+    prob = L.LikelihoodProblem(net, times, KEEP, counts, tol=1e-12, grading_levels=4)
+    rng = np.random.default_rng(7)
+    thetas = np.log(net.coefs)[None, :] + rng.uniform(-0.5, 0.5, (5, net.n_reactions))
+    print("[case] batched gradients, 5 points on 3 workers, gene 3x11x10", flush=True)
+    ll, g = prob.value_and_grad_batch(thetas, workers=3)
+    for i in range(len(thetas)):
+        lli, gi = prob.value_and_grad(thetas[i])
+        assert ll[i] == lli
+        assert np.array_equal(g[i], gi)
+
+
 def test_torch_autograd_bridge():
     import torch
 
