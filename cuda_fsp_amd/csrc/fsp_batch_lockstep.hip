@@ -35,7 +35,7 @@ struct LsInst {
     const double *coefs;
     double *hdiag;          // HYB: diagonal (n)
     void *hvals;            // HYB: slot values [U][n], FP64 or FP32 (mixed)
-    int *hcols;             // HYB: slot columns [U][n]
+    int *hoff;              // HYB: slot column offsets [U] (col = row + off)
     double *hsink;          // HYB: outflow to the sink (n), or null
     double *p;              // solution (nv)
     const double *p0;       // initial vector (nv)
@@ -212,7 +212,7 @@ __device__ inline double ls_row(const LsInst &I, uint32_t r, const KryX<B> &X, d
     HybF<B> A;
     A.diag = I.hdiag;
     A.slot_vals = (const B *)I.hvals;
-    A.slot_cols = I.hcols;
+    A.slot_off = I.hoff;
     A.sink_out = I.with_sink ? I.hsink : nullptr;
     A.n = (uint32_t)I.n;
     A.U = I.U;
@@ -502,13 +502,8 @@ __global__ void __launch_bounds__(kBlock) k_ls_fill_hyb(LsArgs a, int mixed)
             if (I.with_sink) I.hsink[i] = out;
             for (int s = 0; s < I.U; ++s) {
                 const size_t o = (size_t)s * I.n + i;
-                if (source_in_box<D>(dm->rx[dm->slot_rxn[dm->slot_ptr[s]]], xs, dm)) {
-                    vals[o] = (V)slot_value<D>(dm, I.tab, nullptr, I.coefs, s, i, xs, 1);
-                    I.hcols[o] = (int)((int64_t)i - dm->slot_delta[s]);
-                } else {
-                    vals[o] = (V)0.0;
-                    I.hcols[o] = (int)i;
-                }
+                const bool in = source_in_box<D>(dm->rx[dm->slot_rxn[dm->slot_ptr[s]]], xs, dm);
+                vals[o] = (V)(in ? slot_value<D>(dm, I.tab, nullptr, I.coefs, s, i, xs, 1) : 0.0);
             }
         }
     }
@@ -572,7 +567,7 @@ struct LsHost {
     float *d_basis32 = nullptr;
     double *d_hdiag = nullptr, *d_hsink = nullptr;
     void *d_hvals = nullptr;
-    int *d_hcols = nullptr;
+    int *d_hoff = nullptr;
     int T = 0, grid = 0, graph_grid = 0;
     hipStream_t s = nullptr;
     hipGraph_t graph = nullptr;
@@ -609,7 +604,7 @@ void batch_lockstep_free(BatchImpl &B)
     if (L->s) (void)hipStreamDestroy(L->s);
     void *ptrs[] = {L->d_inst, L->d_st, L->d_tile_inst, L->d_tile_r0, L->d_atiles, L->d_cnt,
                     L->d_partials, L->d_basis, L->d_work, L->d_basis32, L->d_hdiag,
-                    L->d_hsink, L->d_hvals, L->d_hcols};
+                    L->d_hsink, L->d_hvals, L->d_hoff};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (L->h_cnt) (void)hipHostFree(L->h_cnt);
@@ -627,6 +622,7 @@ static fsp_status ls_build(BatchImpl &B)
     std::vector<int> tinst, tr0;
     std::vector<LsInst> hi((size_t)count);
     size_t basis_total = 0, basis32_total = 0, work_total = 0, diag_total = 0, slot_total = 0;
+    std::vector<int> offs;  // slot column offsets of every instance, -delta_u
     for (int i = 0; i < count; ++i) {
         const BatchInst &bi = B.h_inst[i];
         LsInst &I = hi[i];
@@ -648,6 +644,10 @@ static fsp_status ls_build(BatchImpl &B)
         I.U = B.hms[i].dm.U;
         diag_total += (size_t)bi.n;
         slot_total += (size_t)I.U * (size_t)bi.n;
+        for (int u = 0; u < I.U; ++u) {
+            const int64_t d = -B.hms[i].dm.slot_delta[u];
+            offs.push_back((int)std::max<int64_t>(-(int64_t)bi.n, std::min<int64_t>((int64_t)bi.n, d)));
+        }
     }
     L.T = (int)tinst.size();
     FSP_HIP(hipMalloc(&L.d_basis, sizeof(double) * basis_total));
@@ -657,8 +657,10 @@ static fsp_status ls_build(BatchImpl &B)
     FSP_HIP(hipMalloc(&L.d_hdiag, sizeof(double) * diag_total));
     FSP_HIP(hipMalloc(&L.d_hsink, sizeof(double) * diag_total));
     FSP_HIP(hipMalloc(&L.d_hvals, vbytes * std::max<size_t>(slot_total, 1)));
-    FSP_HIP(hipMalloc(&L.d_hcols, sizeof(int) * std::max<size_t>(slot_total, 1)));
-    size_t ob = 0, ob32 = 0, ow = 0, od = 0, os = 0;
+    FSP_HIP(hipMalloc(&L.d_hoff, sizeof(int) * std::max<size_t>(offs.size(), 1)));
+    if (!offs.empty())
+        FSP_HIP(copy_sync(L.d_hoff, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice, nullptr));
+    size_t ob = 0, ob32 = 0, ow = 0, od = 0, os = 0, oo = 0;
     for (int i = 0; i < count; ++i) {
         const BatchInst &bi = B.h_inst[i];
         LsInst &I = hi[i];
@@ -668,7 +670,7 @@ static fsp_status ls_build(BatchImpl &B)
         I.hdiag = L.d_hdiag + od;
         I.hsink = L.d_hsink + od;
         I.hvals = (char *)L.d_hvals + vbytes * os;
-        I.hcols = L.d_hcols + os;
+        I.hoff = L.d_hoff + oo;
         I.p = bi.p;
         I.p0 = bi.p0;
         I.V64 = L.d_basis + ob;
@@ -680,6 +682,7 @@ static fsp_status ls_build(BatchImpl &B)
         ow += 2 * (size_t)bi.nv;
         od += (size_t)bi.n;
         os += (size_t)I.U * (size_t)bi.n;
+        oo += (size_t)I.U;
     }
     FSP_HIP(hipMalloc(&L.d_inst, sizeof(LsInst) * (size_t)count));
     FSP_HIP(copy_sync(L.d_inst, hi.data(), sizeof(LsInst) * (size_t)count, hipMemcpyHostToDevice, nullptr));

@@ -500,8 +500,7 @@ fsp_status fsp_mat_export_hyb(fsp_mat A, double *d_diag, double *d_slot_vals, in
         FSP_HIP(hipMemcpyAsync(d_diag, M.diag, sizeof(double) * (size_t)M.n, hipMemcpyDeviceToDevice, s));
     if (d_slot_vals && slots)
         FSP_HIP(hipMemcpyAsync(d_slot_vals, M.slot_vals, sizeof(double) * slots, hipMemcpyDeviceToDevice, s));
-    if (d_slot_cols && slots)
-        FSP_HIP(hipMemcpyAsync(d_slot_cols, M.slot_cols, sizeof(int) * slots, hipMemcpyDeviceToDevice, s));
+    if (d_slot_cols && slots) FSP_TRY(mat_hyb_cols(M, d_slot_cols, s));
     if (d_sink_out && M.with_sink)
         FSP_HIP(hipMemcpyAsync(d_sink_out, M.sink_out, sizeof(double) * (size_t)M.n, hipMemcpyDeviceToDevice, s));
     FSP_HIP(hipStreamSynchronize(s));

@@ -89,37 +89,42 @@ __device__ inline double csr_tile(const CsrF<V> &A, uint32_t tile, const X &x,
 
 // --------------------------------------------------------------------- HYB
 // Merged FP64 diagonal + U ELL slots (SoA [U][n]), one per unique net
-// change; a missing neighbour stores col = row and value 0.
+// change.  The columns are implicit: slot u of row i sits at column
+// i + off[u] (off[u] = -delta_u, the inflow source; +delta_u for the
+// transpose), so only values are stored.  A neighbour outside the box
+// stores value 0; its column is then any valid state (the index itself is
+// clamped to row i when it falls outside 0..n-1), and 0 * x adds nothing.
 template <typename V>
 struct HybF {
     const double *__restrict__ diag;
     const V *__restrict__ slot_vals;
-    const int *__restrict__ slot_cols;
+    const int *__restrict__ slot_off;     // U column offsets
     const double *__restrict__ sink_out;  // dense outflow rates or nullptr
     uint32_t n;
     int U;
 };
 
-// Row i with a compile-time slot count: all slot values and columns are
-// loaded first (non-temporal: streamed once per product), then all gathers
-// of x, then the sum in slot order, so a row costs two memory round trips
-// whatever U is.
+__device__ inline uint32_t hyb_col(uint32_t i, int off, uint32_t n)
+{
+    const int64_t c = (int64_t)i + off;
+    return (c >= 0 && c < (int64_t)n) ? (uint32_t)c : i;
+}
+
+// Row i with a compile-time slot count: all slot values are loaded first
+// (non-temporal: streamed once per product), with all gathers of x at the
+// implicit columns, then the sum in slot order, so a row costs one memory
+// round trip whatever U is.
 template <int NU, typename V, class X>
 __device__ inline double hyb_row_n(const HybF<V> &A, uint32_t i, const X &x, dd &sink)
 {
     const double xi = x(i);
     double y = __builtin_nontemporal_load(A.diag + i) * xi;
     V v[NU];
-    int c[NU];
 #pragma unroll
-    for (int u = 0; u < NU; ++u) {
-        const size_t o = (size_t)u * A.n + i;
-        v[u] = __builtin_nontemporal_load(A.slot_vals + o);
-        c[u] = __builtin_nontemporal_load(A.slot_cols + o);
-    }
+    for (int u = 0; u < NU; ++u) v[u] = __builtin_nontemporal_load(A.slot_vals + (size_t)u * A.n + i);
     double xc[NU];
 #pragma unroll
-    for (int u = 0; u < NU; ++u) xc[u] = x(c[u]);
+    for (int u = 0; u < NU; ++u) xc[u] = x(hyb_col(i, A.slot_off[u], A.n));
 #pragma unroll
     for (int u = 0; u < NU; ++u) y += (double)v[u] * xc[u];
     if (A.sink_out) sink = dd_add_prod(sink, A.sink_out[i], xi);
@@ -149,7 +154,7 @@ __device__ inline double hyb_row(const HybF<V> &A, uint32_t i, const X &x, dd &s
     double y = A.diag[i] * xi;
     for (int u = 0; u < A.U; ++u) {
         const size_t o = (size_t)u * A.n + i;
-        y += (double)A.slot_vals[o] * x(A.slot_cols[o]);
+        y += (double)A.slot_vals[o] * x(hyb_col(i, A.slot_off[u], A.n));
     }
     if (A.sink_out) sink = dd_add_prod(sink, A.sink_out[i], xi);
     return y;
@@ -280,7 +285,7 @@ inline HybF<V> hyb_view(const MatImpl &M)
     HybF<V> A;
     A.diag = (const double *)M.diag;
     A.slot_vals = (const V *)M.slot_vals;
-    A.slot_cols = M.slot_cols;
+    A.slot_off = M.slot_off;
     A.sink_out = M.with_sink ? (const double *)M.sink_out : nullptr;
     A.n = (uint32_t)M.n;
     A.U = M.hm.dm.U;

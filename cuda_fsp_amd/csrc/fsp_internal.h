@@ -151,7 +151,7 @@ struct MatImpl {
     // HYB
     void *diag = nullptr;
     void *slot_vals = nullptr;
-    int *slot_cols = nullptr;
+    int *slot_off = nullptr;         // U column offsets (device): col = row + off[u]
     void *sink_out = nullptr;        // dense outflow rates (HYB, n)
     // reductions (sink row, deterministic last-block finalize)
     double *partials = nullptr;
@@ -172,6 +172,8 @@ fsp_status mat_apply_term(MatImpl &M, int r, double c, const double *d_x, double
 // Generator diagonal -sum_r c_r a_r(x_i), i < n (matrix free, any format).
 fsp_status mat_diagonal(MatImpl &M, double *d_diag, hipStream_t s);
 void mat_free(MatImpl &M);
+// Explicit HYB slot columns [U][n] of a (non-transposed) HYB matrix.
+fsp_status mat_hyb_cols(const MatImpl &M, int *d_cols, hipStream_t s);
 double mat_bytes_per_apply(const MatImpl &M);
 
 fsp_status term_csr(const HostModel &hm, int r, int *d_rp, int *d_cols, double *d_vals,

@@ -196,8 +196,7 @@ __global__ void k_fill_hyb(const DevModel *__restrict__ dm, const double *__rest
                            int use_lds, const double *__restrict__ table,
                            const double *__restrict__ coefs, uint32_t n, int mixed,
                            double *__restrict__ diag, V *__restrict__ slot_vals,
-                           int *__restrict__ slot_cols, double *__restrict__ sink_out,
-                           int transposed)
+                           double *__restrict__ sink_out, int transposed)
 {
     extern __shared__ double smem[];
     const double *tab = table_ptr(dm, tab_g, smem, use_lds);
@@ -217,16 +216,26 @@ __global__ void k_fill_hyb(const DevModel *__restrict__ dm, const double *__rest
             // row i of A^T: the outflow rate of state i towards its target
             const bool in = !target_leaves<D>(rx, xs, dm);
             slot_vals[o] = (V)(in ? slot_value<D>(dm, tab, table, coefs, u, i, xs, 0) : 0.0);
-            slot_cols[o] = in ? (int)((int64_t)i + dm->slot_delta[u]) : (int)i;
-            continue;
-        }
-        if (source_in_box<D>(rx, xs, dm)) {
-            slot_vals[o] = (V)slot_value<D>(dm, tab, table, coefs, u, i, xs, 1);
-            slot_cols[o] = (int)((int64_t)i - dm->slot_delta[u]);
         } else {
-            slot_vals[o] = (V)0.0;
-            slot_cols[o] = (int)i;
+            const bool in = source_in_box<D>(rx, xs, dm);
+            slot_vals[o] = (V)(in ? slot_value<D>(dm, tab, table, coefs, u, i, xs, 1) : 0.0);
         }
+    }
+}
+
+// Explicit HYB columns (for export): the inflow source i - delta_u when it
+// lies in the box, else i.
+template <int D>
+__global__ void k_hyb_cols(const DevModel *__restrict__ dm, uint32_t n, int *__restrict__ cols)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    int xs[D];
+    decode<D>(i, dm, xs);
+    for (int u = 0; u < dm->U; ++u) {
+        const RxnDev &rx = dm->rx[dm->slot_rxn[dm->slot_ptr[u]]];
+        cols[(size_t)u * n + i] =
+            source_in_box<D>(rx, xs, dm) ? (int)((int64_t)i - dm->slot_delta[u]) : (int)i;
     }
 }
 
@@ -325,12 +334,12 @@ static fsp_status fill_values(MatImpl &M, hipStream_t s)
         if (mixed) {
             FSP_SWITCH_D(dm.d, (k_fill_hyb<D_, float><<<blocks, kBlock, lds, s>>>(
                                    M.d_model, tab, use_lds, table, M.d_coefs, n, 1,
-                                   (double *)M.diag, (float *)M.slot_vals, M.slot_cols,
+                                   (double *)M.diag, (float *)M.slot_vals,
                                    (double *)M.sink_out, 0)));
         } else {
             FSP_SWITCH_D(dm.d, (k_fill_hyb<D_, double><<<blocks, kBlock, lds, s>>>(
                                    M.d_model, tab, use_lds, table, M.d_coefs, n, 0,
-                                   (double *)M.diag, (double *)M.slot_vals, M.slot_cols,
+                                   (double *)M.diag, (double *)M.slot_vals,
                                    (double *)M.sink_out, M.transposed)));
         }
     }
@@ -390,7 +399,13 @@ fsp_status mat_assemble(MatImpl &M, hipStream_t s)
         const size_t slots = (size_t)std::max(dm.U, 1) * (size_t)M.n;
         FSP_HIP(hipMalloc(&M.diag, sizeof(double) * (size_t)M.n));
         FSP_HIP(hipMalloc(&M.slot_vals, vb * slots));
-        FSP_HIP(hipMalloc(&M.slot_cols, sizeof(int) * slots));
+        std::vector<int> off((size_t)std::max(dm.U, 1), 0);
+        for (int u = 0; u < dm.U; ++u) {
+            const int64_t d = M.transposed ? dm.slot_delta[u] : -dm.slot_delta[u];
+            off[u] = (int)std::max<int64_t>(-(int64_t)M.n, std::min<int64_t>((int64_t)M.n, d));
+        }
+        FSP_HIP(hipMalloc(&M.slot_off, sizeof(int) * off.size()));
+        FSP_HIP(copy_sync(M.slot_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, s));
         if (M.with_sink) FSP_HIP(hipMalloc(&M.sink_out, sizeof(double) * (size_t)M.n));
         M.nnz_rows = (int64_t)M.n * (dm.U + 1);
     } else {
@@ -703,7 +718,7 @@ void mat_free(MatImpl &M)
 {
     void *ptrs[] = {M.d_model, M.d_compact, M.d_compact_f, M.d_coefs, M.rp, M.cols, M.vals,
                     M.sink_cols, M.sink_vals, M.sink_block_ptr, M.diag, M.slot_vals,
-                    M.slot_cols, M.sink_out, M.partials, M.ticket};
+                    M.slot_off, M.sink_out, M.partials, M.ticket};
     for (void *p : ptrs)
         if (p) hipFree(p);
 }
@@ -719,12 +734,21 @@ double mat_bytes_per_apply(const MatImpl &M)
         if (M.prec == FSP_MIXED) b += 8.0 * n;  // separate FP64 diagonal
         if (M.with_sink) b += 12.0 * (double)M.nnz_sink;
     } else if (M.format == FSP_HYB) {
-        b = 8.0 * n + (vb + 4.0) * (double)M.hm.dm.U * n + 16.0 * n;
+        b = 8.0 * n + vb * (double)M.hm.dm.U * n + 16.0 * n;  // columns implicit
         if (M.with_sink) b += 8.0 * n;
     } else {
         b = 16.0 * n;
     }
     return b;
+}
+
+fsp_status mat_hyb_cols(const MatImpl &M, int *d_cols, hipStream_t s)
+{
+    const uint32_t n = (uint32_t)M.n;
+    const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    FSP_SWITCH_D(M.hm.dm.d, (k_hyb_cols<D_><<<blocks, kBlock, 0, s>>>(M.d_model, n, d_cols)));
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
 }
 
 // ============================================================= term CSR
