@@ -320,7 +320,7 @@ def kernels_only(reps):
         capi.check(capi.lib().fsp_vec_fill(x.data_ptr(), 1.0, n, None))
     for _ in range(reps):
         capi.vec_sum(x, n)
-    for fmt, prec in (("csr", "f64"), ("hyb", "f64"), ("csr", "mixed")):
+    for fmt, prec in (("csr", "f64"), ("hyb", "f64"), ("csr", "mixed"), ("hyb", "mixed")):
         A = capi.Mat(big, fmt, prec)
         y = capi.DeviceBuffer.zeros(n)
         for _ in range(reps):

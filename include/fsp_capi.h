@@ -141,7 +141,9 @@ fsp_status fsp_mat_apply_term(fsp_mat A, int reaction, double c, const double *d
 fsp_status fsp_mat_export_csr(fsp_mat A, int *d_row_ptrs, int *d_cols, double *d_vals,
                               void *stream);
 /* Copy out the HYB slots (FSP_HYB only): diag[n], slot_vals[U*n],
- * slot_cols[U*n], sink_out[n] (may be NULL). */
+ * slot_cols[U*n], sink_out[n] (may be NULL).  The library stores no
+ * columns (slot u of row i reads x at i - delta_u); slot_cols is rebuilt:
+ * i - delta_u where that source lies in the box, else i. */
 fsp_status fsp_mat_export_hyb(fsp_mat A, double *d_diag, double *d_slot_vals,
                               int *d_slot_cols, double *d_sink_out, void *stream);
 
