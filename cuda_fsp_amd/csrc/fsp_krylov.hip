@@ -9,7 +9,9 @@
 //                  into the read), w_{j+1} = A V_j, partial dots V_k . w_{j+1}
 //   k_arn_dots   : H(k, j) = sum of partials (fixed order), sink element
 //   k_arn_update : w_{j+1} -= V h (gemv N of :79-81), partial |w|^2
-//   k_arn_norm   : s = |w|, happy breakdown test, H(j+1, j) = s, sigma = 1/s
+//   (next k_arn_spmv, every workgroup): s = |w|, happy breakdown test,
+//                  H(j+1, j) = s, sigma = 1/s (k_arn_norm with a host
+//                  callback operator)
 // With an FP32 basis (mixed precision) one classical Gram-Schmidt pass, as
 // the reference does in FP64, lets the loss of orthogonality compound by
 // |w| / h(j+1, j) per step until the basis is degenerate.  The mixed path
@@ -80,6 +82,8 @@ struct ArnArgs {
     uint32_t tiles;
     int j, istart, mode;  // mode 0: dots, 1: norm only (avnorm)
     int G;
+    double *npart;        // [G] dd partials of |w_j|^2 from the update kernel
+    int M2;
 };
 
 constexpr int kRedLen = 2 * kBlock / kWave;
@@ -244,6 +248,38 @@ __device__ inline void acc_zero(dd (&acc)[NKB])
     for (int kk = 0; kk < NKB; ++kk) acc[kk] = dd_zero();
 }
 
+// Scale of the input vector: sigma_0 for j = 0; for j >= 1 every workgroup
+// sums the update kernel's |w_j|^2 partials itself (the fixed-order sum of
+// a finalizing kernel, so the same bits in every workgroup) and workgroup 0
+// records H(j, j-1) = |w_j| and sigma_j, or the happy breakdown
+// (src/KryExpvFSP.cu:90-98), after which every workgroup returns.  This
+// replaces a one-workgroup norm kernel between the update and this kernel.
+// Returns false when the kernel has nothing to do.
+__device__ inline bool arn_sigma(const ArnArgs &a, double *red, double &sig)
+{
+    KState *st = a.st;
+    if (a.j == 0) {
+        sig = st->sigma[0];
+        return true;
+    }
+    const double s = sqrt(dd_value(sum_dd_partials(a.npart, a.G, red)));
+    const bool broke = s < st->btol;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (broke) {
+            st->k1 = 0;
+            st->mb = a.j;
+            st->tau = st->t_final - st->t_now;
+            st->broke = 1;
+            st->n_breakdown += 1;
+        } else {
+            st->H[a.j + (a.j - 1) * a.M2] = s;
+            st->sigma[a.j] = 1.0 / s;
+        }
+    }
+    sig = 1.0 / s;
+    return !broke;
+}
+
 template <typename V, typename B>
 __global__ void __launch_bounds__(kBlock) k_arn_csr(CsrF<V> A, ArnArgs a)
 {
@@ -251,8 +287,10 @@ __global__ void __launch_bounds__(kBlock) k_arn_csr(CsrF<V> A, ArnArgs a)
     extern __shared__ double smem[];
     double *red = smem;
     double *prod = red + kArnRedLen;
+    double sig;
+    if (!arn_sigma(a, red, sig)) return;
     dd nrm = dd_zero();
-    const KryX<B> X{a.in, a.st->sigma[a.j], a.j};
+    const KryX<B> X{a.in, sig, a.j};
     dd sink = dd_zero();
     for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
         const double y = csr_tile<V>(A, w.t, X, prod, sink);
@@ -268,8 +306,10 @@ __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
     if (arn_skip<B>(a)) return;
     extern __shared__ double smem[];
     double *red = smem;
+    double sig;
+    if (!arn_sigma(a, red, sig)) return;
     dd nrm = dd_zero();
-    const KryX<B> X{a.in, a.st->sigma[a.j], a.j};
+    const KryX<B> X{a.in, sig, a.j};
     dd sink = dd_zero();
     for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
         const uint32_t i = w.t * kBlock + threadIdx.x;
@@ -291,8 +331,10 @@ __global__ void __launch_bounds__(kBlock) k_arn_kron(KronF<V> A, ArnArgs a, int 
         stage_table(tabl, A.tab_g, A.tab_size);
         tab = tabl;
     }
+    double sig;
+    if (!arn_sigma(a, red, sig)) return;
     dd nrm = dd_zero();
-    const KryX<B> X{a.in, a.st->sigma[a.j], a.j};
+    const KryX<B> X{a.in, sig, a.j};
     dd sink = dd_zero();
     for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
         const uint32_t i = w.t * kBlock + threadIdx.x;
@@ -395,8 +437,8 @@ __global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2)
     }
     const dd t = block_sum_dd(nrm, red);
     if (threadIdx.x == 0) {
-        a.partials[2 * blockIdx.x] = t.s;
-        a.partials[2 * blockIdx.x + 1] = t.c;
+        a.npart[2 * blockIdx.x] = t.s;
+        a.npart[2 * blockIdx.x + 1] = t.c;
     }
 }
 
@@ -405,7 +447,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_norm(ArnArgs a, int M2, int G)
     KState *st = a.st;
     if (!st->active || st->broke) return;
     __shared__ double red[kRedLen];
-    const double s = sqrt(dd_value(sum_dd_partials(a.partials, G, red)));
+    const double s = sqrt(dd_value(sum_dd_partials(a.npart, G, red)));
     if (threadIdx.x == 0) {
         if (s < st->btol) {  // happy breakdown (src/KryExpvFSP.cu:90-98)
             st->k1 = 0;
@@ -527,7 +569,8 @@ static fsp_status expv_alloc(ExpvImpl &E)
     int cap = 1024;
     if (const char *g = std::getenv("FSP_ARN_GRID")) cap = std::max(8, std::atoi(g));
     E.G = grid_for(E.tiles, cap);
-    FSP_HIP(hipMalloc(&E.partials, 2 * sizeof(double) * (size_t)E.G * kArnSlots));
+    // row-kernel slots, then the update kernel's |w|^2 partials
+    FSP_HIP(hipMalloc(&E.partials, 2 * sizeof(double) * (size_t)E.G * (kArnSlots + 1)));
     KState &h = *E.h_st;
     h.t_final = 0.0;
     h.t_now = 0.0;
@@ -624,6 +667,8 @@ static fsp_status enqueue_step(ExpvImpl &E, double *d_v)
     a.nv = nv;
     a.tiles = (uint32_t)E.tiles;
     a.G = E.G;
+    a.npart = E.partials + 2 * (size_t)E.G * kArnSlots;
+    a.M2 = M2;
     int istart = 0;
     for (int j = 0; j <= m; ++j) {
         const bool last = (j == m);
@@ -648,7 +693,8 @@ static fsp_status enqueue_step(ExpvImpl &E, double *d_v)
         if (last) break;
         if (E.mixed) k_gram_coef<<<1, kBlock, kGramLdsBytes, E.s>>>(a, M2);
         k_arn_update<B><<<E.G, kBlock, 0, E.s>>>(a, M2);
-        k_arn_norm<<<1, kBlock, 0, E.s>>>(a, M2, E.G);
+        // a fused operator's next row kernel finishes |w| itself
+        if (!E.A) k_arn_norm<<<1, kBlock, 0, E.s>>>(a, M2, E.G);
         FSP_HIP(hipGetLastError());
     }
     const size_t lds = sizeof(double) * 5 * (size_t)M2 * M2;
