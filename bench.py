@@ -18,6 +18,8 @@ Also reported on the same line:
                 solves/s and worst L1 error vs FP64 over the output times
   cpu_baseline  the CPU oracle (restatement of the reference algorithm) on a
                 bounded sample of the same solve, extrapolated to solves/s
+  chain_mixed   the 16.7M-state chain solved to t = 20 in FP64 and in mixed
+                precision: times, L1 of p and of the 1-D marginals
   bdf, adjoint  the time-varying BDF solve; one log-likelihood gradient, and
                 gradients at several parameter points one at a time and
                 batched on 4 streams
@@ -162,6 +164,46 @@ def cpu_baseline(net, t_out, budget_s=15.0, threads=None):
     L.orc_expv_get_stats(e, C.byref(st))
     L.orc_expv_destroy(e)
     return dt, steps, threads
+
+
+def bench_chain_mixed():
+    """Mixed precision against FP64 at scale: the 16,777,216-state
+    five-species chain (rates log-uniform in [1e-3, 10], seed 0) from
+    delta(0) to t = 20, Krylov m = 30, tol = 1e-8, HYB generator in FP64 and
+    in mixed precision; L1 distance of p(20) and of each 1-D marginal."""
+    net = M.five_species_chain()
+    n = net.n_states
+    p0 = M.delta_p0(net)
+    out, info = {}, {}
+    for prec in ("f64", "mixed"):
+        A = capi.Mat(net, "hyb", prec)
+        E = capi.Expv(A, n, 20.0, m=30, tol=1e-8)
+        v = capi.DeviceBuffer.from_host(p0)
+        capi.check(capi.lib().fsp_stream_sync(None))
+        t0 = time.perf_counter()
+        E.solve(v)
+        capi.check(capi.lib().fsp_stream_sync(None))
+        dt = time.perf_counter() - t0
+        st = E.stats()
+        out[prec] = v.to_host()
+        info[prec] = {"solve_s": round(dt, 3), "krylov_steps": int(st.i_step),
+                      "mass": float(out[prec].sum())}
+        print(f"[bench] chain t=20 {prec}: {dt:.2f} s, {st.i_step} Krylov steps", flush=True)
+        v.free()
+        E.close()
+        A.close()
+    dims = [int(b) + 1 for b in net.bounds]
+    cube = {k: v.reshape(dims[::-1]) for k, v in out.items()}
+    marg = []
+    for sp in range(len(dims)):
+        ax = tuple(a for a in range(len(dims)) if a != len(dims) - 1 - sp)
+        marg.append(float(np.abs(cube["mixed"].sum(axis=ax) - cube["f64"].sum(axis=ax)).sum()))
+    return {"workload": f"five_species_chain {'x'.join(str(d) for d in dims)} ({n:,} states), "
+                        "delta(0) to t=20, m=30 tol=1e-8, HYB generator",
+            "f64": info["f64"], "mixed": info["mixed"],
+            "speedup_mixed_vs_f64": round(info["f64"]["solve_s"] / info["mixed"]["solve_s"], 3),
+            "l1_p_mixed_vs_f64": float(np.abs(out["mixed"] - out["f64"]).sum()),
+            "l1_marginals_mixed_vs_f64": marg}
 
 
 def bench_bdf(net):
@@ -368,6 +410,8 @@ def main():
                     help="parameter points in the batched-gradient section")
     ap.add_argument("--grad-only", action="store_true",
                     help="only the adjoint-gradient section")
+    ap.add_argument("--chain-only", action="store_true",
+                    help="only the 16.7M-state mixed-vs-FP64 solve")
     args = ap.parse_args()
     if args.kernels_only:
         kernels_only(5)
@@ -377,6 +421,9 @@ def main():
         return
     if args.grad_only:
         grad_only(args.grad_points)
+        return
+    if args.chain_only:
+        print(json.dumps(bench_chain_mixed()), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path "
@@ -433,6 +480,8 @@ def main():
 
     extras = {}
     if not args.no_extras:
+        print("[bench] 16.7M-state chain, mixed vs FP64", flush=True)
+        extras["chain_mixed"] = bench_chain_mixed()
         print("[bench] BDF time-varying solve", flush=True)
         extras["bdf"] = bench_bdf(net)
         print("[bench] adjoint gradient", flush=True)
