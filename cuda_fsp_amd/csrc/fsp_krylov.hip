@@ -842,10 +842,19 @@ extern "C" fsp_status fsp_expv_solve(fsp_expv e, double *d_v)
     if (!e || !d_v) return fail(FSP_ERR_INVALID, "NULL argument");
     ExpvImpl &E = e->impl;
     FSP_TRY(pull_state(E));
-    // Steps are launched ahead of the host check (inactive steps do nothing).
+    // Steps are launched ahead of the host check (inactive steps do nothing),
+    // up to 4, but no more than the remaining interval needs at the current
+    // step size: a solve to a nearby output time then launches one step, not
+    // three more that would only return.
     const int ahead = E.A ? 4 : 1;
     while (E.h_st->t_now < E.h_st->t_final) {
-        for (int k = 0; k < ahead; ++k) FSP_TRY(launch_step(E, d_v));
+        const KState &h = *E.h_st;
+        int launch = ahead;
+        if (ahead > 1 && h.i_step > 0 && h.t_new > 0.0) {
+            const double est = std::ceil((h.t_final - h.t_now) / h.t_new);
+            if (std::isfinite(est)) launch = (int)std::max(1.0, std::min((double)ahead, est));
+        }
+        for (int k = 0; k < launch; ++k) FSP_TRY(launch_step(E, d_v));
         FSP_TRY(pull_state(E));
         if (E.h_st->i_step > E.max_nstep)
             return fail(FSP_ERR_NOT_CONVERGED, "Krylov step count exceeded max_nstep");
