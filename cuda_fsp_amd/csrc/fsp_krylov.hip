@@ -428,9 +428,17 @@ __global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2)
     const double *h = st->H + (size_t)a.j * M2;
     dd nrm = dd_zero();
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += gridDim.x * kBlock) {
+        // k ascending as one sum; the basis loads of 8 terms are issued together
         double acc = 0.0;
-        for (int k = a.istart; k <= a.j; ++k)
-            acc += basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) * h[k];
+        int k = a.istart;
+        for (; k + kDotChunk <= a.j + 1; k += kDotChunk) {
+            double v[kDotChunk];
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q) v[q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, k + q, i);
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q) acc += v[q] * h[k + q];
+        }
+        for (; k <= a.j; ++k) acc += basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) * h[k];
         const double w = -1.0 * acc + a.out[i];
         a.out[i] = w;
         nrm = dd_add_prod(nrm, w, w);
@@ -505,7 +513,15 @@ __global__ void __launch_bounds__(kBlock) k_solution(const KState *st, const dou
     const double beta = st->beta;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nv; i += gridDim.x * kBlock) {
         double acc = 0.0;
-        for (int k = 0; k < mx; ++k) acc += basis<B>(V64, Vb, nv, k, i) * st->F0[k];
+        int k = 0;
+        for (; k + kDotChunk <= mx; k += kDotChunk) {
+            double b[kDotChunk];
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q) b[q] = basis<B>(V64, Vb, nv, k + q, i);
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q) acc += b[q] * st->F0[k + q];
+        }
+        for (; k < mx; ++k) acc += basis<B>(V64, Vb, nv, k, i) * st->F0[k];
         v[i] = beta * acc;
     }
 }
