@@ -398,8 +398,17 @@ __global__ void __launch_bounds__(kBlock) k_ls_update(LsArgs a)
         const B *Vb = ls_vb<B>(I);
         dd nrm = dd_zero();
         if (r < (uint32_t)I.nv) {
+            // k ascending as one sum; 8 basis loads issued together
             double acc = 0.0;
-            for (int k = a.istart; k <= a.j; ++k) acc += basis<B>(I.V64, Vb, (size_t)I.nv, k, r) * h[k];
+            int k = a.istart;
+            for (; k + kLsChunk * 2 <= a.j + 1; k += kLsChunk * 2) {
+                double v[kLsChunk * 2];
+#pragma unroll
+                for (int q = 0; q < kLsChunk * 2; ++q) v[q] = basis<B>(I.V64, Vb, (size_t)I.nv, k + q, r);
+#pragma unroll
+                for (int q = 0; q < kLsChunk * 2; ++q) acc += v[q] * h[k + q];
+            }
+            for (; k <= a.j; ++k) acc += basis<B>(I.V64, Vb, (size_t)I.nv, k, r) * h[k];
             const double w = -1.0 * acc + out[r];
             out[r] = w;
             nrm = dd_add_prod(nrm, w, w);
