@@ -77,24 +77,56 @@ __global__ void k_wsq_partials(const double *__restrict__ x, const double *__res
     }
 }
 
-// partials[k][b] = dd sum over rows of block b of V_k . w, k < nk
+constexpr int kDotChunk = 8;  // dots per pass over the rows
+
+// partials[k][b] = dd sum over rows of block b of V_k . w, k < nk; 8 dots
+// per pass, so w is read once per 8 basis vectors and their loads are
+// issued together (each dot's sum runs over the rows in the same order)
 __global__ void k_multi_dot(const double *__restrict__ V, int64_t ldv, int nk,
                             const double *__restrict__ w, int64_t n, double *partials)
 {
     __shared__ double red[2 * kBlock / kWave];
-    for (int k = 0; k < nk; ++k) {
-        const double *vk = V + (int64_t)k * ldv;
-        dd s = dd_zero();
+    for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
+        dd s[kDotChunk];
+#pragma unroll
+        for (int q = 0; q < kDotChunk; ++q) s[q] = dd_zero();
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-             i += (int64_t)gridDim.x * blockDim.x)
-            s = dd_add_prod(s, vk[i], w[i]);
-        s = block_sum_dd(s, red);
-        if (threadIdx.x == 0) {
-            partials[2 * ((int64_t)k * gridDim.x + blockIdx.x)] = s.s;
-            partials[2 * ((int64_t)k * gridDim.x + blockIdx.x) + 1] = s.c;
+             i += (int64_t)gridDim.x * blockDim.x) {
+            const double wi = w[i];
+            double v[kDotChunk];
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q) v[q] = V[(int64_t)min(c0 + q, nk - 1) * ldv + i];
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q) s[q] = dd_add_prod(s[q], v[q], wi);
+        }
+#pragma unroll
+        for (int q = 0; q < kDotChunk; ++q) {
+            if (c0 + q >= nk) break;
+            const dd t = block_sum_dd(s[q], red);
+            if (threadIdx.x == 0) {
+                partials[2 * ((int64_t)(c0 + q) * gridDim.x + blockIdx.x)] = t.s;
+                partials[2 * ((int64_t)(c0 + q) * gridDim.x + blockIdx.x) + 1] = t.c;
+            }
         }
         __syncthreads();
     }
+}
+
+// sum_k c_k V_k[i], k ascending, 8 loads issued together
+__device__ inline double combo(const double *__restrict__ V, int64_t ldv, int nk,
+                               const double *__restrict__ c, int64_t i)
+{
+    double acc = 0.0;
+    int k = 0;
+    for (; k + kDotChunk <= nk; k += kDotChunk) {
+        double v[kDotChunk];
+#pragma unroll
+        for (int q = 0; q < kDotChunk; ++q) v[q] = V[(int64_t)(k + q) * ldv + i];
+#pragma unroll
+        for (int q = 0; q < kDotChunk; ++q) acc += v[q] * c[k + q];
+    }
+    for (; k < nk; ++k) acc += V[(int64_t)k * ldv + i] * c[k];
+    return acc;
 }
 
 // out[k] = value of the dd sum of partials[k][0..G), one workgroup per k
@@ -114,8 +146,7 @@ __global__ void k_gs_update(const double *__restrict__ V, int64_t ldv, int nk,
     dd s = dd_zero();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        double acc = 0.0;
-        for (int k = 0; k < nk; ++k) acc += V[(int64_t)k * ldv + i] * h[k];
+        const double acc = combo(V, ldv, nk, h, i);
         const double v = -1.0 * acc + w[i];
         w[i] = v;
         s = dd_add_prod(s, v, v);
@@ -176,8 +207,7 @@ __global__ void k_sol_update(const double *__restrict__ V, int64_t ldv, int nk,
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        double acc = 0.0;
-        for (int k = 0; k < nk; ++k) acc += V[(int64_t)k * ldv + i] * g[k];
+        const double acc = combo(V, ldv, nk, g, i);
         y[i] += acc / ((alpha0 - diag[i]) * ewt[i]);
     }
 }
