@@ -7,7 +7,8 @@
 namespace fsp {
 
 constexpr int kP10Lo = -300, kP10Hi = 300;
-constexpr int kElimChunk = 8;  // register chunk of the elimination's row updates
+constexpr int kElimChunk = 4;  // register chunk of the elimination's row updates
+constexpr int kSubChunk = 8;   // loads issued together in back substitution
 
 // C = A * B for column-major N x N matrices in LDS.  Each thread computes a
 // BS x BS block of C from BS + BS LDS reads per BS^2 products; every element
@@ -180,8 +181,20 @@ __device__ inline void expm_block(int N, double *A, double *X, double *T, double
     __syncthreads();
     for (int j = tid; j < N; j += blockDim.x) {
         for (int i = N - 1; i >= 0; --i) {
+            // k ascending as one sum; kSubChunk loads issued together
             double acc = X[j + i * N];
-            for (int k = i + 1; k < N; ++k) acc -= Dm[i + k * N] * X[j + k * N];
+            int k = i + 1;
+            for (; k + kSubChunk <= N; k += kSubChunk) {
+                double d[kSubChunk], x[kSubChunk];
+#pragma unroll
+                for (int u = 0; u < kSubChunk; ++u) {
+                    d[u] = Dm[i + (k + u) * N];
+                    x[u] = X[j + (k + u) * N];
+                }
+#pragma unroll
+                for (int u = 0; u < kSubChunk; ++u) acc -= d[u] * x[u];
+            }
+            for (; k < N; ++k) acc -= Dm[i + k * N] * X[j + k * N];
             X[j + i * N] = acc / Dm[i + i * N];
         }
     }
