@@ -115,6 +115,25 @@ def test_batched_gradients_equal_single_points():
         assert np.array_equal(g[i], gi)
 
 
+def test_gradient_with_more_than_16_reactions():
+    # every reaction twice at half its rate: the same generator, R = 18
+    # (two reaction chunks in the sensitivity kernel); each half-rate copy's
+    # dLL/dlog c is half of the original reaction's
+    net = M.gene_expression((2, 9, 8))
+    times = [1.0, 3.0]
+    counts = _data(net, times, 150, seed=8)
+    dup = M.ReactionNetwork("dup", net.bounds, np.concatenate([net.nu, net.nu]),
+                            np.concatenate([net.factors, net.factors]),
+                            np.concatenate([0.5 * net.coefs, 0.5 * net.coefs]))
+    print("[case] gradient with 18 reactions (duplicated gene 3x10x9)", flush=True)
+    ll, g = capi.loglik(net, np.log(net.coefs), times, KEEP, counts, tol=1e-12, grad=True)
+    ll2, g2 = capi.loglik(dup, np.log(dup.coefs), times, KEEP, counts, tol=1e-12, grad=True)
+    R = net.n_reactions
+    assert abs(ll2 - ll) <= 1e-10 * abs(ll)
+    assert np.max(np.abs(g2[:R] - 0.5 * g)) <= 1e-8 * np.max(np.abs(g))
+    assert np.max(np.abs(g2[R:] - 0.5 * g)) <= 1e-8 * np.max(np.abs(g))
+
+
 def test_torch_autograd_bridge():
     import torch
 
