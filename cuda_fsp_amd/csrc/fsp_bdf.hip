@@ -266,7 +266,7 @@ struct BdfImpl {
     double *diag = nullptr;         // generator diagonal at the stage time
     double *V = nullptr;            // (kdim + 1) x n scaled Krylov basis
     double *partials = nullptr;     // 2 x (kdim + 1) x kRedGrid
-    double *dred = nullptr;         // (kdim + 1) reduced values
+    double *dred = nullptr;         // 2 (kdim + 1) + 1 reduced values
     double *h_red = nullptr;        // pinned host copy
     double *d_small = nullptr;      // kdim + 1 coefficients on the device
     std::vector<double> coefs;
@@ -337,7 +337,7 @@ fsp_status gmres(BdfImpl &B, double alpha0, double tol, double *y, bool *converg
     const int64_t n = B.n;
     const int G = vec_grid(n);
     const double sqn = std::sqrt((double)n);
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hcol(m + 1), hc2(m + 1);
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hcol(m + 1);
     *converged = false;
     for (int cycle = 0; cycle <= B.opt.max_restarts; ++cycle) {
         // r0 = D (b - alpha0 y + A y)
@@ -367,23 +367,30 @@ fsp_status gmres(BdfImpl &B, double alpha0, double tol, double *y, bool *converg
             hipLaunchKernelGGL(k_op_post, dim3(G), dim3(kBlock), 0, B.s, vj, B.ewt, B.diag, B.u,
                                alpha0, w, n);
             FSP_HIP(hipGetLastError());
-            // classical Gram-Schmidt, two passes
+            // classical Gram-Schmidt, two passes; the coefficients stay on
+            // the device (pass p in dred[p (j+1) ..], |w|^2 after them) and
+            // reach the host in one copy for the Givens rotations
             std::fill(hcol.begin(), hcol.end(), 0.0);
-            double wn = 0.0;
             for (int pass = 0; pass < 2; ++pass) {
+                double *hp = B.dred + (size_t)pass * (j + 1);
                 hipLaunchKernelGGL(k_multi_dot, dim3(kRedGrid), dim3(kBlock), 0, B.s, B.V, n, j + 1,
                                    w, n, B.partials);
-                FSP_HIP(hipGetLastError());
-                FSP_TRY(pull_reduced(B, j + 1, hc2.data()));
-                FSP_TRY(push_small(B, hc2.data(), j + 1));
+                hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(kBlock), 0, B.s, B.partials,
+                                   kRedGrid, hp);
                 hipLaunchKernelGGL(k_gs_update, dim3(kRedGrid), dim3(kBlock), 0, B.s, B.V, n, j + 1,
-                                   B.d_small, w, n, B.partials);
+                                   hp, w, n, B.partials);
                 FSP_HIP(hipGetLastError());
-                double s2 = 0.0;
-                FSP_TRY(pull_reduced(B, 1, &s2));
-                wn = std::sqrt(s2);
-                for (int i = 0; i <= j; ++i) hcol[i] += hc2[i];
             }
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, B.s, B.partials, kRedGrid,
+                               B.dred + 2 * (size_t)(j + 1));
+            FSP_HIP(hipGetLastError());
+            const int nred = 2 * (j + 1) + 1;
+            FSP_HIP(hipMemcpyAsync(B.h_red, B.dred, sizeof(double) * (size_t)nred,
+                                   hipMemcpyDeviceToHost, B.s));
+            FSP_HIP(hipStreamSynchronize(B.s));
+            for (int pass = 0; pass < 2; ++pass)
+                for (int i = 0; i <= j; ++i) hcol[i] += B.h_red[(size_t)pass * (j + 1) + i];
+            const double wn = std::sqrt(B.h_red[2 * (j + 1)]);
             hcol[j + 1] = wn;
             if (wn > 0.0)
                 hipLaunchKernelGGL(k_scale_copy, dim3(G), dim3(kBlock), 0, B.s, w, 1.0 / wn, w, n);
@@ -432,9 +439,9 @@ fsp_status bdf_alloc(BdfImpl &B)
     for (double **p : vs) FSP_HIP(hipMalloc(p, sizeof(double) * n));
     FSP_HIP(hipMalloc(&B.V, sizeof(double) * n * kd));
     FSP_HIP(hipMalloc(&B.partials, sizeof(double) * 2 * kd * kRedGrid));
-    FSP_HIP(hipMalloc(&B.dred, sizeof(double) * kd));
+    FSP_HIP(hipMalloc(&B.dred, sizeof(double) * (2 * kd + 1)));
     FSP_HIP(hipMalloc(&B.d_small, sizeof(double) * kd));
-    FSP_HIP(hipHostMalloc((void **)&B.h_red, sizeof(double) * kd, hipHostMallocDefault));
+    FSP_HIP(hipHostMalloc((void **)&B.h_red, sizeof(double) * (2 * kd + 1), hipHostMallocDefault));
     return FSP_OK;
 }
 
