@@ -116,12 +116,28 @@ def bench_spmv(net, fmt, precision, reps, ev):
         A.apply(x, y, accumulate=False)
     ev.record(b)
     ms = ev.elapsed_ms(a, b) / reps
-    bytes_ = A.info.bytes_per_apply
+    moved = A.info.bytes_per_apply  # bytes this implementation must move
+    bytes_ = algorithmic_bytes(fmt, precision, n, int(A.info.nnz), moved)
     out = dict(format=fmt, precision=precision, n=n, nnz=int(A.info.nnz),
                us_per_apply=round(ms * 1e3, 2), algorithmic_bytes=int(bytes_),
-               achieved_GBs=round(bytes_ / (ms * 1e-3) / 1e9, 1))
+               achieved_GBs=round(bytes_ / (ms * 1e-3) / 1e9, 1),
+               bytes_moved=int(moved), moved_GBs=round(moved / (ms * 1e-3) / 1e9, 1))
     A.close()
     return out
+
+
+def algorithmic_bytes(fmt, precision, n, nnz, moved):
+    """Algorithmic bytes of one product by the standard per-format count.
+    HYB: v n (diagonal) + (v + 4) U n (slot values and int32 columns) + 16 n
+    (x read once, y written), v = 8 (FP64) or 4 (mixed), U n = nnz - n.
+    This library stores no HYB columns (they follow from the slot's net
+    change), so it moves 4 U n bytes less (`moved`).  Merged CSR and
+    Kronecker: the library's count, which is the standard one (mixed CSR
+    adds its separate FP64 diagonal)."""
+    if fmt == "hyb":
+        v = 8 if precision == "f64" else 4
+        return v * n + (v + 4) * (nnz - n) + 16 * n
+    return moved
 
 
 def cpu_baseline(net, t_out, budget_s=15.0, threads=None):
@@ -468,14 +484,23 @@ def main():
         print(f"[bench] SpMV {fmt}/{prec} on n={big.n_states}", flush=True)
         spmv.append(bench_spmv(big, fmt, prec, args.spmv_reps, ev))
     head = spmv[0]
+    traffic = pmc_traffic()
     roofline = {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(head["achieved_GBs"] / HBM_PEAK_GBS, 4),
-                "traffic": pmc_traffic(),
+                "traffic": traffic,
                 "traffic_source": "profiles/chain_16m_spmv_pmc_traffic.md (FETCH_SIZE, WRITE_SIZE "
                                   "scaled by a read and a write kernel of known size)",
                 "kernel": "k_apply_hyb<10, double> (HYB SpMV, FP64, 10 slots)",
                 "workload": f"five_species_chain n={big.n_states} slots={head['nnz']}",
                 "algorithmic_bytes": head["algorithmic_bytes"],
+                "algorithmic_bytes_formula": "8n + 12 U n + 16n (HYB with int32 columns)",
+                "bytes_moved": head["bytes_moved"],
+                "bytes_moved_formula": "8n + 8 U n + 16n (columns derived, not stored)",
+                "moved_GBs": head["moved_GBs"],
+                "hbm_measured_GBs": (round(traffic / (head["us_per_apply"] * 1e-6) / 1e9, 1)
+                                     if traffic else None),
+                "hbm_measured_frac": (round(traffic / (head["us_per_apply"] * 1e-6) / 1e9
+                                            / HBM_PEAK_GBS, 4) if traffic else None),
                 "us_per_launch": head["us_per_apply"]}
 
     cpu = None
