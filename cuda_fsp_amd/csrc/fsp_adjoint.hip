@@ -15,12 +15,15 @@
 // Krylov expv on A^T, assembled in HYB: row j holds the outflow rates of
 // state j at its targets' columns), p forwards; the integral uses
 // Gauss-Legendre panels graded geometrically towards both ends of every
-// observation interval (the integrand has transients of width ~1/|A| there).  Checkpointing: the
-// forward pass keeps p only at the observation times; the backward pass
-// recomputes p at one interval's nodes from its checkpoint.
+// observation interval (the integrand has transients of width ~1/|A| there).
+// Checkpointing: the forward pass keeps p at the observation times and,
+// when all of them fit in half the free device memory, at every quadrature
+// node; otherwise the backward pass recomputes p at one interval's nodes
+// from the checkpoint at the interval's start.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -313,6 +316,28 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
     FSP_HIP(copy_sync(d_keep, keep.data(), sizeof(int) * (size_t)d, hipMemcpyHostToDevice, s));
     FSP_HIP(copy_sync(cnt, data->counts, sizeof(double) * (size_t)(nb * K), hipMemcpyHostToDevice, s));
 
+    // quadrature nodes of every observation interval (t_{k-1}, t_k]
+    const int nodes_per = 2 * o.grading_levels * o.quad_nodes;
+    std::vector<std::vector<double>> sn((size_t)K), wn((size_t)K);
+    for (int k = 0; k < K; ++k)
+        graded_rule(k ? data->times[k - 1] : 0.0, data->times[k], o.quad_nodes, o.grading_levels,
+                    sn[k], wn[k]);
+    // With the gradient, p at every node is kept from the forward pass when
+    // all K * nodes_per vectors fit in half the free device memory; else
+    // the backward pass recomputes one interval's nodes from the checkpoint
+    // at its start.  FSP_ADJOINT_RECOMPUTE=1 forces the recomputation.
+    double *node_all = nullptr;
+    if (h_grad) {
+        size_t free_b = 0, total_b = 0;
+        FSP_HIP(hipMemGetInfo(&free_b, &total_b));
+        const double need = 8.0 * (double)K * nodes_per * (double)n;
+        const char *env = std::getenv("FSP_ADJOINT_RECOMPUTE");
+        const bool force = env && std::atoi(env) != 0;
+        if (!force && need <= 0.5 * (double)free_b) {
+            node_all = W.alloc((int64_t)K * nodes_per * n);
+            if (!node_all) return fail(FSP_ERR_NOMEM, "likelihood node vectors");
+        }
+    }
     // forward pass: p at the observation times (checkpoints), LL
     FSP_HIP(hipMemcpyAsync(p, d_p0, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
     ckpt[0] = W.alloc(n);
@@ -322,6 +347,14 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
     FSP_TRY(fsp_expv_set_limits(run.E, 1000, 1000000));
     double ll = 0.0;
     for (int k = 0; k < K; ++k) {
+        if (node_all) {
+            for (int q = 0; q < nodes_per; ++q) {
+                FSP_TRY(fsp_expv_set_final_time(run.E, sn[k][q]));
+                FSP_TRY(fsp_expv_solve(run.E, p));
+                FSP_HIP(hipMemcpyAsync(node_all + ((int64_t)k * nodes_per + q) * n, p,
+                                       sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+            }
+        }
         FSP_TRY(fsp_expv_set_final_time(run.E, data->times[k]));
         FSP_TRY(fsp_expv_solve(run.E, p));
         if (h_grad) {
@@ -353,11 +386,12 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
     // and |lambda| reaches N/P ~ 1e6 or more, so lambda is integrated as
     // sig * lam with |lam|_2 = 1 (tol relative to |lambda|).
     double sig = 1.0;
-    const int nodes_per = 2 * o.grading_levels * o.quad_nodes;
     std::vector<double *> node_p((size_t)nodes_per, nullptr);
-    for (int q = 0; q < nodes_per; ++q) {
-        node_p[q] = W.alloc(n);
-        if (!node_p[q]) return fail(FSP_ERR_NOMEM, "likelihood node vectors");
+    if (!node_all) {
+        for (int q = 0; q < nodes_per; ++q) {
+            node_p[q] = W.alloc(n);
+            if (!node_p[q]) return fail(FSP_ERR_NOMEM, "likelihood node vectors");
+        }
     }
     const DevModel *dm = M.d_model;
     for (int k = K - 1; k >= 0; --k) {
@@ -380,23 +414,26 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
         }
         if (!std::isfinite(sig)) return fail(FSP_ERR_NOT_CONVERGED, "adjoint vector is not finite");
         hipLaunchKernelGGL(k_scale, dim3(Gg), dim3(kBlock), 0, s, 1.0 / sig, lam, n);
-        std::vector<double> sn, wn;
-        graded_rule(a, b, o.quad_nodes, o.grading_levels, sn, wn);
-        // p at the nodes from the checkpoint at a
-        FSP_HIP(hipMemcpyAsync(tmp, ckpt[k], sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
-        FSP_TRY(fsp_expv_restart(run.E, sn[0] - a));
-        for (int q = 0; q < nodes_per; ++q) {
-            FSP_TRY(fsp_expv_set_final_time(run.E, sn[q] - a));
-            FSP_TRY(fsp_expv_solve(run.E, tmp));
-            FSP_HIP(hipMemcpyAsync(node_p[q], tmp, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+        const std::vector<double> &snk = sn[k], &wnk = wn[k];
+        if (node_all) {
+            for (int q = 0; q < nodes_per; ++q) node_p[q] = node_all + ((int64_t)k * nodes_per + q) * n;
+        } else {
+            // p at the nodes from the checkpoint at a
+            FSP_HIP(hipMemcpyAsync(tmp, ckpt[k], sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+            FSP_TRY(fsp_expv_restart(run.E, snk[0] - a));
+            for (int q = 0; q < nodes_per; ++q) {
+                FSP_TRY(fsp_expv_set_final_time(run.E, snk[q] - a));
+                FSP_TRY(fsp_expv_solve(run.E, tmp));
+                FSP_HIP(hipMemcpyAsync(node_p[q], tmp, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+            }
         }
         // lambda backwards through the nodes: tau = b - s
-        FSP_TRY(fsp_expv_restart(run.Eb, b - sn[nodes_per - 1]));
+        FSP_TRY(fsp_expv_restart(run.Eb, b - snk[nodes_per - 1]));
         for (int q = nodes_per - 1; q >= 0; --q) {
-            FSP_TRY(fsp_expv_set_final_time(run.Eb, b - sn[q]));
+            FSP_TRY(fsp_expv_set_final_time(run.Eb, b - snk[q]));
             FSP_TRY(fsp_expv_solve(run.Eb, lam));
             FSP_SWITCH_D(d, launch_grad_terms<D_>(M, node_p[q], lam, Gg, gpart, s));
-            hipLaunchKernelGGL(k_grad_accum, dim3(R), dim3(kBlock), 0, s, gpart, Gg, wn[q] * sig,
+            hipLaunchKernelGGL(k_grad_accum, dim3(R), dim3(kBlock), 0, s, gpart, Gg, wnk[q] * sig,
                                d_grad);
             FSP_HIP(hipGetLastError());
         }

@@ -134,6 +134,20 @@ def test_gradient_with_more_than_16_reactions():
     assert np.max(np.abs(g2[R:] - 0.5 * g)) <= 1e-8 * np.max(np.abs(g))
 
 
+def test_gradient_recomputed_nodes_agree(monkeypatch):
+    # node vectors kept from the forward pass (the default when they fit) and
+    # recomputed per interval from checkpoints give the same gradient
+    net = M.gene_expression((2, 10, 10))
+    times = [1.0, 2.5, 4.0]
+    counts = _data(net, times, 200, seed=9)
+    print("[case] gradient, stored vs recomputed quadrature nodes, gene 3x11x11", flush=True)
+    ll, g = capi.loglik(net, np.log(net.coefs), times, KEEP, counts, tol=1e-12, grad=True)
+    monkeypatch.setenv("FSP_ADJOINT_RECOMPUTE", "1")
+    ll2, g2 = capi.loglik(net, np.log(net.coefs), times, KEEP, counts, tol=1e-12, grad=True)
+    assert abs(ll2 - ll) <= 1e-10 * abs(ll)
+    assert np.max(np.abs(g2 - g)) <= 1e-8 * np.max(np.abs(g))
+
+
 def test_torch_autograd_bridge():
     import torch
 
