@@ -18,8 +18,9 @@
 // observation interval (the integrand has transients of width ~1/|A| there).
 // Checkpointing: the forward pass keeps p at the observation times and,
 // when all of them fit in half the free device memory, at every quadrature
-// node; otherwise the backward pass recomputes p at one interval's nodes
-// from the checkpoint at the interval's start.
+// node (dense output from inside the accepted Krylov steps, for lambda as
+// well); otherwise the backward pass recomputes p at one interval's nodes
+// from the checkpoint at the interval's start, with steps clipped at them.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -346,17 +347,16 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
     FSP_TRY(fsp_expv_create(run.A, nullptr, nullptr, n, data->times[0], o.m, o.tol, 0, 2, 1.0, s, &run.E));
     FSP_TRY(fsp_expv_set_limits(run.E, 1000, 1000000));
     double ll = 0.0;
+    std::vector<double *> outs((size_t)nodes_per, nullptr);
     for (int k = 0; k < K; ++k) {
-        if (node_all) {
-            for (int q = 0; q < nodes_per; ++q) {
-                FSP_TRY(fsp_expv_set_final_time(run.E, sn[k][q]));
-                FSP_TRY(fsp_expv_solve(run.E, p));
-                FSP_HIP(hipMemcpyAsync(node_all + ((int64_t)k * nodes_per + q) * n, p,
-                                       sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
-            }
-        }
         FSP_TRY(fsp_expv_set_final_time(run.E, data->times[k]));
-        FSP_TRY(fsp_expv_solve(run.E, p));
+        if (node_all) {
+            // p at the interval's nodes from inside the accepted steps
+            for (int q = 0; q < nodes_per; ++q) outs[q] = node_all + ((int64_t)k * nodes_per + q) * n;
+            FSP_TRY(fsp_expv_solve_dense(run.E, p, nodes_per, sn[k].data(), outs.data()));
+        } else {
+            FSP_TRY(fsp_expv_solve(run.E, p));
+        }
         if (h_grad) {
             ckpt[k + 1] = W.alloc(n);
             if (!ckpt[k + 1]) return fail(FSP_ERR_NOMEM, "likelihood checkpoints");
@@ -386,12 +386,13 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
     // and |lambda| reaches N/P ~ 1e6 or more, so lambda is integrated as
     // sig * lam with |lam|_2 = 1 (tol relative to |lambda|).
     double sig = 1.0;
-    std::vector<double *> node_p((size_t)nodes_per, nullptr);
-    if (!node_all) {
-        for (int q = 0; q < nodes_per; ++q) {
-            node_p[q] = W.alloc(n);
-            if (!node_p[q]) return fail(FSP_ERR_NOMEM, "likelihood node vectors");
-        }
+    std::vector<double *> node_p((size_t)nodes_per, nullptr), node_lam((size_t)nodes_per, nullptr);
+    std::vector<double> taus((size_t)nodes_per);
+    for (int q = 0; q < nodes_per; ++q) {
+        // stored nodes: lambda at every node of one interval; else p there
+        double *v = W.alloc(n);
+        if (!v) return fail(FSP_ERR_NOMEM, "likelihood node vectors");
+        (node_all ? node_lam : node_p)[q] = v;
     }
     const DevModel *dm = M.d_model;
     for (int k = K - 1; k >= 0; --k) {
@@ -426,6 +427,23 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
                 FSP_TRY(fsp_expv_solve(run.E, tmp));
                 FSP_HIP(hipMemcpyAsync(node_p[q], tmp, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
             }
+        }
+        if (node_all) {
+            // lambda backwards over the whole interval (tau = b - s), with
+            // its values at the nodes from inside the accepted steps
+            for (int q = 0; q < nodes_per; ++q) {
+                taus[q] = b - snk[nodes_per - 1 - q];
+                outs[q] = node_lam[nodes_per - 1 - q];
+            }
+            FSP_TRY(fsp_expv_restart(run.Eb, b - a));
+            FSP_TRY(fsp_expv_solve_dense(run.Eb, lam, nodes_per, taus.data(), outs.data()));
+            for (int q = nodes_per - 1; q >= 0; --q) {
+                FSP_SWITCH_D(d, launch_grad_terms<D_>(M, node_p[q], node_lam[q], Gg, gpart, s));
+                hipLaunchKernelGGL(k_grad_accum, dim3(R), dim3(kBlock), 0, s, gpart, Gg,
+                                   wnk[q] * sig, d_grad);
+                FSP_HIP(hipGetLastError());
+            }
+            continue;
         }
         // lambda backwards through the nodes: tau = b - s
         FSP_TRY(fsp_expv_restart(run.Eb, b - snk[nodes_per - 1]));

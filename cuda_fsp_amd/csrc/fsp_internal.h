@@ -194,6 +194,12 @@ fsp_status vec_sum(const double *d_x, int64_t n, double *h_sum, hipStream_t s);
 // exported from the shared object.
 extern "C" __attribute__((visibility("hidden"))) fsp_status fsp_mat_internal(fsp_mat A,
                                                                           fsp::MatImpl **impl);
+// Krylov solve to the final time that also writes v at the output times
+// t_out[0..n_out) (increasing, within (t_now, t_final]) from inside the
+// accepted steps; fused matrix operators only.  Not exported.
+extern "C" __attribute__((visibility("hidden"))) fsp_status fsp_expv_solve_dense(
+    fsp_expv e, double *d_v, int n_out, const double *t_out, double *const *d_out);
+
 // Transposed generator A^T of a model (HYB, F64, no sink), for solves with
 // A^T such as the adjoint pass of the likelihood gradient; not exported.
 extern "C" __attribute__((visibility("hidden"))) fsp_status fsp_mat_create_transposed(

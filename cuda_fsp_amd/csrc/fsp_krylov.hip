@@ -65,6 +65,7 @@ struct ExpvImpl {
     bool use_graph = true;
     hipStream_t cap = nullptr;   // private stream used only for graph capture
     double *tau_log = nullptr;
+    double *dcoef = nullptr;     // dense-output coefficients (m + 2)
 };
 
 namespace {
@@ -526,6 +527,47 @@ __global__ void __launch_bounds__(kBlock) k_solution(const KState *st, const dou
     }
 }
 
+// Dense output inside the last accepted step (t_old, t_old + tau]: c = the
+// first column of exp(delta H) (delta <= tau) over the step's own mx basis
+// vectors, the step's Pade in place of exp(tau H).
+__global__ void __launch_bounds__(kBlock) k_dense_coef(const KState *st, int m, double delta,
+                                                      double *coef)
+{
+    extern __shared__ double smem[];
+    const int M2 = m + 2, NN = M2 * M2;
+    double *A = smem, *X = A + NN, *T = X + NN, *Nm = T + NN, *Dm = Nm + NN;
+    __shared__ double red[kBlock / kWave];
+    for (int e = threadIdx.x; e < NN; e += blockDim.x) A[e] = delta * st->H[e];
+    __syncthreads();
+    expm_block(M2, A, X, T, Nm, Dm, red);
+    const int mx = st->mx;
+    for (int k = threadIdx.x; k < M2; k += blockDim.x) coef[k] = k < mx ? Nm[k] : 0.0;
+}
+
+// out = beta V[:, 0:mx] c (the solution formula with the dense coefficients).
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_dense_out(const KState *st, const double *__restrict__ V64,
+                                                     const B *__restrict__ Vb, uint32_t nv,
+                                                     const double *__restrict__ coef,
+                                                     double *__restrict__ out)
+{
+    const int mx = st->mx;
+    const double beta = st->beta;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nv; i += gridDim.x * kBlock) {
+        double acc = 0.0;
+        int k = 0;
+        for (; k + kDotChunk <= mx; k += kDotChunk) {
+            double b[kDotChunk];
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q) b[q] = basis<B>(V64, Vb, nv, k + q, i);
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q) acc += b[q] * coef[k + q];
+        }
+        for (; k < mx; ++k) acc += basis<B>(V64, Vb, nv, k, i) * coef[k];
+        out[i] = beta * acc;
+    }
+}
+
 // ---- callback operator path: explicit scale / zero / dots kernels ----
 template <typename B>
 __global__ void __launch_bounds__(kBlock) k_cb_scale(ArnArgs a)
@@ -603,6 +645,7 @@ static fsp_status expv_alloc(ExpvImpl &E)
         h.p10[k - kP10Lo] = std::strtod(buf, nullptr);
     }
     FSP_HIP(hipMalloc(&E.tau_log, sizeof(double) * (size_t)kTauLogCap));
+    FSP_HIP(hipMalloc(&E.dcoef, sizeof(double) * (size_t)(E.m + 2)));
     h.tau_log = E.tau_log;
     h.tau_cap = kTauLogCap;
     FSP_HIP(copy_sync(E.d_st, &h, sizeof(KState), hipMemcpyHostToDevice, E.s));
@@ -614,7 +657,7 @@ static void expv_free(ExpvImpl &E)
     if (E.gexec) hipGraphExecDestroy(E.gexec);
     if (E.graph) hipGraphDestroy(E.graph);
     if (E.cap) (void)hipStreamDestroy(E.cap);
-    void *ptrs[] = {E.d_st, E.V64, E.V32, E.W[0], E.W[1], E.partials, E.tau_log};
+    void *ptrs[] = {E.d_st, E.V64, E.V32, E.W[0], E.W[1], E.partials, E.tau_log, E.dcoef};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (E.h_st) hipHostFree(E.h_st);
@@ -875,6 +918,50 @@ extern "C" fsp_status fsp_expv_solve(fsp_expv e, double *d_v)
         if (E.h_st->i_step > E.max_nstep)
             return fail(FSP_ERR_NOT_CONVERGED, "Krylov step count exceeded max_nstep");
     }
+    return FSP_OK;
+}
+
+// Solve to the final time and write v(t_out[q]) to d_out[q] for output
+// times t_now < t_out[0] < ... <= t_final, from the steps the plain solve
+// takes (one step per launch, so that each step's basis and Hessenberg are
+// still in place for the outputs inside it).  Internal (adjoint).
+__attribute__((visibility("hidden"))) fsp_status fsp_expv_solve_dense(fsp_expv e, double *d_v,
+                                                                     int n_out,
+                                                                     const double *t_out,
+                                                                     double *const *d_out)
+{
+    using namespace fsp;
+    if (!e || !d_v || (n_out > 0 && (!t_out || !d_out))) return fail(FSP_ERR_INVALID, "NULL argument");
+    ExpvImpl &E = e->impl;
+    if (!E.A) return fail(FSP_ERR_UNSUPPORTED, "dense output needs a matrix operator");
+    FSP_TRY(pull_state(E));
+    for (int q = 0; q < n_out; ++q)
+        if (!(t_out[q] > E.h_st->t_now) || t_out[q] > E.h_st->t_final || (q && !(t_out[q] > t_out[q - 1])))
+            return fail(FSP_ERR_INVALID, "output times must increase within (t_now, t_final]");
+    const uint32_t nv = (uint32_t)E.n;
+    const size_t lds = sizeof(double) * 5 * (size_t)(E.m + 2) * (E.m + 2);
+    int q = 0;
+    while (E.h_st->t_now < E.h_st->t_final) {
+        const double t0 = E.h_st->t_now;
+        FSP_TRY(launch_step(E, d_v));
+        FSP_TRY(pull_state(E));
+        if (E.h_st->i_step > E.max_nstep)
+            return fail(FSP_ERR_NOT_CONVERGED, "Krylov step count exceeded max_nstep");
+        const double t1 = E.h_st->t_now;
+        for (; q < n_out && t_out[q] <= t1; ++q) {
+            if (t_out[q] == t1) {
+                FSP_HIP(hipMemcpyAsync(d_out[q], d_v, sizeof(double) * nv, hipMemcpyDeviceToDevice, E.s));
+                continue;
+            }
+            k_dense_coef<<<1, kBlock, lds, E.s>>>(E.d_st, E.m, t_out[q] - t0, E.dcoef);
+            if (E.mixed)
+                k_dense_out<float><<<E.G, kBlock, 0, E.s>>>(E.d_st, E.V64, E.V32, nv, E.dcoef, d_out[q]);
+            else
+                k_dense_out<double><<<E.G, kBlock, 0, E.s>>>(E.d_st, E.V64, E.V64, nv, E.dcoef, d_out[q]);
+            FSP_HIP(hipGetLastError());
+        }
+    }
+    FSP_HIP(hipStreamSynchronize(E.s));
     return FSP_OK;
 }
 
