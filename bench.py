@@ -439,6 +439,8 @@ def main():
                     help="only the adjoint-gradient section")
     ap.add_argument("--chain-only", action="store_true",
                     help="only the 16.7M-state mixed-vs-FP64 solve")
+    ap.add_argument("--bdf-only", action="store_true",
+                    help="only the time-varying BDF solve")
     args = ap.parse_args()
     if args.kernels_only:
         kernels_only(5)
@@ -451,6 +453,9 @@ def main():
         return
     if args.chain_only:
         print(json.dumps(bench_chain_mixed()), flush=True)
+        return
+    if args.bdf_only:
+        print(json.dumps(bench_bdf(M.gene_expression())), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path "
