@@ -95,6 +95,7 @@ constexpr int kRedLen = 2 * kBlock / kWave;
 constexpr int kArnSlots = 2 * kMaxKrylov + 4;
 constexpr int kArnRedLen = 2 * kArnSlots * (kBlock / kWave);
 constexpr int kDotChunk = 8;
+constexpr int kTileGroup = 3;  // tiles whose dot loads are issued together
 
 template <typename B>
 __device__ inline bool arn_skip(const ArnArgs &a)
@@ -178,17 +179,29 @@ __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red
             dd acc[kDotChunk];
 #pragma unroll
             for (int q = 0; q < kDotChunk; ++q) acc[q] = dd_zero();
-            for (TileWalk t = tile_walk(a.tiles); t.t < t.end; t.t += t.step) {
-                const uint32_t i = t.t * kBlock + threadIdx.x;
-                if (i >= a.n) continue;
-                const double y = a.out[i];
-                double vk[kDotChunk];
+            // kTileGroup of the workgroup's tiles at a time: all their loads
+            // are issued before the sums, which still run in tile order
+            for (TileWalk t = tile_walk(a.tiles); t.t < t.end; t.t += kTileGroup * t.step) {
+                double y[kTileGroup], vk[kTileGroup][kDotChunk];
+                bool ok[kTileGroup];
 #pragma unroll
-                for (int q = 0; q < kDotChunk; ++q)
-                    vk[q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + c0 + q, a.j), i);
+                for (int g = 0; g < kTileGroup; ++g) {
+                    const uint32_t tt = t.t + (uint32_t)g * t.step;
+                    const uint32_t i = tt * kBlock + threadIdx.x;
+                    ok[g] = tt < t.end && i < a.n;
+                    const uint32_t r = ok[g] ? i : 0u;
+                    y[g] = a.out[r];
 #pragma unroll
-                for (int q = 0; q < kDotChunk; ++q)
-                    if (c0 + q < nk) acc[q] = dd_add_prod(acc[q], y, vk[q]);
+                    for (int q = 0; q < kDotChunk; ++q)
+                        vk[g][q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + c0 + q, a.j), r);
+                }
+#pragma unroll
+                for (int g = 0; g < kTileGroup; ++g) {
+                    if (!ok[g]) continue;
+#pragma unroll
+                    for (int q = 0; q < kDotChunk; ++q)
+                        if (c0 + q < nk) acc[q] = dd_add_prod(acc[q], y[g], vk[g][q]);
+                }
             }
 #pragma unroll
             for (int q = 0; q < kDotChunk; ++q)
@@ -199,19 +212,29 @@ __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red
             double acc[kDotChunk], accg[kDotChunk];
 #pragma unroll
             for (int q = 0; q < kDotChunk; ++q) acc[q] = accg[q] = 0.0;
-            for (TileWalk t = tile_walk(a.tiles); t.t < t.end; t.t += t.step) {
-                const uint32_t i = t.t * kBlock + threadIdx.x;
-                if (i >= a.n) continue;
-                const double y = a.out[i];
-                const double xj = basis<B>(a.V64, (const B *)a.Vb, a.nv, a.j, i);
-                double vk[kDotChunk];
+            for (TileWalk t = tile_walk(a.tiles); t.t < t.end; t.t += kTileGroup * t.step) {
+                double y[kTileGroup], xj[kTileGroup], vk[kTileGroup][kDotChunk];
+                bool ok[kTileGroup];
 #pragma unroll
-                for (int q = 0; q < kDotChunk; ++q)
-                    vk[q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + c0 + q, a.j), i);
+                for (int g = 0; g < kTileGroup; ++g) {
+                    const uint32_t tt = t.t + (uint32_t)g * t.step;
+                    const uint32_t i = tt * kBlock + threadIdx.x;
+                    ok[g] = tt < t.end && i < a.n;
+                    const uint32_t r = ok[g] ? i : 0u;
+                    y[g] = a.out[r];
+                    xj[g] = basis<B>(a.V64, (const B *)a.Vb, a.nv, a.j, r);
 #pragma unroll
-                for (int q = 0; q < kDotChunk; ++q) {
-                    acc[q] = fma(y, vk[q], acc[q]);
-                    accg[q] = fma(xj, vk[q], accg[q]);
+                    for (int q = 0; q < kDotChunk; ++q)
+                        vk[g][q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + c0 + q, a.j), r);
+                }
+#pragma unroll
+                for (int g = 0; g < kTileGroup; ++g) {
+                    if (!ok[g]) continue;
+#pragma unroll
+                    for (int q = 0; q < kDotChunk; ++q) {
+                        acc[q] = fma(y[g], vk[g][q], acc[q]);
+                        accg[q] = fma(xj[g], vk[g][q], accg[q]);
+                    }
                 }
             }
 #pragma unroll
