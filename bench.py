@@ -459,8 +459,8 @@ def main():
         print(json.dumps(bench_bdf(M.gene_expression())), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this engine has no multi-GPU path "
-              "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this engine has no multi-GPU path; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if capi.device_count() < 1:
         print("no GPU visible", file=sys.stderr)

@@ -1,7 +1,7 @@
 // cme_util.h -- drop-in for the reference's src/cme_util.h (namespace cuFSP):
 // shared types, state-space index maps and a device vector.
 //
-// Differences a caller sees (details in INTEGRATION.md):
+// Differences a caller sees:
 //  * PropFun is a HOST function (double(int*, int)); the reference fetched a
 //    __device__ pointer with cudaMemcpyFromSymbol.  Propensities written
 //    `__device__ __host__` in the reference tests work unchanged.
