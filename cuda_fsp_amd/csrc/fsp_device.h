@@ -116,8 +116,23 @@ __device__ inline dd block_sum_dd(dd v, double *red)
 // Sum of `count` dd partials stored as (s, c) pairs, fixed order.
 __device__ inline dd sum_dd_partials(const double *__restrict__ p, int count, double *red)
 {
+    // each thread merges its partials b = tid, tid + blockDim, ... in that
+    // order; four are loaded ahead of their merges
+    constexpr int kAhead = 4;
     dd s = dd_zero();
-    for (int b = threadIdx.x; b < count; b += blockDim.x) s = dd_merge(s, dd{p[2 * b], p[2 * b + 1]});
+    int b = threadIdx.x;
+    const int step = blockDim.x;
+    for (; b + (kAhead - 1) * step < count; b += kAhead * step) {
+        double v[2 * kAhead];
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            v[2 * u] = p[2 * (b + u * step)];
+            v[2 * u + 1] = p[2 * (b + u * step) + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) s = dd_merge(s, dd{v[2 * u], v[2 * u + 1]});
+    }
+    for (; b < count; b += step) s = dd_merge(s, dd{p[2 * b], p[2 * b + 1]});
     return block_sum_dd(s, red);
 }
 
