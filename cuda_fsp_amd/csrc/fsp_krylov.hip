@@ -324,7 +324,11 @@ __global__ void __launch_bounds__(kBlock) k_arn_csr(CsrF<V> A, ArnArgs a)
     arn_finish<B>(a, nrm, sink, red);
 }
 
-template <typename V, typename B>
+// NU > 0: the slot count as a compile-time constant; the rows of up to
+// kTileGroup tiles are formed together (all their loads issued before their
+// sums), then stored and their sink terms added in tile order.  NU = 0: any
+// slot count, one tile at a time.
+template <int NU, typename V, typename B>
 __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
 {
     if (arn_skip<B>(a)) return;
@@ -335,10 +339,38 @@ __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
     dd nrm = dd_zero();
     const KryX<B> X{a.in, sig, a.j};
     dd sink = dd_zero();
-    for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
-        const uint32_t i = w.t * kBlock + threadIdx.x;
-        const double y = hyb_row<V>(A, i, X, sink);
-        arn_epi<B>(a, i, y, i < a.n ? X(i) : 0.0, nrm);
+    if (NU > 0) {
+        // three tiles at a time up to 6 slots; more slots would cost
+        // occupancy (registers), so one tile at a time there
+        constexpr int TG = NU <= 6 ? kTileGroup : 1;
+        HybF<V> A0 = A;
+        A0.sink_out = nullptr;  // sink terms below, in row order
+        for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += TG * w.step) {
+            double y[TG];
+#pragma unroll
+            for (int g = 0; g < TG; ++g) {
+                const uint32_t tt = w.t + (uint32_t)g * w.step;
+                const uint32_t i = tt * kBlock + threadIdx.x;
+                const uint32_t r = (tt < w.end && i < A.n) ? i : 0u;
+                dd unused = dd_zero();
+                y[g] = hyb_row_n<(NU > 0 ? NU : 1)>(A0, r, X, unused);
+            }
+#pragma unroll
+            for (int g = 0; g < TG; ++g) {
+                const uint32_t tt = w.t + (uint32_t)g * w.step;
+                if (tt >= w.end) break;
+                const uint32_t i = tt * kBlock + threadIdx.x;
+                const bool in = i < A.n;
+                if (A.sink_out && in) sink = dd_add_prod(sink, A.sink_out[i], X(i));
+                arn_epi<B>(a, i, in ? y[g] : 0.0, i < a.n ? X(i) : 0.0, nrm);
+            }
+        }
+    } else {
+        for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
+            const uint32_t i = w.t * kBlock + threadIdx.x;
+            const double y = hyb_row<V>(A, i, X, sink);
+            arn_epi<B>(a, i, y, i < a.n ? X(i) : 0.0, nrm);
+        }
     }
     arn_finish<B>(a, nrm, sink, red);
 }
@@ -686,6 +718,23 @@ static void expv_free(ExpvImpl &E)
     if (E.h_st) hipHostFree(E.h_st);
 }
 
+template <typename V, typename B>
+static void launch_arn_hyb(const HybF<V> &A, const ArnArgs &a, int G, size_t red_b, hipStream_t s)
+{
+#define FSP_ARN_HYB_CASE(K)                                          \
+    case K:                                                          \
+        k_arn_hyb<K, V, B><<<G, kBlock, red_b, s>>>(A, a);           \
+        return;
+    switch (A.U) {
+        FSP_ARN_HYB_CASE(1) FSP_ARN_HYB_CASE(2) FSP_ARN_HYB_CASE(3) FSP_ARN_HYB_CASE(4)
+        FSP_ARN_HYB_CASE(5) FSP_ARN_HYB_CASE(6) FSP_ARN_HYB_CASE(7) FSP_ARN_HYB_CASE(8)
+        FSP_ARN_HYB_CASE(9) FSP_ARN_HYB_CASE(10) FSP_ARN_HYB_CASE(11) FSP_ARN_HYB_CASE(12)
+    default:
+        k_arn_hyb<0, V, B><<<G, kBlock, red_b, s>>>(A, a);
+    }
+#undef FSP_ARN_HYB_CASE
+}
+
 template <typename B>
 static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
 {
@@ -702,9 +751,9 @@ static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
         }
     } else if (M.format == FSP_HYB) {
         if (mixed)
-            k_arn_hyb<float, B><<<E.G, kBlock, red_b, E.s>>>(hyb_view<float>(M), a);
+            launch_arn_hyb<float, B>(hyb_view<float>(M), a, E.G, red_b, E.s);
         else
-            k_arn_hyb<double, B><<<E.G, kBlock, red_b, E.s>>>(hyb_view<double>(M), a);
+            launch_arn_hyb<double, B>(hyb_view<double>(M), a, E.G, red_b, E.s);
     } else {
         if (mixed && M.hm.has_factors) {
             KronF<float> A = kron_view<float>(M, -1, 0.0);
