@@ -207,16 +207,44 @@ __global__ void __launch_bounds__(kBlock) k_ls_step_begin(LsArgs a, long long ma
 // instance's model), partial dots V_k . w_{j+1} (k = istart..j) and the sink
 // inflow; mode 1: |A V_m|^2 partials.
 template <typename B>
-__device__ inline double ls_row(const LsInst &I, uint32_t r, const KryX<B> &X, dd &sink)
+__device__ inline HybF<B> ls_hyb(const LsInst &I, bool with_sink)
 {
     HybF<B> A;
     A.diag = I.hdiag;
     A.slot_vals = (const B *)I.hvals;
     A.slot_off = I.hoff;
-    A.sink_out = I.with_sink ? I.hsink : nullptr;
+    A.sink_out = (with_sink && I.with_sink) ? I.hsink : nullptr;
     A.n = (uint32_t)I.n;
     A.U = I.U;
-    return hyb_row<B>(A, r, X, sink);
+    return A;
+}
+
+template <typename B>
+__device__ inline double ls_row(const LsInst &I, uint32_t r, const KryX<B> &X, dd &sink,
+                                bool with_sink = true)
+{
+    return hyb_row<B>(ls_hyb<B>(I, with_sink), r, X, sink);
+}
+
+// Row sums of the thread's kLsRpt rows of a tile (r0 + u kBlock) for a
+// compile-time slot count, RG rows at a time (the loads of a group issued
+// before their sums; 2 with an FP64 basis, 4 with FP32, sized to keep the
+// kernel's occupancy); rows past the instance read row 0 and are
+// discarded.  No sink terms (the caller adds them in row order).
+template <int NU, typename B>
+__device__ inline void ls_rows(const LsInst &I, int r0, const KryX<B> &X, double (&y)[kLsRpt])
+{
+    constexpr int RG = std::is_same<B, double>::value ? 2 : 4;
+    const HybF<B> A = ls_hyb<B>(I, false);
+#pragma unroll 1
+    for (int u0 = 0; u0 < kLsRpt; u0 += RG) {
+#pragma unroll
+        for (int u = u0; u < u0 + RG; ++u) {
+            const uint32_t r = (uint32_t)(r0 + u * kBlock);
+            dd unused = dd_zero();
+            y[u] = hyb_row_n<NU>(A, r < (uint32_t)I.n ? r : 0u, X, unused);
+        }
+    }
 }
 
 template <typename B>
@@ -238,17 +266,31 @@ __global__ void __launch_bounds__(kBlock) k_ls_arn(LsArgs a)
         const int nk = a.nk;
         dd sink = dd_zero(), nrm = dd_zero();
         double y[kLsRpt];
+        switch (I.U) {  // the instance's slot count, uniform over the tile
+        case 4: ls_rows<4, B>(I, r0, X, y); break;
+        case 5: ls_rows<5, B>(I, r0, X, y); break;
+        case 6: ls_rows<6, B>(I, r0, X, y); break;
+        default:
+#pragma unroll
+            for (int u = 0; u < kLsRpt; ++u) {
+                const uint32_t r = (uint32_t)(r0 + u * kBlock);
+                dd unused = dd_zero();
+                y[u] = r < (uint32_t)I.n ? ls_row<B>(I, r, X, unused, false) : 0.0;
+            }
+        }
+        // stores and sink terms in row order
 #pragma unroll
         for (int u = 0; u < kLsRpt; ++u) {
             const uint32_t r = (uint32_t)(r0 + u * kBlock);
-            y[u] = 0.0;
             if (r < (uint32_t)I.n) {
-                y[u] = ls_row<B>(I, r, X, sink);
+                if (I.with_sink) sink = dd_add_prod(sink, I.hsink[r], X(r));
                 store_basis<B>(I.V64, Vb, (size_t)I.nv, a.j, r, X(r));
                 if (a.mode == 0)
                     out[r] = y[u];
                 else
                     nrm = dd_add_prod(nrm, y[u], y[u]);
+            } else {
+                y[u] = 0.0;
             }
         }
         constexpr bool GRAM = gram_basis<B>();
