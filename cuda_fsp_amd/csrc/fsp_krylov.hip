@@ -16,10 +16,10 @@
 // the reference does in FP64, lets the loss of orthogonality compound by
 // |w| / h(j+1, j) per step until the basis is degenerate.  The mixed path
 // therefore projects with the Gram matrix G = V^T V of the stored basis:
-// k_arn_* also returns the dots V_k . V_j (column j of G), k_gram_coef turns
-// the Hessenberg column h into c = 2h - G h (the second Gram-Schmidt pass
-// h2 = V^T (w - V h) = h - G h folded into the first), and k_arn_update
-// subtracts V c in one sweep over the basis.
+// k_arn_* also returns the dots V_k . V_j (column j of G); every workgroup
+// of k_arn_update turns the Hessenberg column h into c = 2h - G h (the
+// second Gram-Schmidt pass h2 = V^T (w - V h) = h - G h folded into the
+// first) and subtracts V c in one sweep over the basis.
 // then k_expm_control (one workgroup: Pade expm of tau*H, error estimate,
 // reject loop, step-size update) and k_solution (v = beta V F0).
 // Reductions are deterministic: per-wave butterflies, per-block partials in
@@ -445,6 +445,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_dots(ArnArgs a, int with_sink, i
             const double s = dd_value(sd);
             st->H[kk + a.j * M2] = s;
             if (gram_basis<B>()) {
+                st->hs[kk] = s;
                 const double g = dd_value(sg);
                 st->G[kk + a.j * kGramLd] = g;
                 st->G[a.j + kk * kGramLd] = g;
@@ -465,14 +466,6 @@ __global__ void __launch_bounds__(kBlock) k_arn_dots(ArnArgs a, int with_sink, i
     }
 }
 
-// FP32 basis: H(., j) <- 2h - G h (gram_coefs), one workgroup.
-__global__ void __launch_bounds__(kBlock) k_gram_coef(ArnArgs a, int M2)
-{
-    KState *st = a.st;
-    if (!st->active || st->broke) return;
-    HIP_DYNAMIC_SHARED(double, lds)
-    gram_coefs(st, a.istart, a.j, M2, lds);
-}
 
 // w_{j+1} -= V[istart..j] H(istart..j, j); partial |w|^2.
 template <typename B>
@@ -480,8 +473,20 @@ __global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2)
 {
     if (arn_skip<B>(a)) return;
     __shared__ double red[kRedLen];
-    const KState *st = a.st;
+    KState *st = a.st;
     const double *h = st->H + (size_t)a.j * M2;
+    if (gram_basis<B>()) {
+        // FP32 basis: every workgroup forms the Gram-corrected coefficients
+        // 2h - G h from the raw dots in LDS; workgroup 0 also stores them as
+        // the Hessenberg column (no workgroup reads that column here)
+        HIP_DYNAMIC_SHARED(double, lds)
+        double *cw = lds + kGramLdsBytes / sizeof(double);
+        gram_coefs_lds(st, a.istart, a.j, lds, cw);
+        if (blockIdx.x == 0)
+            for (int k = threadIdx.x; k <= a.j - a.istart; k += blockDim.x)
+                st->H[(a.istart + k) + (size_t)a.j * M2] = cw[k];
+        h = cw - a.istart;
+    }
     dd nrm = dd_zero();
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += gridDim.x * kBlock) {
         // k ascending as one sum; the basis loads of 8 terms are issued together
@@ -822,8 +827,9 @@ static fsp_status enqueue_step(ExpvImpl &E, double *d_v)
         }
         FSP_HIP(hipGetLastError());
         if (last) break;
-        if (E.mixed) k_gram_coef<<<1, kBlock, kGramLdsBytes, E.s>>>(a, M2);
-        k_arn_update<B><<<E.G, kBlock, 0, E.s>>>(a, M2);
+        // with an FP32 basis the update forms the Gram-corrected coefficients
+        const size_t upd_lds = E.mixed ? kGramLdsBytes + sizeof(double) * (kMaxKrylov + 2) : 0;
+        k_arn_update<B><<<E.G, kBlock, upd_lds, E.s>>>(a, M2);
         // a fused operator's next row kernel finishes |w| itself
         if (!E.A) k_arn_norm<<<1, kBlock, 0, E.s>>>(a, M2, E.G);
         FSP_HIP(hipGetLastError());

@@ -25,6 +25,9 @@ struct KState {
     // FP32 basis: Gram matrix V_k . V_l of the stored basis, leading
     // dimension kMaxKrylov + 1
     double G[(kMaxKrylov + 1) * (kMaxKrylov + 1)];
+    // FP32 basis: the raw dots V^T w of the current column, read by the
+    // update kernel while its workgroup 0 replaces H(., j) by 2h - G h
+    double hs[kMaxKrylov + 2];
     double p10[kP10Hi - kP10Lo + 1];   // correctly rounded 1e-300 .. 1e300
     double *tau_log;                   // accepted step sizes, by step index
     long long tau_cap;
@@ -61,6 +64,25 @@ __device__ inline void gram_coefs(KState *st, int istart, int j, int M2, double 
 }
 
 constexpr size_t kGramLdsBytes = sizeof(double) * ((kMaxKrylov + 1) * (kMaxKrylov + 1) + kMaxKrylov + 2);
+
+// The same coefficients from the raw dots st->hs into LDS (cw[0..w)), for
+// every workgroup of a kernel; nothing is written to global memory.
+// lds: kGramLdsBytes.
+__device__ inline void gram_coefs_lds(const KState *st, int istart, int j, double *lds, double *cw)
+{
+    const int w = j - istart + 1;
+    double *gw = lds, *hw = lds + (kMaxKrylov + 1) * (kMaxKrylov + 1);
+    for (int e = threadIdx.x; e < w * w; e += blockDim.x)
+        gw[e] = st->G[(istart + e % w) + (istart + e / w) * kGramLd];
+    for (int e = threadIdx.x; e < w; e += blockDim.x) hw[e] = st->hs[istart + e];
+    __syncthreads();
+    for (int k = threadIdx.x; k < w; k += blockDim.x) {
+        double s = 0.0;
+        for (int l = 0; l < w; ++l) s += gw[k + l * w] * hw[l];
+        cw[k] = 2.0 * hw[k] - s;
+    }
+    __syncthreads();
+}
 
 // Krylov vector j = in * sigma_j as the basis stores it: rounded to B for
 // j >= 1; V_0 (the scaled solution vector) is kept in FP64 with any B, so
