@@ -682,8 +682,9 @@ static fsp_status expv_alloc(ExpvImpl &E)
     FSP_HIP(hipMalloc(&E.W[1], sizeof(double) * nv));
     E.tiles = ((int64_t)nv + kBlock - 1) / kBlock;
     // row-kernel grid: 4 workgroups per CU measured fastest on the 786K-state
-    // solve (1.70 s vs 1.93 s at 2048 and 2.07 s at 512); FSP_ARN_GRID
-    // overrides the cap for timing sweeps
+    // solve (1.34 s against 1.43 s at 768, 1.46 s at 1280 and 1.77 s at
+    // 2048; the finalizers and the norm prologue sum one partial per
+    // workgroup); FSP_ARN_GRID overrides the cap for timing sweeps
     int cap = 1024;
     if (const char *g = std::getenv("FSP_ARN_GRID")) cap = std::max(8, std::atoi(g));
     E.G = grid_for(E.tiles, cap);
