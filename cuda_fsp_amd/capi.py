@@ -52,6 +52,13 @@ class FspExpvStats(C.Structure):
                 ("n_reject", C.c_int64), ("n_breakdown", C.c_int64)]
 
 
+PROF_CLASSES = ("prologue", "arnoldi", "dots", "update", "control", "solution")
+
+
+class FspExpvProfile(C.Structure):
+    _fields_ = [("launches", C.c_int64 * 6), ("ms", C.c_double * 6), ("bytes", C.c_double * 6)]
+
+
 class FspBatchOptions(C.Structure):
     _fields_ = [("m", C.c_int), ("tol", C.c_double), ("anorm", C.c_double),
                 ("precision", C.c_int), ("iop", C.c_int), ("q_iop", C.c_int),
@@ -128,6 +135,8 @@ _SIGS = {
     "fsp_expv_set_limits": (C.c_int, [_vp, _i, _i64]),
     "fsp_expv_get_tau_history": (C.c_int, [_vp, _vp, _i64, _vp]),
     "fsp_expv_set_iop": (C.c_int, [_vp, _i, _i]),
+    "fsp_expv_set_profiling": (C.c_int, [_vp, _i]),
+    "fsp_expv_get_profile": (C.c_int, [_vp, _vp]),
     "fsp_batch_create": (C.c_int, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "fsp_batch_destroy": (C.c_int, [_vp]),
     "fsp_batch_set_initial": (C.c_int, [_vp, _i, _vp]),
@@ -566,6 +575,16 @@ class Expv:
             check(lib().fsp_expv_get_tau_history(self.h, out.ctypes.data, cnt.value,
                                                  C.byref(cnt)))
         return out
+
+    def set_profiling(self, enable=True):
+        check(lib().fsp_expv_set_profiling(self.h, 1 if enable else 0))
+
+    def profile(self) -> dict:
+        """{class: (launches, total ms, algorithmic bytes)} of the profiled steps."""
+        p = FspExpvProfile()
+        check(lib().fsp_expv_get_profile(self.h, C.byref(p)))
+        return {name: (int(p.launches[k]), float(p.ms[k]), float(p.bytes[k]))
+                for k, name in enumerate(PROF_CLASSES)}
 
     def set_limits(self, max_reject=1000, max_nstep=10000):
         check(lib().fsp_expv_set_limits(self.h, int(max_reject), int(max_nstep)))

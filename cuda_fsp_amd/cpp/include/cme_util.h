@@ -20,7 +20,13 @@
 
 namespace cuFSP {
 
-// src/cme_util.h:25 (device pointer there; host function here)
+// Species per model the engine supports.
+constexpr int kMaxSpeciesFacade = 8;
+
+// src/cme_util.h:25 (device pointer there; host function here).  Called
+// from several host threads at once while the box is tabulated, so it must
+// be a pure function of its arguments, as the reference's device propensity
+// (evaluated by every GPU thread) is.
 typedef double (*PropFun)(int *x, int reaction);
 // src/fspmat_kron_kernels.h:13
 typedef double (*PropFactorFun)(int state, int species, int reaction);

@@ -190,25 +190,38 @@ FSPMat::FSPMat(int n_reactions, int n_species, int *fsp_dim, CSRMatInt stoich, T
     : ns_(n_species), nr_(n_reactions), tcoeffunc_(t_func)
 {
     if (!prop_func) throw std::runtime_error("FSPMat: propensity function is NULL");
+    if (n_species < 1 || n_species > kMaxSpeciesFacade)
+        throw std::runtime_error("FSPMat: n_species must be in 1..8");
+    if (n_reactions < 1) throw std::runtime_error("FSPMat: n_reactions must be >= 1");
     const int n = rect_fsp_num_states(n_species, fsp_dim);
     // Tabulate a_r(x) over the box on the host (the reference calls its
     // device PropFun inside the assembly kernel, src/fspmat_csr_kernels.cu:79)
-    std::vector<double> table((size_t)n_reactions * (size_t)n);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i) {
-        int x[16];
-        indx2state(i, x, n_species, fsp_dim);
-        for (int r = 0; r < n_reactions; ++r) {
-            int xc[16];
-            std::memcpy(xc, x, sizeof(int) * (size_t)n_species);
-            table[(size_t)r * n + i] = prop_func(xc, r);
-        }
-    }
+    // into the device table [R][n], a chunk of states at a time, so the host
+    // holds R * chunk values, not R * n.  prop_func is called from several
+    // OpenMP threads at once: like the reference's device propensity (called
+    // by every GPU thread), it must be a pure function of its arguments.
     void *p = nullptr;
-    throw_on(fsp_malloc(&p, table.size() * sizeof(double)), "FSPMat::FSPMat");
+    throw_on(fsp_malloc(&p, (size_t)n_reactions * (size_t)n * sizeof(double)), "FSPMat::FSPMat");
     table_dev_ = (double *)p;
-    throw_on(fsp_memcpy_h2d(table_dev_, table.data(), table.size() * sizeof(double)),
-             "FSPMat::FSPMat");
+    constexpr int kChunk = 1 << 20;
+    std::vector<double> buf((size_t)n_reactions * (size_t)std::min(n, kChunk));
+    for (int c0 = 0; c0 < n; c0 += kChunk) {
+        const int cn = std::min(kChunk, n - c0);
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < cn; ++k) {
+            int x[kMaxSpeciesFacade];
+            indx2state(c0 + k, x, n_species, fsp_dim);
+            for (int r = 0; r < n_reactions; ++r) {
+                int xc[kMaxSpeciesFacade];
+                std::memcpy(xc, x, sizeof(int) * (size_t)n_species);
+                buf[(size_t)r * cn + k] = prop_func(xc, r);
+            }
+        }
+        for (int r = 0; r < n_reactions; ++r)
+            throw_on(fsp_memcpy_h2d(table_dev_ + (size_t)r * n + c0, buf.data() + (size_t)r * cn,
+                                    (size_t)cn * sizeof(double)),
+                     "FSPMat::FSPMat");
+    }
     init(n_reactions, n_species, fsp_dim, stoich, nullptr, table_dev_, format, precision,
          with_sink);
 }

@@ -240,7 +240,10 @@ struct Work {
     double *alloc(int64_t n)
     {
         double *p = nullptr;
-        if (hipMalloc(&p, sizeof(double) * (size_t)n) != hipSuccess) return nullptr;
+        if (hipMalloc(&p, sizeof(double) * (size_t)n) != hipSuccess) {
+            (void)hipGetLastError();  // clear the error, so later launch checks do not see it
+            return nullptr;
+        }
         bufs.push_back(p);
         return p;
     }
@@ -265,9 +268,11 @@ struct LikRun {
     }
 };
 
+// node_budget: device bytes the stored node vectors may take (< 0: half the
+// free device memory at the time of the call).
 fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const double *d_p0,
                        const fsp_lik_data *data, const fsp_grad_options *opt, double *h_ll,
-                       double *h_grad, hipStream_t s)
+                       double *h_grad, hipStream_t s, double node_budget = -1.0)
 {
     if (!model || !log_coefs || !d_p0 || !data || !h_ll) return fail(FSP_ERR_INVALID, "NULL argument");
     if (data->n_times < 1 || !data->times || !data->keep_mask || !data->counts)
@@ -324,20 +329,24 @@ fsp_status loglik_impl(const fsp_model *model, const double *log_coefs, const do
         graded_rule(k ? data->times[k - 1] : 0.0, data->times[k], o.quad_nodes, o.grading_levels,
                     sn[k], wn[k]);
     // With the gradient, p at every node is kept from the forward pass when
-    // all K * nodes_per vectors fit in half the free device memory; else
-    // the backward pass recomputes one interval's nodes from the checkpoint
-    // at its start.  FSP_ADJOINT_RECOMPUTE=1 forces the recomputation.
+    // all K * nodes_per vectors fit in the node budget (default: half the
+    // free device memory) and their allocation succeeds; else the backward
+    // pass recomputes one interval's nodes from the checkpoint at its start
+    // (FSP_ADJOINT_RECOMPUTE=1 forces that).  The two paths evaluate the
+    // node values differently (dense output inside the accepted steps, or
+    // steps clipped at the nodes), so their gradients agree to the Krylov
+    // tolerance, not bit for bit.
     double *node_all = nullptr;
     if (h_grad) {
-        size_t free_b = 0, total_b = 0;
-        FSP_HIP(hipMemGetInfo(&free_b, &total_b));
+        if (node_budget < 0.0) {
+            size_t free_b = 0, total_b = 0;
+            FSP_HIP(hipMemGetInfo(&free_b, &total_b));
+            node_budget = 0.5 * (double)free_b;
+        }
         const double need = 8.0 * (double)K * nodes_per * (double)n;
         const char *env = std::getenv("FSP_ADJOINT_RECOMPUTE");
         const bool force = env && std::atoi(env) != 0;
-        if (!force && need <= 0.5 * (double)free_b) {
-            node_all = W.alloc((int64_t)K * nodes_per * n);
-            if (!node_all) return fail(FSP_ERR_NOMEM, "likelihood node vectors");
-        }
+        if (!force && need <= node_budget) node_all = W.alloc((int64_t)K * nodes_per * n);
     }
     // forward pass: p at the observation times (checkpoints), LL
     FSP_HIP(hipMemcpyAsync(p, d_p0, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
@@ -477,6 +486,7 @@ struct GradBatch {
     const fsp_lik_data *data;
     const fsp_grad_options *opt;
     double *h_ll, *h_grad;
+    double node_budget = -1.0;  // device bytes per point for stored node vectors
     std::atomic<int> next{0};
     std::atomic<int> failed{0};
 };
@@ -496,7 +506,8 @@ void grad_worker(GradBatch *B, fsp_status *st, std::string *err)
         const int i = B->next.fetch_add(1);
         if (i >= B->n_points) break;
         const fsp_status r = loglik_impl(B->model, B->log_coefs + (size_t)i * B->R, B->d_p0, B->data,
-                                         B->opt, B->h_ll + i, B->h_grad + (size_t)i * B->R, s);
+                                         B->opt, B->h_ll + i, B->h_grad + (size_t)i * B->R, s,
+                                         B->node_budget);
         if (r != FSP_OK) {
             *st = r;
             *err = "point " + std::to_string(i) + ": " + fsp_last_error();
@@ -529,6 +540,16 @@ fsp_status loglik_grad_batch(const fsp_model *model, int n_points, const double 
     B.opt = opt;
     B.h_ll = h_ll;
     B.h_grad = h_grad;
+    // Free device memory is measured once, before any worker allocates:
+    // each worker's stored node vectors may take free / (2 workers), so the
+    // choice between stored and recomputed nodes (and with it the bits of
+    // every point's result) does not depend on how the workers interleave.
+    {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+            return fail(FSP_ERR_HIP, "batched gradient: hipMemGetInfo");
+        B.node_budget = (double)free_b / (2.0 * workers);
+    }
     std::vector<fsp_status> st((size_t)workers, FSP_OK);
     std::vector<std::string> err((size_t)workers);
     std::vector<std::thread> th;

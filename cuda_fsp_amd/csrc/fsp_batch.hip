@@ -215,6 +215,13 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
         delete h;
         return st;
     };
+    // a HIP error after the handle exists frees it and its device buffers
+#define FSP_BATCH_HIP(call)                                                          \
+    do {                                                                             \
+        hipError_t e__ = (call);                                                     \
+        if (e__ != hipSuccess)                                                       \
+            return bail(fail(FSP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e__))); \
+    } while (0)
     B.count = count;
     if (opt) {
         B.m = opt->m > 0 ? opt->m : 30;
@@ -236,8 +243,8 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return bail(fail(FSP_ERR_NO_DEVICE, "no HIP device visible"));
     int dev = 0;
-    FSP_HIP(hipGetDevice(&dev));
-    FSP_HIP(hipDeviceGetAttribute(&B.cus, hipDeviceAttributeMultiprocessorCount, dev));
+    FSP_BATCH_HIP(hipGetDevice(&dev));
+    FSP_BATCH_HIP(hipDeviceGetAttribute(&B.cus, hipDeviceAttributeMultiprocessorCount, dev));
     // models
     B.hms.resize((size_t)count);
     std::vector<double> tabs, cvec;
@@ -296,20 +303,20 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
     // LDS factor tables when they fit next to the Pade workspace
     const size_t base = batch_lds_bytes(B, 1024);
     B.tab_lds = (base + sizeof(double) * max_tab <= 150 * 1024) ? max_tab : 0;
-    FSP_HIP(hipMalloc(&B.d_models, sizeof(DevModel) * (size_t)count));
+    FSP_BATCH_HIP(hipMalloc(&B.d_models, sizeof(DevModel) * (size_t)count));
     for (int i = 0; i < count; ++i)
-        FSP_HIP(copy_sync(B.d_models + i, &B.hms[i].dm, sizeof(DevModel), hipMemcpyHostToDevice, nullptr));
-    FSP_HIP(hipMalloc(&B.d_tabs, sizeof(double) * std::max<size_t>(tabs.size(), 1)));
+        FSP_BATCH_HIP(copy_sync(B.d_models + i, &B.hms[i].dm, sizeof(DevModel), hipMemcpyHostToDevice, nullptr));
+    FSP_BATCH_HIP(hipMalloc(&B.d_tabs, sizeof(double) * std::max<size_t>(tabs.size(), 1)));
     if (!tabs.empty())
-        FSP_HIP(copy_sync(B.d_tabs, tabs.data(), sizeof(double) * tabs.size(), hipMemcpyHostToDevice, nullptr));
-    FSP_HIP(hipMalloc(&B.d_coefs, sizeof(double) * cvec.size()));
-    FSP_HIP(copy_sync(B.d_coefs, cvec.data(), sizeof(double) * cvec.size(), hipMemcpyHostToDevice, nullptr));
-    FSP_HIP(hipMalloc(&B.d_vecs, sizeof(double) * (size_t)vec_total));
-    FSP_HIP(fill_sync(B.d_vecs, 0, sizeof(double) * (size_t)vec_total, nullptr));
+        FSP_BATCH_HIP(copy_sync(B.d_tabs, tabs.data(), sizeof(double) * tabs.size(), hipMemcpyHostToDevice, nullptr));
+    FSP_BATCH_HIP(hipMalloc(&B.d_coefs, sizeof(double) * cvec.size()));
+    FSP_BATCH_HIP(copy_sync(B.d_coefs, cvec.data(), sizeof(double) * cvec.size(), hipMemcpyHostToDevice, nullptr));
+    FSP_BATCH_HIP(hipMalloc(&B.d_vecs, sizeof(double) * (size_t)vec_total));
+    FSP_BATCH_HIP(fill_sync(B.d_vecs, 0, sizeof(double) * (size_t)vec_total, nullptr));
     // default initial vectors: delta at state 0
     for (int i = 0; i < count; ++i) {
         const double one = 1.0;
-        FSP_HIP(copy_sync(B.d_vecs + B.vec_off[i] + B.h_inst[i].nv, &one, sizeof(double),
+        FSP_BATCH_HIP(copy_sync(B.d_vecs + B.vec_off[i] + B.h_inst[i].nv, &one, sizeof(double),
                           hipMemcpyHostToDevice, nullptr));
     }
     for (int i = 0; i < count; ++i) {
@@ -320,25 +327,25 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
         I.p = B.d_vecs + B.vec_off[i];
         I.p0 = I.p + I.nv;
     }
-    FSP_HIP(hipMalloc(&B.d_inst, sizeof(BatchInst) * (size_t)count));
-    FSP_HIP(copy_sync(B.d_inst, B.h_inst.data(), sizeof(BatchInst) * (size_t)count,
+    FSP_BATCH_HIP(hipMalloc(&B.d_inst, sizeof(BatchInst) * (size_t)count));
+    FSP_BATCH_HIP(copy_sync(B.d_inst, B.h_inst.data(), sizeof(BatchInst) * (size_t)count,
                       hipMemcpyHostToDevice, nullptr));
     std::vector<int> order((size_t)count);
     for (int i = 0; i < count; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
     B.order = order;
     B.cost = cost;
-    FSP_HIP(hipMalloc(&B.d_order, sizeof(int) * (size_t)count));
-    FSP_HIP(copy_sync(B.d_order, order.data(), sizeof(int) * (size_t)count, hipMemcpyHostToDevice, nullptr));
-    FSP_HIP(hipMalloc(&B.d_queue, sizeof(unsigned int)));
+    FSP_BATCH_HIP(hipMalloc(&B.d_order, sizeof(int) * (size_t)count));
+    FSP_BATCH_HIP(copy_sync(B.d_order, order.data(), sizeof(int) * (size_t)count, hipMemcpyHostToDevice, nullptr));
+    FSP_BATCH_HIP(hipMalloc(&B.d_queue, sizeof(unsigned int)));
     std::vector<double> p10(kP10Hi - kP10Lo + 1);
     for (int k = kP10Lo; k <= kP10Hi; ++k) {
         char buf[16];
         std::snprintf(buf, sizeof buf, "1e%d", k);
         p10[k - kP10Lo] = std::strtod(buf, nullptr);
     }
-    FSP_HIP(hipMalloc(&B.d_p10, sizeof(double) * p10.size()));
-    FSP_HIP(copy_sync(B.d_p10, p10.data(), sizeof(double) * p10.size(), hipMemcpyHostToDevice, nullptr));
+    FSP_BATCH_HIP(hipMalloc(&B.d_p10, sizeof(double) * p10.size()));
+    FSP_BATCH_HIP(copy_sync(B.d_p10, p10.data(), sizeof(double) * p10.size(), hipMemcpyHostToDevice, nullptr));
     // workspace for the largest grid any configuration may use (2 per CU);
     // none where the persistent kernel is not launched
     bool persistent = false;
@@ -361,6 +368,7 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
     (void)stream;
     *out = h;
     return FSP_OK;
+#undef FSP_BATCH_HIP
 }
 
 extern "C" fsp_status fsp_batch_destroy(fsp_batch b)

@@ -317,7 +317,7 @@ fsp_status fsp_states_enumerate(int n_species, const int *bounds, int *d_states,
     FSP_SWITCH_D(n_species, (k_enumerate<D_><<<blocks, kBlock, 0, s>>>(dm, n, d_states)));
     FSP_HIP(hipGetLastError());
     FSP_HIP(hipStreamSynchronize(s));
-    hipFree(dm);
+    (void)hipFree(dm);
     return FSP_OK;
 }
 
@@ -337,7 +337,7 @@ fsp_status fsp_states_to_index(int n_species, const int *bounds, const int *d_st
                                                                            d_index);
     FSP_HIP(hipGetLastError());
     FSP_HIP(hipStreamSynchronize(s));
-    hipFree(dm);
+    (void)hipFree(dm);
     return FSP_OK;
 }
 

@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "fsp_krylov_dev.h"
 
@@ -66,6 +67,17 @@ struct ExpvImpl {
     hipStream_t cap = nullptr;   // private stream used only for graph capture
     double *tau_log = nullptr;
     double *dcoef = nullptr;     // dense-output coefficients (m + 2)
+    // per-kernel timing (fsp_expv_set_profiling): while enabled, steps run
+    // without the captured graph, one per host check, and each launch is
+    // bracketed by HIP events on the handle's stream
+    bool profile = false;
+    std::vector<hipEvent_t> pev;      // event pool, two per launch of a step
+    std::vector<int> pcls;            // kernel class of each launch
+    std::vector<double> pbytes;       // algorithmic bytes of each launch
+    int pcount = 0;                   // launches recorded in the current step
+    int64_t prof_n[FSP_PROF_CLASSES] = {};
+    double prof_ms[FSP_PROF_CLASSES] = {};
+    double prof_bytes[FSP_PROF_CLASSES] = {};
 };
 
 namespace {
@@ -666,6 +678,89 @@ __global__ void __launch_bounds__(kBlock) k_cb_dots(ArnArgs a)
 }  // namespace
 
 // ======================================================================= host
+// ------------------------------------------------------------ profiling
+// Event pair k brackets the k-th launch of the step being enqueued.
+static fsp_status prof_mark(ExpvImpl &E, int cls, double bytes)
+{
+    if (!E.profile) return FSP_OK;
+    const size_t k = (size_t)E.pcount;
+    while (E.pev.size() < 2 * (k + 1)) {
+        hipEvent_t ev = nullptr;
+        FSP_HIP(hipEventCreate(&ev));
+        E.pev.push_back(ev);
+    }
+    if (E.pcls.size() < k + 1) {
+        E.pcls.resize(k + 1);
+        E.pbytes.resize(k + 1);
+    }
+    E.pcls[k] = cls;
+    E.pbytes[k] = bytes;
+    FSP_HIP(hipEventRecord(E.pev[2 * k], E.s));
+    return FSP_OK;
+}
+
+static fsp_status prof_done(ExpvImpl &E)
+{
+    if (!E.profile) return FSP_OK;
+    FSP_HIP(hipEventRecord(E.pev[2 * (size_t)E.pcount + 1], E.s));
+    ++E.pcount;
+    return FSP_OK;
+}
+
+// Adds the recorded launches of the last step to the totals.  Launches of
+// an inactive step (the final time reached, or a failed solve) return at
+// once and are not counted; after a happy breakdown the remaining Arnoldi
+// launches do nothing and are not counted either.
+static fsp_status prof_collect(ExpvImpl &E, bool active, int mb)
+{
+    if (!E.profile || E.pcount == 0) return FSP_OK;
+    FSP_HIP(hipEventSynchronize(E.pev[2 * (size_t)E.pcount - 1]));
+    int arn = 0;
+    for (int k = 0; k < E.pcount; ++k) {
+        const int c = E.pcls[k];
+        bool count = active;
+        if (c == FSP_PROF_ARNOLDI || c == FSP_PROF_DOTS || c == FSP_PROF_UPDATE) {
+            // launch j of its class; the avnorm launch (j = m) runs unless
+            // the step broke down
+            const int j = c == FSP_PROF_ARNOLDI ? arn++ : (c == FSP_PROF_DOTS ? arn - 1 : arn - 1);
+            count = count && (mb == E.m ? true : j < mb);
+        }
+        if (!count) continue;
+        float ms = 0.0f;
+        FSP_HIP(hipEventElapsedTime(&ms, E.pev[2 * (size_t)k], E.pev[2 * (size_t)k + 1]));
+        E.prof_n[c] += 1;
+        E.prof_ms[c] += ms;
+        E.prof_bytes[c] += E.pbytes[k];
+    }
+    E.pcount = 0;
+    return FSP_OK;
+}
+
+// Algorithmic bytes of the step's kernels: every vector element and stored
+// matrix value the operation needs once (cached re-reads not counted).
+// Basis vector k has elem_bytes(k) bytes per entry.
+static double basis_elem(const ExpvImpl &E, int k) { return (!E.mixed || k == 0) ? 8.0 : 4.0; }
+
+static double arn_bytes(const ExpvImpl &E, int j, int istart, int mode)
+{
+    const double n = (double)E.n;
+    const double apply = E.A ? mat_bytes_per_apply(*E.A) : 16.0 * n;  // matrix + x + y
+    double b = apply + basis_elem(E, j) * n;                            // + V_j store
+    if (mode == 0) {
+        for (int k = istart; k <= j; ++k) b += basis_elem(E, k) * n;    // dots V_k . w
+    } else {
+        b -= 8.0 * n;                                                   // norm only, no w store
+    }
+    return b;
+}
+
+static double update_bytes(const ExpvImpl &E, int j, int istart)
+{
+    double b = 16.0 * (double)E.n;                                      // w read and written
+    for (int k = istart; k <= j; ++k) b += basis_elem(E, k) * (double)E.n;
+    return b;
+}
+
 static fsp_status expv_alloc(ExpvImpl &E)
 {
     const size_t nv = (size_t)E.n;
@@ -720,8 +815,9 @@ static void expv_free(ExpvImpl &E)
     if (E.cap) (void)hipStreamDestroy(E.cap);
     void *ptrs[] = {E.d_st, E.V64, E.V32, E.W[0], E.W[1], E.partials, E.tau_log, E.dcoef};
     for (void *p : ptrs)
-        if (p) hipFree(p);
-    if (E.h_st) hipHostFree(E.h_st);
+        if (p) (void)hipFree(p);
+    if (E.h_st) (void)hipHostFree(E.h_st);
+    for (hipEvent_t ev : E.pev) (void)hipEventDestroy(ev);
 }
 
 template <typename V, typename B>
@@ -793,8 +889,10 @@ static fsp_status enqueue_step(ExpvImpl &E, double *d_v)
     const uint32_t nv = (uint32_t)E.n;
     const uint32_t nbox = E.A ? (uint32_t)E.A->n : nv;
     const int with_sink = (E.A && E.A->with_sink) ? 1 : 0;
+    FSP_TRY(prof_mark(E, FSP_PROF_PROLOGUE, 8.0 * nv));
     k_norm_partials<<<E.G, kBlock, 0, E.s>>>(d_v, nv, E.partials, E.d_st);
     k_step_begin<<<1, kBlock, 0, E.s>>>(E.d_st, E.partials, E.G, m, M2);
+    FSP_TRY(prof_done(E));
     ArnArgs a;
     a.st = E.d_st;
     a.V64 = E.V64;
@@ -817,8 +915,12 @@ static fsp_status enqueue_step(ExpvImpl &E, double *d_v)
         a.mode = last ? 1 : 0;
         const int nk = last ? 1 : j - istart + 1;
         if (E.A) {
+            FSP_TRY(prof_mark(E, FSP_PROF_ARNOLDI, arn_bytes(E, j, a.istart, a.mode)));
             FSP_TRY(launch_arn_spmv<B>(E, a));
+            FSP_TRY(prof_done(E));
+            FSP_TRY(prof_mark(E, FSP_PROF_DOTS, 16.0 * E.G * nk));
             k_arn_dots<B><<<nk, kBlock, 0, E.s>>>(a, with_sink, M2);
+            FSP_TRY(prof_done(E));
         } else {
             k_cb_scale<B><<<E.G, kBlock, 0, E.s>>>(a);
             FSP_HIP(hipGetLastError());
@@ -830,14 +932,24 @@ static fsp_status enqueue_step(ExpvImpl &E, double *d_v)
         if (last) break;
         // with an FP32 basis the update forms the Gram-corrected coefficients
         const size_t upd_lds = E.mixed ? kGramLdsBytes + sizeof(double) * (kMaxKrylov + 2) : 0;
+        FSP_TRY(prof_mark(E, FSP_PROF_UPDATE, update_bytes(E, j, istart)));
         k_arn_update<B><<<E.G, kBlock, upd_lds, E.s>>>(a, M2);
+        FSP_TRY(prof_done(E));
         // a fused operator's next row kernel finishes |w| itself
         if (!E.A) k_arn_norm<<<1, kBlock, 0, E.s>>>(a, M2, E.G);
         FSP_HIP(hipGetLastError());
     }
     const size_t lds = sizeof(double) * 5 * (size_t)M2 * M2;
+    FSP_TRY(prof_mark(E, FSP_PROF_CONTROL, 0.0));
     k_expm_control<<<1, kBlock, lds, E.s>>>(E.d_st, m, E.max_reject);
+    FSP_TRY(prof_done(E));
+    // v = beta V F0 over the basis vectors the step used (mx <= m + 1; the
+    // host counts m + 1)
+    double sol_bytes = 8.0 * nv;
+    for (int k = 0; k <= m; ++k) sol_bytes += basis_elem(E, k) * nv;
+    FSP_TRY(prof_mark(E, FSP_PROF_SOLUTION, sol_bytes));
     k_solution<B><<<E.G, kBlock, 0, E.s>>>(E.d_st, E.V64, (const B *)(E.mixed ? (void *)E.V32 : (void *)E.V64), nv, d_v);
+    FSP_TRY(prof_done(E));
     FSP_HIP(hipGetLastError());
     return FSP_OK;
 }
@@ -849,7 +961,7 @@ static fsp_status enqueue_any(ExpvImpl &E, double *d_v)
 
 static fsp_status launch_step(ExpvImpl &E, double *d_v)
 {
-    if (!E.A || !E.use_graph) return enqueue_any(E, d_v);
+    if (!E.A || !E.use_graph || E.profile) return enqueue_any(E, d_v);
     if (!E.gexec || E.graph_v != d_v) {
         if (E.gexec) hipGraphExecDestroy(E.gexec);
         if (E.graph) hipGraphDestroy(E.graph);
@@ -876,10 +988,20 @@ static fsp_status launch_step(ExpvImpl &E, double *d_v)
     return FSP_OK;
 }
 
-static fsp_status pull_state(ExpvImpl &E)
+// Device-to-host copy of the step state, whatever its status: the setters
+// and getters use it, so a handle whose last solve failed can still report
+// its statistics and be restarted.
+static fsp_status read_state(ExpvImpl &E)
 {
     FSP_HIP(hipMemcpyAsync(E.h_st, E.d_st, sizeof(KState), hipMemcpyDeviceToHost, E.s));
     FSP_HIP(hipStreamSynchronize(E.s));
+    return FSP_OK;
+}
+
+// read_state plus the status check of a step or solve.
+static fsp_status pull_state(ExpvImpl &E)
+{
+    FSP_TRY(read_state(E));
     if (E.h_st->status != FSP_OK)
         return fail((fsp_status)E.h_st->status, "Krylov reject loop exhausted max_reject");
     return FSP_OK;
@@ -970,8 +1092,11 @@ extern "C" fsp_status fsp_expv_step(fsp_expv e, double *d_v)
     using namespace fsp;
     if (!e || !d_v) return fail(FSP_ERR_INVALID, "NULL argument");
     ExpvImpl &E = e->impl;
+    const int64_t steps0 = E.h_st->i_step;
     FSP_TRY(launch_step(E, d_v));
-    return pull_state(E);
+    const fsp_status st = pull_state(E);
+    FSP_TRY(prof_collect(E, E.h_st->i_step > steps0, E.h_st->mb));
+    return st;
 }
 
 extern "C" fsp_status fsp_expv_solve(fsp_expv e, double *d_v)
@@ -984,7 +1109,7 @@ extern "C" fsp_status fsp_expv_solve(fsp_expv e, double *d_v)
     // up to 4, but no more than the remaining interval needs at the current
     // step size: a solve to a nearby output time then launches one step, not
     // three more that would only return.
-    const int ahead = E.A ? 4 : 1;
+    const int ahead = (E.A && !E.profile) ? 4 : 1;
     while (E.h_st->t_now < E.h_st->t_final) {
         const KState &h = *E.h_st;
         int launch = ahead;
@@ -992,8 +1117,11 @@ extern "C" fsp_status fsp_expv_solve(fsp_expv e, double *d_v)
             const double est = std::ceil((h.t_final - h.t_now) / h.t_new);
             if (std::isfinite(est)) launch = (int)std::max(1.0, std::min((double)ahead, est));
         }
+        const int64_t steps0 = h.i_step;
         for (int k = 0; k < launch; ++k) FSP_TRY(launch_step(E, d_v));
-        FSP_TRY(pull_state(E));
+        const fsp_status pst = pull_state(E);
+        FSP_TRY(prof_collect(E, E.h_st->i_step > steps0, E.h_st->mb));
+        FSP_TRY(pst);
         if (E.h_st->i_step > E.max_nstep)
             return fail(FSP_ERR_NOT_CONVERGED, "Krylov step count exceeded max_nstep");
     }
@@ -1048,7 +1176,7 @@ extern "C" fsp_status fsp_expv_reset_time(fsp_expv e)
 {
     using namespace fsp;
     if (!e) return fail(FSP_ERR_INVALID, "NULL argument");
-    FSP_TRY(pull_state(e->impl));
+    FSP_TRY(read_state(e->impl));
     e->impl.h_st->t_now = 0.0;
     return push_state(e->impl);
 }
@@ -1057,7 +1185,7 @@ extern "C" fsp_status fsp_expv_restart(fsp_expv e, double t_final)
 {
     using namespace fsp;
     if (!e) return fail(FSP_ERR_INVALID, "NULL argument");
-    FSP_TRY(pull_state(e->impl));
+    FSP_TRY(read_state(e->impl));
     KState &h = *e->impl.h_st;
     h.t_now = 0.0;
     h.t_final = t_final;
@@ -1073,7 +1201,7 @@ extern "C" fsp_status fsp_expv_set_final_time(fsp_expv e, double t_final)
 {
     using namespace fsp;
     if (!e) return fail(FSP_ERR_INVALID, "NULL argument");
-    FSP_TRY(pull_state(e->impl));
+    FSP_TRY(read_state(e->impl));
     e->impl.h_st->t_final = t_final;
     e->impl.h_st->k1 = 2;
     return push_state(e->impl);
@@ -1083,7 +1211,7 @@ extern "C" fsp_status fsp_expv_final_time_reached(fsp_expv e, int *reached)
 {
     using namespace fsp;
     if (!e || !reached) return fail(FSP_ERR_INVALID, "NULL argument");
-    FSP_TRY(pull_state(e->impl));
+    FSP_TRY(read_state(e->impl));
     *reached = e->impl.h_st->t_now >= e->impl.h_st->t_final;
     return FSP_OK;
 }
@@ -1092,7 +1220,7 @@ extern "C" fsp_status fsp_expv_get_stats(fsp_expv e, fsp_expv_stats *s)
 {
     using namespace fsp;
     if (!e || !s) return fail(FSP_ERR_INVALID, "NULL argument");
-    FSP_TRY(pull_state(e->impl));
+    FSP_TRY(read_state(e->impl));
     const KState &h = *e->impl.h_st;
     s->t_now = h.t_now;
     s->t_new = h.t_new;
@@ -1111,12 +1239,42 @@ extern "C" fsp_status fsp_expv_get_tau_history(fsp_expv e, double *h_taus, int64
 {
     using namespace fsp;
     if (!e || !count) return fail(FSP_ERR_INVALID, "NULL argument");
-    FSP_TRY(pull_state(e->impl));
+    FSP_TRY(read_state(e->impl));
     const int64_t n = std::min<int64_t>(e->impl.h_st->i_step, kTauLogCap);
     *count = n;
     if (h_taus && cap > 0 && n > 0)
         FSP_HIP(copy_sync(h_taus, e->impl.tau_log, sizeof(double) * (size_t)std::min(n, cap),
                           hipMemcpyDeviceToHost, e->impl.s));
+    return FSP_OK;
+}
+
+extern "C" fsp_status fsp_expv_set_profiling(fsp_expv e, int enable)
+{
+    using namespace fsp;
+    if (!e) return fail(FSP_ERR_INVALID, "NULL argument");
+    ExpvImpl &E = e->impl;
+    if (E.profile && !enable) FSP_HIP(hipStreamSynchronize(E.s));
+    E.profile = enable != 0 && E.A != nullptr;
+    E.pcount = 0;
+    for (int c = 0; c < FSP_PROF_CLASSES; ++c) {
+        E.prof_n[c] = 0;
+        E.prof_ms[c] = 0.0;
+        E.prof_bytes[c] = 0.0;
+    }
+    if (enable && !E.A) return fail(FSP_ERR_UNSUPPORTED, "profiling needs a matrix operator");
+    return FSP_OK;
+}
+
+extern "C" fsp_status fsp_expv_get_profile(fsp_expv e, fsp_expv_profile *p)
+{
+    using namespace fsp;
+    if (!e || !p) return fail(FSP_ERR_INVALID, "NULL argument");
+    const ExpvImpl &E = e->impl;
+    for (int c = 0; c < FSP_PROF_CLASSES; ++c) {
+        p->launches[c] = E.prof_n[c];
+        p->ms[c] = E.prof_ms[c];
+        p->bytes[c] = E.prof_bytes[c];
+    }
     return FSP_OK;
 }
 

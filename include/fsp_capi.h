@@ -201,6 +201,28 @@ fsp_status fsp_expv_set_limits(fsp_expv e, int max_reject, int64_t max_nstep);
 /* Public knobs IOP / q_iop (src/KryExpvFSP.h:64-65), read by the next step. */
 fsp_status fsp_expv_set_iop(fsp_expv e, int iop, int q_iop);
 
+/* Per-kernel timing of the fused Krylov step (no reference counterpart; the
+ * reference has no tracing).  While enabled, steps run without the captured
+ * HIP graph (one step per host check) and every kernel launch is bracketed
+ * by HIP events on the handle's stream; each launch of an active step adds
+ * its time and its algorithmic bytes (every stored matrix value and vector
+ * entry the operation needs once) to its class.  Enabling (or disabling)
+ * clears the totals.  Matrix operators only. */
+#define FSP_PROF_PROLOGUE 0  /* |v| partials and the step prologue           */
+#define FSP_PROF_ARNOLDI 1   /* row kernel: SpMV, V_j store, dots vs V_k      */
+#define FSP_PROF_DOTS 2      /* per-dot finalizers                            */
+#define FSP_PROF_UPDATE 3    /* w -= V h and |w|^2 partials                   */
+#define FSP_PROF_CONTROL 4   /* Pade exponential, error estimate, step size   */
+#define FSP_PROF_SOLUTION 5  /* v = beta V F0                                 */
+#define FSP_PROF_CLASSES 6
+typedef struct fsp_expv_profile {
+    int64_t launches[FSP_PROF_CLASSES];
+    double ms[FSP_PROF_CLASSES];
+    double bytes[FSP_PROF_CLASSES];
+} fsp_expv_profile;
+fsp_status fsp_expv_set_profiling(fsp_expv e, int enable);
+fsp_status fsp_expv_get_profile(fsp_expv e, fsp_expv_profile *profile);
+
 /* ---- batched solves (no reference counterpart: the reference solves one
  * system at a time, src/KryExpvFSP.cu:184-188) ---- */
 typedef struct fsp_batch_s *fsp_batch;
@@ -352,7 +374,12 @@ fsp_status fsp_loglik_grad(const fsp_model *model, const double *log_coefs,
 /* LL and gradient at n_points parameter points: log_coefs and h_grad are
  * n_points x n_reactions (row-major), h_ll has n_points entries.  Points
  * run on `workers` host threads with a stream each (<= 0: 4, the hardware
- * queues of a process); every point gives exactly fsp_loglik_grad's result.
+ * queues of a process).  The gradient keeps p at every quadrature node in
+ * device memory when that fits a budget (fsp_loglik_grad: half the free
+ * memory; here free / (2 workers), measured once before the workers
+ * start) and recomputes the nodes from checkpoints otherwise.  When a
+ * point and fsp_loglik_grad take the same path the results are bit for bit
+ * equal; across the two paths they agree to the Krylov tolerance.
  * The error names the first failed point.  The reference has no gradient;
  * its batched counterpart would be a loop over cme::Likelihood::evaluate
  * (src/Likelihood.cpp:8-33). */

@@ -71,6 +71,62 @@ static double dot_dd(int64_t n, const double *a, const double *b)
     return dd_value(tot);
 }
 
+/* out[k] = V_{k0+k} . w for k = 0..nk-1 (V column-major with leading
+ * dimension n): cublasDgemv with CUBLAS_OP_T (src/KryExpvFSP.cu:76-77) as
+ * one pass over the rows.  Every dot is an order-independent dd sum, as in
+ * dot_dd; each thread keeps 8 dots' accumulators at a time over its rows
+ * (independent chains), then the threads' partials merge in thread order. */
+#define ORC_DOT_GROUP 8
+#define ORC_MAX_DOTS 64
+static void dots_dd(int64_t n, const double *V, int k0, int nk, const double *w, double *out)
+{
+    int nt = g_threads;
+    if (nt > 256) nt = 256;
+    static ddv part[256][ORC_MAX_DOTS];
+    for (int t = 0; t < nt; ++t)
+        for (int k = 0; k < nk; ++k) { part[t][k].s = 0.0; part[t][k].c = 0.0; }
+#pragma omp parallel num_threads(nt)
+    {
+#ifdef _OPENMP
+        int t = omp_get_thread_num();
+#else
+        int t = 0;
+#endif
+        int64_t chunk = (n + nt - 1) / nt;
+        int64_t i0 = (int64_t)t * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
+        for (int g = 0; g < nk; g += ORC_DOT_GROUP) {
+            int gn = nk - g < ORC_DOT_GROUP ? nk - g : ORC_DOT_GROUP;
+            ddv acc[ORC_DOT_GROUP];
+            const double *col[ORC_DOT_GROUP];
+            for (int q = 0; q < ORC_DOT_GROUP; ++q) {
+                acc[q].s = 0.0; acc[q].c = 0.0;
+                col[q] = V + (int64_t)(k0 + g + (q < gn ? q : 0)) * n;
+            }
+            if (gn == ORC_DOT_GROUP) {
+                for (int64_t i = i0; i < i1; ++i) {
+                    const double wi = w[i];
+                    for (int q = 0; q < ORC_DOT_GROUP; ++q) acc[q] = dd_add_prod(acc[q], col[q][i], wi);
+                }
+            } else {
+                for (int64_t i = i0; i < i1; ++i) {
+                    const double wi = w[i];
+                    for (int q = 0; q < gn; ++q) acc[q] = dd_add_prod(acc[q], col[q][i], wi);
+                }
+            }
+            for (int q = 0; q < gn; ++q) part[t][g + q] = acc[q];
+        }
+    }
+    for (int k = 0; k < nk; ++k) {
+        ddv tot = {0.0, 0.0};
+        for (int t = 0; t < nt; ++t) tot = dd_merge(tot, part[t][k]);
+        out[k] = dd_value(tot);
+    }
+}
+
+static int g_verbose = 0;
+/* verbose != 0: the expv drivers print a line per output time to stderr */
+void orc_set_verbose(int verbose) { g_verbose = verbose; }
+
 void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 int orc_get_threads(void) { return g_threads; }
 
@@ -701,11 +757,13 @@ struct orc_expv {
     double *H, *F;   /* (m+2)^2 column-major */
     const double *replay;
     int replay_count, replay_pos;
+    double *tau_log;  /* accepted step sizes, optional */
+    int64_t tau_cap;
 };
 
 /* cublasDdot / cublasDnrm2 (src/KryExpvFSP.cu:41,76,86,117) as
  * order-independent sums */
-static double dot_n(int64_t n, const double *a, const double *b) { return dot_dd(n, a, b); }
+
 
 static double nrm2_n(int64_t n, const double *a) { return sqrt(dot_dd(n, a, a)); }
 
@@ -714,6 +772,7 @@ orc_expv *orc_expv_create(int64_t n, double t_final, orc_matvec_fn mv,
                           double anorm)
 {
     /* constructor: src/KryExpvFSP.cu:6-36; knobs src/KryExpvFSP.h:64-69 */
+    if (m < 1 || m + 1 > ORC_MAX_DOTS) return NULL;
     orc_expv *e = (orc_expv *)calloc(1, sizeof(orc_expv));
     e->n = n; e->mv = mv; e->ctx = ctx;
     e->t_final = t_final; e->t_now = 0.0; e->t_new = 0.0;
@@ -738,6 +797,11 @@ void orc_expv_set_max_reject(orc_expv *e, int mr) { e->max_reject = mr; }
 void orc_expv_set_replay(orc_expv *e, const double *taus, int count)
 {
     e->replay = taus; e->replay_count = count; e->replay_pos = 0;
+}
+
+void orc_expv_set_tau_log(orc_expv *e, double *taus, int64_t cap)
+{
+    e->tau_log = taus; e->tau_cap = cap;
 }
 
 void orc_expv_get_stats(const orc_expv *e, orc_expv_stats *s)
@@ -820,8 +884,11 @@ int orc_expv_step(orc_expv *e, double *v)
         e->mv(e->ctx, V + (int64_t)j * n, Vj1);
         e->n_matvec++;
         if (e->iop) istart = (j >= e->q_iop - 1) ? j - e->q_iop + 1 : 0;
-        for (int k = istart; k <= j; ++k)                     /* gemv T, :76-77 */
-            H[k + j * M2] += dot_n(n, V + (int64_t)k * n, Vj1);
+        {                                                     /* gemv T, :76-77 */
+            double hk[ORC_MAX_DOTS];
+            dots_dd(n, V, istart, j - istart + 1, Vj1, hk);
+            for (int k = istart; k <= j; ++k) H[k + j * M2] += hk[k - istart];
+        }
         /* gemv N, :79-81: Vj1 -= V[istart..j] * H(istart..j, j) */
 #pragma omp parallel for num_threads(g_threads) schedule(static)
         for (int64_t i = 0; i < n; ++i) {
@@ -887,6 +954,7 @@ int orc_expv_step(orc_expv *e, double *v)
     e->t_new = 0.9 * tau * pow(tau * e->tol / e->err_loc, e->xm);
     e->t_new = round_up_2sig(e->t_new);
     e->tau_last = tau;
+    if (e->tau_log && e->i_step < e->tau_cap) e->tau_log[e->i_step] = tau;
     e->i_step++;
     return rc;
 }
@@ -972,17 +1040,24 @@ static void op_mv(void *ctx, const double *x, double *y) { orc_op_apply((const o
 
 int orc_expv_op_outputs(const orc_op *op, double *v, const double *t_out, int n_out,
                         double *out, int m, double tol, int iop, int q_iop, double anorm,
-                        const double *replay, int n_replay, orc_expv_stats *stats)
+                        const double *replay, int n_replay, orc_expv_stats *stats,
+                        double *tau_log, int64_t tau_cap)
 {
     const int64_t n = op->n_rows;
     orc_expv *e = orc_expv_create(n, t_out[0], op_mv, (void *)op, m, tol, iop, q_iop, anorm);
     if (replay && n_replay > 0) orc_expv_set_replay(e, replay, n_replay);
+    if (tau_log && tau_cap > 0) orc_expv_set_tau_log(e, tau_log, tau_cap);
     int rc = 0;
     for (int k = 0; k < n_out; ++k) {
         orc_expv_set_final_time(e, t_out[k]);
         int r = orc_expv_solve(e, v);
         if (r) rc = r;
         memcpy(out + (int64_t)k * n, v, sizeof(double) * (size_t)n);
+        if (g_verbose) {
+            fprintf(stderr, "[oracle] t = %g reached after %lld Krylov steps\n", t_out[k],
+                    (long long)e->i_step);
+            fflush(stderr);
+        }
     }
     if (stats) orc_expv_get_stats(e, stats);
     orc_expv_destroy(e);

@@ -140,6 +140,11 @@ void orc_expv_set_max_reject(orc_expv *e, int max_reject);
 /* Replay mode: force the next step sizes (tau) instead of the adaptive
  * choice; used to separate arithmetic error from control-flow divergence. */
 void orc_expv_set_replay(orc_expv *e, const double *taus, int count);
+/* Records the accepted step sizes of the following steps into taus[i_step]
+ * (the first cap of them). */
+void orc_expv_set_tau_log(orc_expv *e, double *taus, int64_t cap);
+/* verbose != 0: orc_expv_op_outputs prints a line per output time to stderr. */
+void orc_set_verbose(int verbose);
 
 /* Convenience: run expv to t_final with a CSR operator (y += A x). */
 int orc_expv_csr(int64_t n, const int *row_ptrs, const int *cols,
@@ -174,7 +179,8 @@ typedef struct {
 void orc_op_apply(const orc_op *op, const double *x, double *y);
 int orc_expv_op_outputs(const orc_op *op, double *v, const double *t_out, int n_out,
                         double *out, int m, double tol, int iop, int q_iop, double anorm,
-                        const double *replay, int n_replay, orc_expv_stats *stats);
+                        const double *replay, int n_replay, orc_expv_stats *stats,
+                        double *tau_log, int64_t tau_cap);
 
 /* 2-significant-digit upward rounding of a step size (see fsp_oracle.c). */
 double orc_round_up_2sig(double t);

@@ -84,6 +84,11 @@ def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
 
 
+def set_verbose(on: bool) -> None:
+    """Progress lines (one per output time) from the expv drivers."""
+    lib().orc_set_verbose(1 if on else 0)
+
+
 def _p(a):
     return None if a is None else a.ctypes.data
 
@@ -285,19 +290,24 @@ class Operator:
 
 
 def expv_outputs(opr: Operator, v, t_out, m=30, tol=1e-8, iop=False, q_iop=2, anorm=1.0,
-                 replay=None):
+                 replay=None, taus=False, tau_cap=1 << 16):
     """KryExpvFSP restatement through output times with an Operator; returns
-    (rc, outputs, stats).  replay: step sizes to force (accepted as given)."""
+    (rc, outputs, stats), plus the accepted step sizes with taus=True.
+    replay: step sizes to force (accepted as given)."""
     L = lib()
     L.orc_expv_op_outputs.restype = C.c_int
     L.orc_expv_op_outputs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                       C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
-                                      C.c_void_p, C.c_int, C.c_void_p]
+                                      C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int64]
     t = np.ascontiguousarray(t_out, np.float64)
     out = np.empty((len(t), opr.n_rows), np.float64)
     st = ExpvStats()
     rep = None if replay is None else np.ascontiguousarray(replay, np.float64)
+    log = np.zeros(tau_cap if taus else 0, np.float64)
     rc = L.orc_expv_op_outputs(C.addressof(opr.op), _p(v), _p(t), len(t), _p(out), int(m),
                                float(tol), int(iop), int(q_iop), float(anorm), _p(rep),
-                               0 if rep is None else len(rep), C.addressof(st))
+                               0 if rep is None else len(rep), C.addressof(st),
+                               _p(log) if taus else None, len(log))
+    if taus:
+        return rc, out, st, log[:min(int(st.i_step), len(log))].copy()
     return rc, out, st

@@ -9,7 +9,9 @@ fixtures.npz:
               definition (column = source state, outflow leaving the box to
               a sink state n), p(t) = expm(t A) p0 at several times
   bdf_*       a time-varying generator A(t) = sum_r c_r(t) A_r on a tiny box,
-              integrated by scipy solve_ivp (DOP853, rtol 1e-12)
+              integrated by scipy solve_ivp (DOP853, rtol 1e-12): the
+              reference's time-varying toggle coefficients, and the
+              gene-expression network with a sin-modulated G2 -> G1 rate
 """
 import os
 import sys
@@ -96,6 +98,26 @@ def main():
                     t_eval=t_eval)
     out["bdf_toggle_t"] = t_eval
     out["bdf_toggle_p"] = sol.y.T
+    # the gene-expression network with the time-varying promoter rate
+    # G2 -> G1 = 0.5 (1 + sin(2 pi t / 30)), on a small box with the sink
+    # state, at several output times up to t = 60
+    net = M.gene_expression((2, 12, 12))
+    gterms = [dense_generator(net.with_coefs(np.eye(9)[r]), sink=True) for r in range(9)]
+
+    def gene_coefs(t):
+        c = net.coefs.copy()
+        c[3] = 0.5 * (1.0 + np.sin(2.0 * np.pi * t / 30.0))
+        return c
+
+    p0 = np.zeros(net.n_states + 1)
+    p0[0] = 1.0
+    t_eval = np.array([2.0, 10.0, 24.0, 40.0, 60.0])
+    sol = solve_ivp(lambda t, p: sum(ci * Ti for ci, Ti in zip(gene_coefs(t), gterms)) @ p,
+                    (0.0, t_eval[-1]), p0, method="DOP853", rtol=1e-12, atol=1e-15,
+                    t_eval=t_eval)
+    assert sol.success
+    out["bdf_gene_sin_t"] = t_eval
+    out["bdf_gene_sin_p"] = sol.y.T
     np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **out)
     print("wrote", os.path.join(HERE, "fixtures.npz"), sorted(out))
 

@@ -289,6 +289,34 @@ def test_happy_breakdown():
     assert np.array_equal(got[0], p0)
 
 
+def test_failed_solve_reports_stats_and_restarts():
+    # max_reject = 1: the first step whose first step size is rejected
+    # fails the solve.  The handle still reports its statistics, and a
+    # restart with the default limits gives the clean solve's result.
+    net = M.toggle_switch((31, 31))
+    A = capi.Mat(net, "csr", "f64")
+    p0 = M.delta_p0(net)
+    clean = dev(p0)
+    E0 = capi.Expv(A, A.n_rows, 5.0)
+    E0.solve(clean)
+    want, st_clean = host(clean), E0.stats()
+    assert st_clean.n_reject > 0
+    v = dev(p0)
+    E = capi.Expv(A, A.n_rows, 5.0)
+    E.set_limits(max_reject=1, max_nstep=10000)
+    with pytest.raises(capi.FspError):
+        E.solve(v)
+    st = E.stats()
+    assert st.n_reject >= 1 and st.t_now < 5.0
+    assert len(E.tau_history()) == st.i_step
+    E.set_limits(max_reject=1000, max_nstep=10000)
+    v.copy_from(p0)
+    E.restart(5.0)
+    E.solve(v)
+    assert E.stats().i_step == st_clean.i_step
+    assert np.array_equal(host(v), want)
+
+
 @pytest.mark.parametrize("fmt", ["csr", "hyb", "kron"])
 def test_mixed_precision_l1(fmt):
     net = M.gene_expression((2, 60, 60))
