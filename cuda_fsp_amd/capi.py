@@ -146,6 +146,7 @@ _SIGS = {
     "fsp_batch_get_config": (C.c_int, [_vp, _vp, _vp]),
     "fsp_batch_autotune": (C.c_int, [_vp, C.c_char_p, _i, _vp]),
     "fsp_loglik_indices": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
+    "fsp_loglik_indices_noalloc": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
     "fsp_marginal": (C.c_int, [_vp, _i, _vp, _vp, _vp, _vp]),
     "fsp_loglik_weighted": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
     "fsp_expm_dense": (C.c_int, [_i, _vp, _vp, _vp]),
@@ -387,6 +388,13 @@ def loglik_indices(p, idx, count, floor_val=1e-44, stream=None) -> float:
     s = C.c_double(0.0)
     check(lib().fsp_loglik_indices(ptr(p), ptr(idx), int(count), float(floor_val), C.byref(s),
                                    stream))
+    return s.value
+
+
+def loglik_indices_noalloc(p, idx, count, floor_val=1e-44, stream=None) -> float:
+    s = C.c_double(0.0)
+    check(lib().fsp_loglik_indices_noalloc(ptr(p), ptr(idx), int(count), float(floor_val),
+                                           C.byref(s), stream))
     return s.value
 
 

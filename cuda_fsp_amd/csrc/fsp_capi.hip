@@ -1,6 +1,7 @@
 // fsp_capi.hip -- C ABI entry points (include/fsp_capi.h): errors, device
 // memory helpers, state enumeration, vector reductions and fsp_mat.
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "fsp_formats.h"
@@ -525,6 +526,70 @@ fsp_status fsp_loglik_indices(const double *d_p, const int *d_idx, int64_t count
     fsp_status st = reduce_partials(tmp, G, h_ll, s);
     (void)hipFree(tmp);
     return st;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ allocation-free variant
+// Scratch of fsp_loglik_indices_noalloc: the G <= kNoAllocGrid dd partials
+// (2 G doubles) and the result slot.  File-scope device memory: each GPU
+// the library runs on holds its own copy (16 KiB) until the process ends.
+// Its address on each device comes from hipGetSymbolAddress: on the host
+// the array's name is not a device address.
+constexpr int kNoAllocGrid = 1024;
+constexpr int kNoAllocScratch = 2 * kNoAllocGrid + 1;
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the
+// first call there.
+fsp_status scratch_on_current_device(double **out)
+{
+    int dev = 0;
+    FSP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(FSP_ERR_UNSUPPORTED, "device index >= 64");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(FSP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FSP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+fsp_status fsp_loglik_indices_noalloc(const double *d_p, const int *d_idx, int64_t count,
+                                      double floor_val, double *h_ll, void *stream)
+{
+    if (!h_ll || count < 0 || (count > 0 && (!d_p || !d_idx)))
+        return fail(FSP_ERR_INVALID, "bad argument");
+    if (count == 0) {
+        *h_ll = 0.0;
+        return FSP_OK;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    // the same grid rule as fsp_loglik_indices, so both reduce their
+    // partials in the same order and agree bit for bit
+    const int G = grid_for((count + kBlock - 1) / kBlock, 1024);
+    if (2 * (int64_t)G + 1 > kNoAllocScratch)
+        return fail(FSP_ERR_UNSUPPORTED, "fsp_loglik_indices_noalloc: more than 1024 partial "
+                                         "sums do not fit the 16 KiB scratch");
+    double *tmp = nullptr;
+    FSP_TRY(scratch_on_current_device(&tmp));
+    // every partial slot 0..2G-1 is written by its block before k_sum_final
+    // reads it, and k_sum_final writes slot 2G: nothing needs clearing
+    k_loglik_idx<<<G, kBlock, 0, s>>>(d_p, d_idx, count, floor_val, tmp);
+    FSP_HIP(hipGetLastError());
+    return reduce_partials(tmp, G, h_ll, s);
 }
 
 fsp_status fsp_loglik_weighted(const double *d_q, const double *d_w, int64_t count,

@@ -331,6 +331,16 @@ fsp_status fsp_bdf_get_stats(fsp_bdf b, fsp_bdf_stats *stats);
  * outside the box) counts as probability 0.  Deterministic reduction. */
 fsp_status fsp_loglik_indices(const double *d_p, const int *d_idx, int64_t count,
                               double floor_val, double *h_ll, void *stream);
+/* fsp_loglik_indices without any device allocation.  No reference
+ * counterpart: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * partial sums live in a fixed 16 KiB device array of the library (one per
+ * GPU, held until the process ends), so any count is accepted (at most
+ * 1024 partial sums); the result equals fsp_loglik_indices' bit for bit.
+ * Calls that overlap on one GPU (other threads or streams) share that array
+ * and race: serialise them. */
+fsp_status fsp_loglik_indices_noalloc(const double *d_p, const int *d_idx, int64_t count,
+                                      double floor_val, double *h_ll, void *stream);
 /* Marginal of p over the species with keep_mask[s] != 0: d_out has
  * prod_{kept} (bounds[s]+1) entries, kept species in order, first fastest.
  * No reference counterpart (the joint-histogram likelihood needs it). */
