@@ -33,7 +33,18 @@ struct Model {
     cuFSP::PropFactorFun propensity_factor = nullptr;
 };
 
-// src/FSPSolver.h:16-111, src/FSPSolver.cpp:9-22
+// src/FSPSolver.h:16-111, src/FSPSolver.cpp:9-22.
+//
+// fsp_tol (accepted but never read by the reference, src/FSPSolver.h:36-44)
+// drives the adaptive FSP: with 0 < fsp_tol < 1 each output time t_k must
+// satisfy the truncation bound 1 - sum(p(t_k)) <= fsp_tol * t_k / t_max
+// (the FSP error of a generator whose diagonal carries the outflow leaving
+// the box, Munsky & Khammash 2006).  When it does not, the interval is
+// solved again from t_{k-1} on a larger box: every species with probability
+// on its upper edge (at least a tenth of the largest edge mass) grows by
+// half, the probabilities are carried over state by state, and the
+// generator is rebuilt.  P then has the new box's size.  fsp_tol <= 0 or
+// >= 1 keeps the box fixed.
 class FSPSolver {
 public:
     FSPSolver(std::vector<double> &p, const Model &model, const std::vector<double> &times,
@@ -54,17 +65,36 @@ public:
     const cuFSP::dvec &device_solution() const { return p_dev_; }
     size_t current_time_index() const { return current_; }
     int n_states() const { return n_; }
+    // the current FSP box (the model's FSPSize until the box grows)
+    const std::vector<int> &bounds() const { return bounds_; }
+    // 1 - sum(p) at the last output time: the L1 distance to the CME solution
+    double truncation_error() const { return err_; }
+    int n_expansions() const { return n_expand_; }
 
 private:
-    void init(double kry_tol, cuFSP::MatrixFormat format);
+    void build(const std::vector<int> &bounds);
+    void release();
+    void advance_to(double t);       // one output interval, with expansion
+    void grow_box(const std::vector<double> &p_fail);
+    void embed(const std::vector<double> &p_old, const std::vector<int> &old_bounds,
+               std::vector<double> &p_new) const;
     void load_initial();
     const Model &model_;
     std::vector<double> &P_;
     std::vector<double> times_;
     double fsp_tol_;
+    double kry_tol_;
+    cuFSP::MatrixFormat format_;
     size_t current_ = 0;
     int n_ = 0;
-    std::vector<double> p_init_;
+    std::vector<int> bounds_;
+    double t_now_ = 0.0;             // time of the solution in p_dev_
+    double t_origin_ = 0.0;          // time the Krylov object's clock starts at
+    double mass0_ = 1.0;             // sum of the initial distribution
+    double err_ = 0.0;
+    int n_expand_ = 0;
+    std::vector<double> p_start_;    // host copy of p at t_now_ (adaptive FSP)
+    std::vector<double> p_init_;     // initial distribution on the model's box
     cuFSP::dvec p_dev_;
     double *table_dev_ = nullptr;
     fsp_mat A_ = nullptr;
@@ -85,11 +115,13 @@ public:
     double evaluate(const std::vector<double> &parameters);
 
 private:
+    void map_data();                 // data -> state indices of the current box
     SingleCellData &data_;
     Model &model_;
     double floor_;
     std::vector<double> full_dist_;
     FSPSolver fsp_;
+    std::vector<int> mapped_bounds_;
     std::vector<int *> idx_dev_;
     std::vector<int64_t> idx_count_;
 };

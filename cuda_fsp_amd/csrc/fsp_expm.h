@@ -13,9 +13,13 @@ constexpr int kSubChunk = 8;   // loads issued together in back substitution
 // C = A * B for column-major N x N matrices in LDS.  Each thread computes a
 // BS x BS block of C from BS + BS LDS reads per BS^2 products; every element
 // is still sum_l A(i, l) B(l, j) accumulated in increasing l, the oracle's
-// order, so the block size changes speed only.
+// order, so the block size changes speed only.  A has lower bandwidth lba
+// (A(i, l) = 0 for i - l > lba) and B lower bandwidth lbb: the terms outside
+// l in [i - lba, j + lbb] are exact zeros and are skipped (adding +-0 to the
+// running sum leaves its value unchanged); pass N for dense operands.
 template <int BS>
-__device__ inline void matmul_lds_bs(int N, const double *A, const double *Bm, double *C)
+__device__ inline void matmul_lds_bs(int N, const double *A, const double *Bm, double *C,
+                                     int lba, int lbb)
 {
     const int nb = (N + BS - 1) / BS;
     for (int b = threadIdx.x; b < nb * nb; b += blockDim.x) {
@@ -25,7 +29,9 @@ __device__ inline void matmul_lds_bs(int N, const double *A, const double *Bm, d
         for (int a = 0; a < BS; ++a)
 #pragma unroll
             for (int c = 0; c < BS; ++c) acc[a][c] = 0.0;
-        for (int l = 0; l < N; ++l) {
+        const int l0 = max(0, i0 - lba);
+        const int l1 = min(N - 1, j0 + BS - 1 + lbb);
+        for (int l = l0; l <= l1; ++l) {
             double av[BS], bv[BS];
 #pragma unroll
             for (int a = 0; a < BS; ++a) av[a] = (i0 + a < N) ? A[(i0 + a) + l * N] : 0.0;
@@ -46,38 +52,85 @@ __device__ inline void matmul_lds_bs(int N, const double *A, const double *Bm, d
 
 // 2 x 2 blocks when the workgroup has a thread for most of them (more
 // waves in flight), 4 x 4 otherwise (one wave).
-__device__ inline void matmul_lds(int N, const double *A, const double *Bm, double *C)
+__device__ inline void matmul_lds(int N, const double *A, const double *Bm, double *C,
+                                  int lba, int lbb)
 {
     const int nb2 = (N + 1) / 2;
     if ((int)blockDim.x * 2 >= nb2 * nb2)
-        matmul_lds_bs<2>(N, A, Bm, C);
+        matmul_lds_bs<2>(N, A, Bm, C, lba, lbb);
     else
-        matmul_lds_bs<4>(N, A, Bm, C);
+        matmul_lds_bs<4>(N, A, Bm, C, lba, lbb);
+}
+
+constexpr int kExpmMaxN = 64;
+
+// Back substitution U X = B for the 32 x 32 case, one right-hand side per
+// thread with its column in registers (fully unrolled): x_i = (b_i -
+// sum_{k > i} U(i, k) x_k) / U(i, i), k ascending, as the oracle.  U(i, k)
+// is D(perm[i], k) and B(i, j) is Nm(perm[i], j); the solution overwrites
+// column j of Nm in natural row order (each thread reads and writes only
+// its own column).
+__device__ inline void back_subst_32(const double *Dm, double *Nm, const int *perm)
+{
+    constexpr int N = 32;
+    const int j = threadIdx.x;
+    if (j >= N) return;
+    double x[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = Nm[perm[i] + j * N];
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        const int ri = perm[i];
+        double acc = x[i];
+#pragma unroll
+        for (int k = i + 1; k < N; ++k) acc -= Dm[ri + k * N] * x[k];
+        x[i] = acc / Dm[ri + i * N];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) Nm[i + j * N] = x[i];
 }
 
 // --------------------------------------------------------------- dense expm
 // Golub & Van Loan, Matrix Computations, Alg. 11.3.1 (diagonal Pade q = 6,
 // scaling by 2^-j with j = max(0, 1 + floor(log2 |A|_inf)), Gaussian
 // elimination with partial pivoting, j squarings), column-major N x N
-// (N <= 64) in LDS, one workgroup of any size.  Same
-// operation order as the oracle's orc_expm, so the two agree bit for bit on
-// equal input.  Result in Nm.
+// (N <= 64) in LDS, one workgroup of any size.  Same operation order as the
+// oracle's orc_expm, so the two agree bit for bit on equal input.  Result in
+// Nm.
+//
+// Work that only adds exact zeros is skipped.  With b the lower bandwidth
+// of A (1 for the Krylov Hessenberg), A^k has lower bandwidth k b and the
+// Pade denominator D lower bandwidth 6 b; elimination with partial pivoting
+// keeps that bandwidth in the multipliers (the rows below it hold zeros in
+// the pivot column, so their multiplier is 0 and their update subtracts
+// zeros), so each step searches and updates only the 6 b rows under the
+// pivot.  Row exchanges are a logical permutation (perm), so a step needs
+// one barrier.
 __device__ inline void expm_block(int N, double *A, double *X, double *T, double *Nm, double *Dm,
-                           double *red)
+                                  double *red)
 {
     (void)red;
     const int NN = N * N;
     const int tid = threadIdx.x;
-    __shared__ int js_sh, piv_sh;
-    // infinity norm: row sums in column order, max taken in row order
+    __shared__ int js_sh, lb_sh;
+    __shared__ int perm[2][kExpmMaxN];
+    // infinity norm (row sums in column order, max in any order: exact)
+    // and the lower bandwidth
     if (tid < kWave) {
         double rs = 0.0;
+        int lb = 0;
         if (tid < N)
-            for (int j = 0; j < N; ++j) rs += fabs(A[tid + j * N]);
-        // max over lanes (the maximum is exact, its order irrelevant)
+            for (int j = 0; j < N; ++j) {
+                const double v = A[tid + j * N];
+                rs += fabs(v);
+                if (v != 0.0 && tid - j > lb) lb = tid - j;
+            }
         double mx = rs;
 #pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, kWave));
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            mx = fmax(mx, __shfl_xor(mx, off, kWave));
+            lb = max(lb, __shfl_xor(lb, off, kWave));
+        }
         if (tid == 0) {
             int js = 0;
             if (mx > 0.0) {
@@ -86,10 +139,12 @@ __device__ inline void expm_block(int N, double *A, double *X, double *T, double
                 js = e > 0 ? e : 0;
             }
             js_sh = js;
+            lb_sh = lb;
         }
     }
     __syncthreads();
     const int js = js_sh;
+    const int lb = lb_sh;
     const double scale = ldexp(1.0, -js);
     for (int k = tid; k < NN; k += blockDim.x) {
         A[k] = A[k] * scale;
@@ -98,12 +153,14 @@ __device__ inline void expm_block(int N, double *A, double *X, double *T, double
         Nm[k] = id;
         Dm[k] = id;
     }
+    for (int i = tid; i < N; i += blockDim.x) perm[0][i] = i;
     __syncthreads();
     const int q = 6;
     double c = 1.0;
     for (int kk = 1; kk <= q; ++kk) {
         c = c * (double)(q - kk + 1) / (double)((2 * q - kk + 1) * kk);
-        matmul_lds(N, A, X, T);
+        // X = A^(kk-1) has lower bandwidth (kk-1) lb
+        matmul_lds(N, A, X, T, lb, min(N, (kk - 1) * lb));
         __syncthreads();
         const double sgn = (kk % 2) ? -1.0 : 1.0;
         for (int e = tid; e < NN; e += blockDim.x) {
@@ -113,96 +170,98 @@ __device__ inline void expm_block(int N, double *A, double *X, double *T, double
         }
         __syncthreads();
     }
-    // solve Dm F = Nm by Gaussian elimination with partial pivoting
+    // solve Dm F = Nm by Gaussian elimination with partial pivoting; row
+    // exchanges go to the permutation (logical row i is physical row
+    // perm[cur][i])
+    const int bw = min(N - 1, q * lb);
+    int cur = 0;
     for (int k = 0; k < N; ++k) {
-        if (tid == 0) {
-            // first row of maximal |Dm(i, k)|, i >= k (the oracle's choice);
-            // one thread: N - k independent LDS loads, no shuffle round trips
-            int p = k;
-            double best = fabs(Dm[k + k * N]);
-            for (int i = k + 1; i < N; ++i) {
-                const double v = fabs(Dm[i + k * N]);
-                if (v > best) {
-                    best = v;
-                    p = i;
-                }
+        const int *pc = perm[cur];
+        int *pn = perm[cur ^ 1];
+        const int iend = min(N - 1, k + bw);
+        // first logical row of maximal |D(row, k)|, k <= row <= iend (the
+        // oracle's choice: the rows below hold zeros), found by every thread
+        int p = k;
+        double best = fabs(Dm[pc[k] + k * N]);
+        for (int i = k + 1; i <= iend; ++i) {
+            const double v = fabs(Dm[pc[i] + k * N]);
+            if (v > best) {
+                best = v;
+                p = i;
             }
-            piv_sh = p;
         }
-        __syncthreads();
-        const int p = piv_sh;
-        if (p != k) {
-            for (int j = tid; j < N; j += blockDim.x) {
-                double t = Dm[k + j * N];
-                Dm[k + j * N] = Dm[p + j * N];
-                Dm[p + j * N] = t;
-                t = Nm[k + j * N];
-                Nm[k + j * N] = Nm[p + j * N];
-                Nm[p + j * N] = t;
-            }
-            __syncthreads();
-        }
-        // Row updates below the pivot by (row, column chunk) pairs: each pair
-        // forms its row's multiplier (the same division in every pair, so the
-        // same value), loads its chunk of the unchanged pivot row and of its
-        // own row, then stores the updated chunk.  Consecutive pairs own
-        // consecutive rows (conflict-free LDS).  The multipliers are not kept:
-        // only the upper triangle is read afterwards.
-        const double pv = Dm[k + k * N];
-        const int rows = N - k - 1;
+        for (int i = tid; i < N; i += blockDim.x) pn[i] = (i == k) ? pc[p] : (i == p ? pc[k] : pc[i]);
+        const int rk = pc[p];
+        const double pv = Dm[rk + k * N];
+        // updates of the logical rows k+1..iend by (row, column chunk)
+        // pairs: each pair forms its row's multiplier (the same division in
+        // every pair), loads its chunk of the pivot row and of its own row,
+        // then stores the updated chunk.  The multipliers are not kept: only
+        // the upper triangle is read afterwards.
+        const int rows = iend - k;
         const int chunks = (N + kElimChunk - 1) / kElimChunk;
         for (int e = tid; e < rows * chunks; e += blockDim.x) {
             const int i = k + 1 + e % rows;
+            const int ri = (i == p) ? pc[k] : pc[i];
             const int j0 = (e / rows) * kElimChunk;
-            const double l = Dm[i + k * N] / pv;
-            double pd[kElimChunk], pn[kElimChunk], rd[kElimChunk], rn[kElimChunk];
+            const double l = Dm[ri + k * N] / pv;
+            double pd[kElimChunk], pn2[kElimChunk], rd[kElimChunk], rn[kElimChunk];
 #pragma unroll
-            for (int q = 0; q < kElimChunk; ++q) {
-                const int j = min(j0 + q, N - 1);
-                pd[q] = Dm[k + j * N];
-                pn[q] = Nm[k + j * N];
-                rd[q] = Dm[i + j * N];
-                rn[q] = Nm[i + j * N];
+            for (int u = 0; u < kElimChunk; ++u) {
+                const int j = min(j0 + u, N - 1);
+                pd[u] = Dm[rk + j * N];
+                pn2[u] = Nm[rk + j * N];
+                rd[u] = Dm[ri + j * N];
+                rn[u] = Nm[ri + j * N];
             }
 #pragma unroll
-            for (int q = 0; q < kElimChunk; ++q) {
-                const int j = j0 + q;
+            for (int u = 0; u < kElimChunk; ++u) {
+                const int j = j0 + u;
                 if (j < N) {
-                    if (j > k) Dm[i + j * N] = rd[q] - l * pd[q];
-                    Nm[i + j * N] = rn[q] - l * pn[q];
+                    if (j > k) Dm[ri + j * N] = rd[u] - l * pd[u];
+                    Nm[ri + j * N] = rn[u] - l * pn2[u];
                 }
             }
         }
         __syncthreads();
+        cur ^= 1;
     }
-    // back substitution, one right-hand side per thread, on a row-major copy
-    // of Nm in X (consecutive lanes read consecutive LDS words)
-    for (int e = tid; e < NN; e += blockDim.x) X[(e / N) + (e % N) * N] = Nm[e];
-    __syncthreads();
-    for (int j = tid; j < N; j += blockDim.x) {
-        for (int i = N - 1; i >= 0; --i) {
-            // k ascending as one sum; kSubChunk loads issued together
-            double acc = X[j + i * N];
-            int k = i + 1;
-            for (; k + kSubChunk <= N; k += kSubChunk) {
-                double d[kSubChunk], x[kSubChunk];
-#pragma unroll
-                for (int u = 0; u < kSubChunk; ++u) {
-                    d[u] = Dm[i + (k + u) * N];
-                    x[u] = X[j + (k + u) * N];
-                }
-#pragma unroll
-                for (int u = 0; u < kSubChunk; ++u) acc -= d[u] * x[u];
-            }
-            for (; k < N; ++k) acc -= Dm[i + k * N] * X[j + k * N];
-            X[j + i * N] = acc / Dm[i + i * N];
+    const int *pf = perm[cur];
+    if (N == 32) {
+        back_subst_32(Dm, Nm, pf);
+    } else {
+        // one right-hand side per thread on a row-major copy of Nm in X in
+        // logical row order (consecutive lanes read consecutive LDS words)
+        for (int e = tid; e < NN; e += blockDim.x) {
+            const int i = e % N, j = e / N;
+            X[j + i * N] = Nm[pf[i] + j * N];
         }
+        __syncthreads();
+        for (int j = tid; j < N; j += blockDim.x) {
+            for (int i = N - 1; i >= 0; --i) {
+                const int ri = pf[i];
+                double acc = X[j + i * N];
+                int k = i + 1;
+                for (; k + kSubChunk <= N; k += kSubChunk) {
+                    double d[kSubChunk], x[kSubChunk];
+#pragma unroll
+                    for (int u = 0; u < kSubChunk; ++u) {
+                        d[u] = Dm[ri + (k + u) * N];
+                        x[u] = X[j + (k + u) * N];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kSubChunk; ++u) acc -= d[u] * x[u];
+                }
+                for (; k < N; ++k) acc -= Dm[ri + k * N] * X[j + k * N];
+                X[j + i * N] = acc / Dm[ri + i * N];
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < NN; e += blockDim.x) Nm[e] = X[(e / N) + (e % N) * N];
     }
-    __syncthreads();
-    for (int e = tid; e < NN; e += blockDim.x) Nm[e] = X[(e / N) + (e % N) * N];
     __syncthreads();
     for (int sq = 0; sq < js; ++sq) {
-        matmul_lds(N, Nm, Nm, T);
+        matmul_lds(N, Nm, Nm, T, N, N);
         __syncthreads();
         for (int e = tid; e < NN; e += blockDim.x) Nm[e] = T[e];
         __syncthreads();

@@ -218,6 +218,86 @@ __device__ inline double kron_row(const KronF<V> &A, uint32_t i, const X &x, con
     return y;
 }
 
+// Per-reaction geometry of the Kronecker apply, passed by value to the
+// kernels (so it sits in the kernel-argument segment and every field is a
+// scalar load shared by the wave): the inflow source offset, each species'
+// net change and factor-table offset (-1: the factor is 1 everywhere), and
+// the box radices.  Models with more than kKronPlanR reactions use
+// kron_row.
+constexpr int kKronPlanR = 24;
+struct KronRx {
+    int delta;                          // inflow source of row i: i - delta
+    int live;                           // 0: net change zero, contributes 0
+    int nu[kMaxSpecies];
+    int foff[kMaxSpecies];
+};
+struct KronPlan {
+    int R;
+    int radix[kMaxSpecies];
+    KronRx rx[kKronPlanR];              // one contiguous record per reaction
+};
+
+// kron_row with the species loop unrolled (no per-species selects on the
+// digits) and the reaction geometry read from a KronPlan: per reaction,
+// the source range checks of the species it changes, the factor products
+// at x and at x - nu in species order, one gather, then
+// y += c (x_src alpha - x_i beta) in reaction order, as the oracle's
+// orc_kron_apply does, so rows agree bit for bit.  The gather is issued
+// for every reaction (row i itself when the source is outside the box,
+// weighted by 0).
+template <int D, typename V, class X>
+__device__ inline double kron_row_plan(const KronF<V> &A, const KronPlan &P, uint32_t i,
+                                       const X &x, const V *tab, dd &sink)
+{
+    if (i >= A.n) return 0.0;
+    int xs[D];
+    decode<D>(i, A.dm, xs);
+    const double xh = x(i);
+    double y = 0.0, out = 0.0;
+    for (int r = 0; r < P.R; ++r) {
+        const KronRx q = P.rx[r];  // the whole record in scalar registers
+        if (!q.live) continue;
+        const double c = A.coefs[r];
+        // source range checks and target leaves of the changed species
+        bool in = true, leaves = false;
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            if (q.nu[s] != 0) {
+                const int src = xs[s] - q.nu[s], tgt = xs[s] + q.nu[s];
+                in = in && src >= 0 && src < P.radix[s];
+                leaves = leaves || tgt < 0 || tgt >= P.radix[s];
+            }
+        }
+        // every factor load issued before the products
+        double fb[D], fa[D];
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            fb[s] = 1.0;
+            fa[s] = 1.0;
+            if (q.foff[s] >= 0) {
+                fb[s] = (double)tab[q.foff[s] + xs[s]];
+                if (q.nu[s] != 0)
+                    fa[s] = (double)tab[q.foff[s] + min(max(xs[s] - q.nu[s], 0), P.radix[s] - 1)];
+            }
+        }
+        const uint32_t si = in ? (uint32_t)((int64_t)i - q.delta) : i;
+        const double xsrc = x(si);
+        // products in species order (factors equal to 1 are skipped: exact)
+        double alpha = 1.0, beta = 1.0;
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            if (q.foff[s] >= 0) {
+                beta *= fb[s];
+                alpha *= (q.nu[s] != 0) ? fa[s] : fb[s];
+            }
+        }
+        y += c * (xsrc * (in ? alpha : 0.0) - xh * beta);
+        if (A.with_sink && leaves) out += c * beta;
+    }
+    if (A.with_sink) sink = dd_add_prod(sink, out, xh);
+    return y;
+}
+
 // Transpose (adjoint) row: (A^T x)_j = sum_r c_r a_r(x_j) (x_{j+delta_r}
 // [target in box] - x_j).  No sink column (the sink is absorbing).
 template <int D, typename V, class X>
@@ -290,6 +370,28 @@ inline HybF<V> hyb_view(const MatImpl &M)
     A.n = (uint32_t)M.n;
     A.U = M.hm.dm.U;
     return A;
+}
+
+// Plan of a model with at most kKronPlanR reactions (false otherwise).
+inline bool kron_plan(const HostModel &hm, KronPlan &P)
+{
+    const DevModel &dm = hm.dm;
+    if (dm.R > kKronPlanR || !hm.has_factors) return false;
+    P = KronPlan{};
+    P.R = dm.R;
+    for (int s = 0; s < kMaxSpecies; ++s) P.radix[s] = s < dm.d ? dm.radix[s] : 1;
+    for (int r = 0; r < dm.R; ++r) {
+        const RxnDev &rx = dm.rx[r];
+        KronRx &q = P.rx[r];
+        q.delta = (int)rx.delta;
+        q.live = rx.zero_nu ? 0 : 1;
+        for (int s = 0; s < kMaxSpecies; ++s) q.foff[s] = -1;
+        for (int k = 0; k < rx.nsp; ++k) {
+            q.nu[rx.sp[k]] = rx.nu[k];
+            q.foff[rx.sp[k]] = rx.foff[k];
+        }
+    }
+    return true;
 }
 
 template <typename V>

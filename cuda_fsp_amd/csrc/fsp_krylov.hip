@@ -412,6 +412,30 @@ __global__ void __launch_bounds__(kBlock) k_arn_kron(KronF<V> A, ArnArgs a, int 
     arn_finish<B>(a, nrm, sink, red);
 }
 
+// k_arn_kron with the reaction geometry in a KronPlan; the factor tables are
+// read from LDS only (LDS loads).
+template <int D, typename V, typename B>
+__global__ void __launch_bounds__(kBlock) k_arn_kron_plan(KronF<V> A, KronPlan P, ArnArgs a)
+{
+    if (arn_skip<B>(a)) return;
+    extern __shared__ double smem[];
+    double *red = smem;
+    V *tabl = (V *)(red + kArnRedLen);
+    stage_table(tabl, A.tab_g, A.tab_size);
+    const V *tab = tabl;
+    double sig;
+    if (!arn_sigma(a, red, sig)) return;
+    dd nrm = dd_zero();
+    const KryX<B> X{a.in, sig, a.j};
+    dd sink = dd_zero();
+    for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
+        const uint32_t i = w.t * kBlock + threadIdx.x;
+        const double y = kron_row_plan<D, V>(A, P, i, X, tab, sink);
+        arn_epi<B>(a, i, y, i < a.n ? X(i) : 0.0, nrm);
+    }
+    arn_finish<B>(a, nrm, sink, red);
+}
+
 #define FSP_SWITCH_NK(nk, CALL)                                                \
     if ((nk) <= 8) {                                                           \
         constexpr int NK_ = 8;                                                 \
@@ -810,8 +834,8 @@ static fsp_status expv_alloc(ExpvImpl &E)
 
 static void expv_free(ExpvImpl &E)
 {
-    if (E.gexec) hipGraphExecDestroy(E.gexec);
-    if (E.graph) hipGraphDestroy(E.graph);
+    if (E.gexec) (void)hipGraphExecDestroy(E.gexec);
+    if (E.graph) (void)hipGraphDestroy(E.graph);
     if (E.cap) (void)hipStreamDestroy(E.cap);
     void *ptrs[] = {E.d_st, E.V64, E.V32, E.W[0], E.W[1], E.partials, E.tau_log, E.dcoef};
     for (void *p : ptrs)
@@ -857,16 +881,28 @@ static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
         else
             launch_arn_hyb<double, B>(hyb_view<double>(M), a, E.G, red_b, E.s);
     } else {
+        KronPlan P;
+        const bool plan0 = !M.hm.prop_table_dev && kron_plan(M.hm, P);
         if (mixed && M.hm.has_factors) {
             KronF<float> A = kron_view<float>(M, -1, 0.0);
             const size_t tb = sizeof(float) * (size_t)A.tab_size;
             const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
-            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, float, B><<<E.G, kBlock, red_b + (use_lds ? tb : 0), E.s>>>(A, a, use_lds)));
+            const size_t lds = red_b + (use_lds ? tb : 0);
+            if (plan0 && use_lds) {
+                FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron_plan<D_, float, B><<<E.G, kBlock, lds, E.s>>>(A, P, a)));
+            } else {
+                FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, float, B><<<E.G, kBlock, lds, E.s>>>(A, a, use_lds)));
+            }
         } else {
             KronF<double> A = kron_view<double>(M, -1, 0.0);
             const size_t tb = sizeof(double) * (size_t)A.tab_size;
             const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
-            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, double, B><<<E.G, kBlock, red_b + (use_lds ? tb : 0), E.s>>>(A, a, use_lds)));
+            const size_t lds = red_b + (use_lds ? tb : 0);
+            if (plan0 && use_lds) {
+                FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron_plan<D_, double, B><<<E.G, kBlock, lds, E.s>>>(A, P, a)));
+            } else {
+                FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron<D_, double, B><<<E.G, kBlock, lds, E.s>>>(A, a, use_lds)));
+            }
         }
     }
     FSP_HIP(hipGetLastError());
@@ -963,8 +999,8 @@ static fsp_status launch_step(ExpvImpl &E, double *d_v)
 {
     if (!E.A || !E.use_graph || E.profile) return enqueue_any(E, d_v);
     if (!E.gexec || E.graph_v != d_v) {
-        if (E.gexec) hipGraphExecDestroy(E.gexec);
-        if (E.graph) hipGraphDestroy(E.graph);
+        if (E.gexec) (void)hipGraphExecDestroy(E.gexec);
+        if (E.graph) (void)hipGraphDestroy(E.graph);
         E.gexec = nullptr;
         E.graph = nullptr;
         if (!E.cap) FSP_HIP(hipStreamCreateWithFlags(&E.cap, hipStreamNonBlocking));
@@ -1300,8 +1336,8 @@ extern "C" fsp_status fsp_expv_set_limits(fsp_expv e, int max_reject, int64_t ma
     ExpvImpl &E = e->impl;
     E.max_reject = max_reject;
     E.max_nstep = max_nstep;
-    if (E.gexec) hipGraphExecDestroy(E.gexec);
-    if (E.graph) hipGraphDestroy(E.graph);
+    if (E.gexec) (void)hipGraphExecDestroy(E.gexec);
+    if (E.graph) (void)hipGraphDestroy(E.graph);
     E.gexec = nullptr;
     E.graph = nullptr;
     return FSP_OK;

@@ -573,6 +573,28 @@ __global__ void __launch_bounds__(kBlock) k_apply_kron(KronF<V> A, PlainX X, dou
     if (A.with_sink && !transpose) put_dd_partial(partials, sink, red);
 }
 
+// The same product with the reaction geometry in a KronPlan (models with
+// at most kKronPlanR reactions whose factor tables fit in LDS).  The tables
+// are read from LDS only, so their loads are LDS loads.
+template <int D, typename V>
+__global__ void __launch_bounds__(kBlock) k_apply_kron_plan(KronF<V> A, KronPlan P, PlainX X,
+                                                            double *__restrict__ y, int accumulate,
+                                                            uint32_t tiles,
+                                                            double *__restrict__ partials)
+{
+    extern __shared__ double smem[];
+    __shared__ double red[2 * kBlock / kWave];
+    stage_table((V *)smem, A.tab_g, A.tab_size);
+    const V *tab = (const V *)smem;
+    dd sink = dd_zero();
+    for (TileWalk w = tile_walk(tiles); w.t < w.end; w.t += w.step) {
+        const uint32_t i = w.t * kBlock + threadIdx.x;
+        const double v = kron_row_plan<D, V>(A, P, i, X, tab, sink);
+        if (i < A.n) y[i] = accumulate ? y[i] + v : v;
+    }
+    if (A.with_sink) put_dd_partial(partials, sink, red);
+}
+
 // Diagonal of the generator: -sum_r c_r a_r(x_i) over the reactions that
 // change the state (the whole outflow, including outflow leaving the box).
 template <int D, typename V>
@@ -630,9 +652,15 @@ static fsp_status launch_kron(const MatImpl &M, const double *x, double *y, int 
     const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
     const int with_sink = M.with_sink && !transpose && single < 0;
     A.with_sink = with_sink;
-    FSP_SWITCH_D(M.hm.dm.d, (k_apply_kron<D_, V><<<grid, kBlock, use_lds ? tb : 0, s>>>(
-                                A, PlainX{x}, y, accumulate, tiles, M.partials, use_lds,
-                                transpose)));
+    KronPlan P;
+    if (!transpose && single < 0 && !A.prop_table && use_lds && kron_plan(M.hm, P)) {
+        FSP_SWITCH_D(M.hm.dm.d, (k_apply_kron_plan<D_, V><<<grid, kBlock, tb, s>>>(
+                                    A, P, PlainX{x}, y, accumulate, tiles, M.partials)));
+    } else {
+        FSP_SWITCH_D(M.hm.dm.d, (k_apply_kron<D_, V><<<grid, kBlock, use_lds ? tb : 0, s>>>(
+                                    A, PlainX{x}, y, accumulate, tiles, M.partials, use_lds,
+                                    transpose)));
+    }
     FSP_HIP(hipGetLastError());
     if (with_sink) {
         k_finalize_sink<<<1, kBlock, 0, s>>>(M.partials, grid, y + M.n, accumulate);

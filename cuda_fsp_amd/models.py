@@ -21,6 +21,9 @@ The builders below define the workloads used by tests and the benchmark:
   (9 terms, 3 x 512 x 512 = 786,432 states by default).
 * ``five_species_chain`` -- G in 0..3 and four counts bounded 31, 31, 63, 63
   (16,777,216 states, 14 reactions, rates log-uniform in [1e-3, 10]).
+* ``hog1p`` -- the reference's 5-species Hog1p problem on the box
+  {3, 50, 50, 60, 60} (38,713,284 states, 9 reactions, one time-varying
+  coefficient, ``hog1p_coefs``).
 """
 from __future__ import annotations
 
@@ -221,6 +224,66 @@ def five_species_chain(bounds=(3, 31, 31, 63, 63), seed=0) -> ReactionNetwork:
     return ReactionNetwork("five_species_chain", np.asarray(bounds, np.int32), nu,
                            _tabulate(bounds, R, fn), rates,
                            tuple(f"{k}{a}" for k, a in kinds))
+
+
+# Parameters of the reference's Hog1p problem (problems/hog1p_5species.cu:22-30).
+HOG1P = dict(k12=1.29, k21=1.0e0, k23=0.0067, k32=0.027, k34=0.133, k43=0.0381,
+             kr2=0.0116, kr3=0.987, kr4=0.0538, trans=0.01, gamma=0.0049,
+             r1=6.9e-5, r2=7.1e-3, eta=3.1, Ahog=9.3e09, Mhog=6.4e-4)
+
+
+def hog1p(bounds=(3, 50, 50, 60, 60)) -> ReactionNetwork:
+    """The reference's 5-species Hog1p problem (problems/hog1p_5species.cu):
+    gene state G in 0..3, two transcripts made in the nucleus (species 1, 2)
+    and moved to the cytoplasm (species 3, 4), 9 reactions (stoichiometry at
+    :239-241): G+1, G-1, G-1 (signal-repressed), x1+1, x2+1, x1->x3, x2->x4,
+    x3-1, x4-1.  The propensities are those of the device function at
+    :33-99, written as separable factor tables: the G-dependent rate
+    constants in the species-0 factor, trans x_s and gamma x_s in the count
+    factor.  (The reference's host factor function at :101-184 falls through
+    its switch cases and gives reactions 5-7 factors of other species; the
+    tables here follow the device function the reference's HYB run uses.)
+    Every time coefficient is 1 except reaction 2's, hog1p_coefs(t)
+    (:187-197).  Box {3, 50, 50, 60, 60}: 38,713,284 states."""
+    p = HOG1P
+    nu = np.zeros((9, 5), np.int32)
+    nu[0, 0], nu[1, 0], nu[2, 0] = 1, -1, -1
+    nu[3, 1], nu[4, 2] = 1, 1
+    nu[5, 1], nu[5, 3] = -1, 1
+    nu[6, 2], nu[6, 4] = -1, 1
+    nu[7, 3], nu[8, 4] = -1, -1
+    gtab = {0: (p["k12"], p["k23"], p["k34"], 0.0),
+            1: (0.0, 0.0, p["k32"], p["k43"]),
+            2: (0.0, p["k21"], 0.0, 0.0),
+            3: (0.0, p["kr2"], p["kr3"], p["kr4"]),
+            4: (0.0, p["kr2"], p["kr3"], p["kr4"])}
+    count = {5: (1, p["trans"]), 6: (2, p["trans"]), 7: (3, p["gamma"]), 8: (4, p["gamma"])}
+
+    def fn(r, s, i):
+        x = i.astype(np.float64)
+        if s == 0 and r in gtab:
+            g = np.asarray(gtab[r], np.float64)
+            return g[np.minimum(i, 3)] * (i <= 3)
+        if r in count and s == count[r][0]:
+            return count[r][1] * x
+        return np.ones_like(x)
+
+    return ReactionNetwork("hog1p", np.asarray(bounds, np.int32), nu,
+                           _tabulate(bounds, 9, fn), np.ones(9, np.float64),
+                           ("G+", "G-", "G-sig", "x1+", "x2+", "x1->x3", "x2->x4", "x3-", "x4-"))
+
+
+def hog1p_coefs(t: float) -> np.ndarray:
+    """Time coefficients of the Hog1p problem (problems/hog1p_5species.cu:
+    187-197): 1 for every reaction except reaction 2, which is
+    max(0, 3200 - 7710 hog1p(t)) with hog1p(t) = A (h / (1 + h / M))^eta,
+    h = (1 - exp(-r1 t)) exp(-r2 t)."""
+    p = HOG1P
+    out = np.ones(9, np.float64)
+    h1 = (1.0 - math.exp(-p["r1"] * t)) * math.exp(-p["r2"] * t)
+    sig = math.pow(h1 / (1.0 + h1 / p["Mhog"]), p["eta"]) * p["Ahog"]
+    out[2] = max(0.0, 3200.0 - 7710.0 * sig)
+    return out
 
 
 def birth_death(bounds=(63,), k=5.0, g=1.0) -> ReactionNetwork:

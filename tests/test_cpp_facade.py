@@ -10,7 +10,7 @@ import scipy.stats as ss
 from cuda_fsp_amd import build as B, models as M
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-PROGRAMS = ["test_fsp_get_states", "test_fsp_mat", "test_kry_expv", "test_cme"]
+PROGRAMS = ["test_fsp_get_states", "test_fsp_mat", "test_kry_expv", "test_cme", "test_fsp_expand"]
 
 
 def _exe(name, tmp_path_factory):
@@ -25,7 +25,8 @@ def test_cpp_programs_compile(name, tmp_path_factory):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["test_fsp_get_states", "test_fsp_mat", "test_kry_expv"])
+@pytest.mark.parametrize("name", ["test_fsp_get_states", "test_fsp_mat", "test_kry_expv",
+                                  "test_fsp_expand"])
 def test_cpp_reference_tests(name, tmp_path_factory):
     exe = _exe(name, tmp_path_factory)
     print(f"[case] {name}", flush=True)

@@ -94,8 +94,15 @@ def test_gene_expression_full_box_matches_oracle():
     assert net.n_states == 786_432
     t_out = [2.0 * k for k in range(1, 31)]
     got = _free_run_parity(net, "hyb", t_out)
-    # no sink state: the box keeps the mass to t = 60
-    assert abs(got[-1].sum() - 1.0) <= 1e-9
+    # no sink state: 1 - sum(p) is the FSP truncation error of the box,
+    # 5.7e-8 at t = 60 (nascent counts reaching 512), growing with t
+    mass = [1.0 - g.sum() for g in got]
+    print(f"[parity] 1 - sum(p) at t = 2, 20, 40, 60: {mass[0]:.2e} {mass[9]:.2e} "
+          f"{mass[19]:.2e} {mass[29]:.2e}", flush=True)
+    # the Krylov steps keep the mass to their tolerance; the deficiency is
+    # the truncation error, which is below 1e-7 up to t = 60
+    assert all(a >= -1e-10 for a in mass)
+    assert mass[-1] <= 1e-7
 
 
 CHAIN_SMALL = (3, 7, 7, 15, 15)
