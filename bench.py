@@ -38,6 +38,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import platform
 import sys
@@ -132,6 +133,30 @@ def bench_spmv(net, fmt, precision, reps, ev):
                achieved_GBs=round(moved / (ms * 1e-3) / 1e9, 1),
                frac_of_peak=round(moved / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
     A.close()
+    return out
+
+
+def bench_expm(ev, reps=200):
+    """The device Pade exponential (fsp_expm_dense: the step-control kernels'
+    expm, one workgroup) on 32 x 32 upper Hessenberg matrices (the Krylov
+    shape, m = 30) at several infinity norms (more squarings for larger
+    norms): microseconds per call, HIP events around `reps` calls."""
+    rng = np.random.default_rng(5)
+    out = []
+    for norm in (0.5, 10.0, 200.0):
+        H = np.triu(rng.standard_normal((32, 32)), -1)
+        H *= norm / np.abs(H).sum(axis=1).max()
+        d_A = capi.DeviceBuffer.from_host(np.asfortranarray(H).ravel(order="F"))
+        d_F = capi.DeviceBuffer.zeros(32 * 32)
+        for _ in range(3):
+            capi.expm_dense(d_A, d_F, 32)
+        a, b = ev.event(), ev.event()
+        ev.record(a)
+        for _ in range(reps):
+            capi.expm_dense(d_A, d_F, 32)
+        ev.record(b)
+        out.append({"inf_norm": norm, "squarings": max(0, math.frexp(norm)[1]),
+                    "us_per_call": round(ev.elapsed_ms(a, b) / reps * 1e3, 2)})
     return out
 
 
@@ -609,6 +634,10 @@ def main():
         print(f"[bench] SpMV {fmt}/{prec} on n={big.n_states}", flush=True)
         spmv.append(bench_spmv(big, fmt, prec, args.spmv_reps, ev))
 
+    print("[bench] device Pade exponential", flush=True)
+    expm = bench_expm(ev)
+    print(f"[bench] expm 32x32: {expm}", flush=True)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU oracle: the whole solve", flush=True)
@@ -656,6 +685,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kern,
+        "expm_dense": expm,
+        "krylov_rejects_per_solve": int(st.n_reject),
         "profiled_solve_s": round(dt_prof, 3),
         "spmv": spmv,
         "csr_solve": {"solves_per_s": round(k_side / dtc, 4),

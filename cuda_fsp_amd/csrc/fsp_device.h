@@ -21,6 +21,16 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
+// Orders the LDS accesses of one wavefront's lanes: a wave executes its LDS
+// instructions in order, so after this fence (outstanding LDS operations
+// complete, no memory access moves across it) every lane sees what the other
+// lanes of its wave stored before it.  Called by every lane of the wave.
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Block sum for blockDim.x == kBlock; result valid in every thread.
 // `red` is LDS scratch of at least kBlock / kWave doubles.
 __device__ inline double block_sum(double v, double *red)

@@ -90,6 +90,141 @@ __device__ inline void back_subst_32(const double *Dm, double *Nm, const int *pe
     for (int i = 0; i < N; ++i) Nm[i + j * N] = x[i];
 }
 
+// The 32 x 32 exponential of a scaled matrix with lower bandwidth <= 1 (the
+// Krylov Hessenberg) by the first wavefront, every matrix held by columns in
+// registers: lane j and lane 32 + j hold column j of the Pade powers, lane j
+// accumulates column j of D and lane 32 + j column j of N, so that [D | N]
+// is one column per lane for the elimination.  Values another lane needs
+// (the pivot column, the triangular factor, F for a squaring) go through LDS
+// with wavefront-scope fences, no workgroup barrier.  Every value follows the
+// operation order of the oracle's orc_expm (products with l ascending from
+// 0.0, the elimination row by row, back substitution with k ascending, the
+// squarings), skipping only terms that are exact zeros: A(i, l) = 0 for
+// l < i - 1, and the rows more than 6 below the pivot (D has lower
+// bandwidth 6).  A: the scaled matrix (column-major, LDS); At, Ur, Fr: LDS
+// scratch of N x N; the result goes to Nm (column-major).
+__device__ __forceinline__ void expm_wave32(const double *A, double *At, double *Ur, double *Fr,
+                                   double *Nm, int js)
+{
+    constexpr int N = 32, q = 6, BW = 6;
+    __shared__ double colbuf[2][BW + 1];
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) At[(e % N) * N + e / N] = A[e];
+    __syncthreads();
+    if (threadIdx.x < kWave) {
+        const int lane = threadIdx.x;
+        const int j = lane & (N - 1);
+        const bool nside = lane >= N;
+        double x[N], acc[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            x[i] = (i == j) ? 1.0 : 0.0;
+            acc[i] = x[i];
+        }
+        // Pade polynomials: X = A^kk, N += c X, D += (-1)^kk c X.  X <- A X
+        // in place: row i of the product reads x[i-1..], so x[i-1] takes its
+        // new value once row i is formed.
+        double c = 1.0;
+        for (int kk = 1; kk <= q; ++kk) {
+            // keeps the loads of A inside the loop (hoisted, their 559 values
+            // would not fit in registers)
+            wave_sync();
+            c = c * (double)(q - kk + 1) / (double)((2 * q - kk + 1) * kk);
+            const double sgn = (kk % 2) ? -1.0 : 1.0;
+            const double cf = nside ? c : sgn * c;
+            double prev = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double sum = 0.0;
+#pragma unroll
+                for (int l = (i > 0 ? i - 1 : 0); l < N; ++l) sum += At[i * N + l] * x[l];
+                if (i > 0) x[i - 1] = prev;
+                prev = sum;
+                acc[i] += cf * sum;
+                // one row's loads in flight at a time (bounded registers)
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            x[N - 1] = prev;
+        }
+        // D F = N by Gaussian elimination with partial pivoting
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const int kend = (k + BW < N) ? k + BW : N - 1;
+            double *cb = colbuf[k & 1];
+            if (lane == k) {
+#pragma unroll
+                for (int i = k; i <= kend; ++i) cb[i - k] = acc[i];
+            }
+            wave_sync();
+            // first row of maximal |D(i, k)|, k <= i <= kend (zeros below)
+            double best = fabs(cb[0]);
+            int p = k;
+#pragma unroll
+            for (int i = k + 1; i <= kend; ++i) {
+                const double v = fabs(cb[i - k]);
+                if (v > best) {
+                    best = v;
+                    p = i;
+                }
+            }
+#pragma unroll
+            for (int i = k + 1; i <= kend; ++i) {
+                if (p == i) {
+                    const double tmp = acc[k];
+                    acc[k] = acc[i];
+                    acc[i] = tmp;
+                }
+            }
+            const double pv = cb[p - k];
+#pragma unroll
+            for (int i = k + 1; i <= kend; ++i) {
+                const double dik = (i == p) ? cb[0] : cb[i - k];  // D(i, k) after the exchange
+                const double l = dik / pv;
+                if (nside || j > k) acc[i] = acc[i] - l * acc[k];
+            }
+        }
+        // back substitution U F = N with U the upper triangle of D (LDS,
+        // row-major); every lane runs it, the N lanes keep F
+        if (!nside) {
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if (i <= j) Ur[i * N + j] = acc[i];
+        }
+        wave_sync();
+#pragma unroll
+        for (int i = N - 1; i >= 0; --i) {
+            double a = acc[i];
+#pragma unroll
+            for (int cc = i + 1; cc < N; ++cc) a -= Ur[i * N + cc] * acc[cc];
+            acc[i] = a / Ur[i * N + i];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // squarings F <- F F (x holds the product)
+        for (int sq = 0; sq < js; ++sq) {
+            if (nside) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) Fr[i * N + j] = acc[i];
+            }
+            wave_sync();
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double sum = 0.0;
+#pragma unroll
+                for (int l = 0; l < N; ++l) sum += Fr[i * N + l] * acc[l];
+                x[i] = sum;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            wave_sync();
+#pragma unroll
+            for (int i = 0; i < N; ++i) acc[i] = x[i];
+        }
+        if (nside) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) Nm[i + j * N] = acc[i];
+        }
+    }
+    __syncthreads();
+}
+
 // --------------------------------------------------------------- dense expm
 // Golub & Van Loan, Matrix Computations, Alg. 11.3.1 (diagonal Pade q = 6,
 // scaling by 2^-j with j = max(0, 1 + floor(log2 |A|_inf)), Gaussian
@@ -98,7 +233,8 @@ __device__ inline void back_subst_32(const double *Dm, double *Nm, const int *pe
 // oracle's orc_expm, so the two agree bit for bit on equal input.  Result in
 // Nm.
 //
-// Work that only adds exact zeros is skipped.  With b the lower bandwidth
+// A 32 x 32 Hessenberg input (the Krylov case) goes to expm_wave32.  Work
+// that only adds exact zeros is skipped.  With b the lower bandwidth
 // of A (1 for the Krylov Hessenberg), A^k has lower bandwidth k b and the
 // Pade denominator D lower bandwidth 6 b; elimination with partial pivoting
 // keeps that bandwidth in the multipliers (the rows below it hold zeros in
@@ -106,7 +242,7 @@ __device__ inline void back_subst_32(const double *Dm, double *Nm, const int *pe
 // zeros), so each step searches and updates only the 6 b rows under the
 // pivot.  Row exchanges are a logical permutation (perm), so a step needs
 // one barrier.
-__device__ inline void expm_block(int N, double *A, double *X, double *T, double *Nm, double *Dm,
+__device__ __forceinline__ void expm_block(int N, double *A, double *X, double *T, double *Nm, double *Dm,
                                   double *red)
 {
     (void)red;
@@ -155,6 +291,10 @@ __device__ inline void expm_block(int N, double *A, double *X, double *T, double
     }
     for (int i = tid; i < N; i += blockDim.x) perm[0][i] = i;
     __syncthreads();
+    if (N == 32 && lb <= 1 && blockDim.x >= kWave) {
+        expm_wave32(A, T, X, Dm, Nm, js);
+        return;
+    }
     const int q = 6;
     double c = 1.0;
     for (int kk = 1; kk <= q; ++kk) {

@@ -298,6 +298,121 @@ __device__ inline double kron_row_plan(const KronF<V> &A, const KronPlan &P, uin
     return y;
 }
 
+// ------------------------------------------------------ KRON, slot form
+// The reference's Kronecker apply sums over reactions (per row R factor
+// products).  Grouped by net change (the HYB slots), the same operator is
+//   y_i = sum_u [x_i - nu_u in box] h_u(x_{s_u} - nu_{u,s_u}) x_{i - delta_u}
+//         - (sum_u h_u(x_{s_u})) x_i,
+// with h_u(k) = sum_{r in slot u} c_r f_{r,s_u}(k) when every member's
+// propensity varies with one species s_u at most (its other factors are 1):
+// per row, U slot lookups in place of R products.  The tables depend on the
+// coefficients and are rebuilt with them; a model with a slot whose members
+// vary with different species, or with tables too large for LDS, keeps
+// kron_row_plan.  The sink collects sum_u [x + nu_u leaves] h_u(x_{s_u}).
+constexpr int kKronSlots = 24;
+struct KronSlot {
+    int delta;                 // inflow source of row i: i - delta
+    int sp;                    // species the table runs over, -1: constant
+    int nu_sp;                 // net change of that species
+    int toff;                  // table offset
+    int nu[kMaxSpecies];       // net change per species
+};
+struct KronSlotPlan {
+    int U;
+    int tab_size;              // table entries
+    int radix[kMaxSpecies];
+    KronSlot u[kKronSlots];
+};
+
+template <int D, typename V, class X>
+__device__ inline double kron_row_slot(const KronF<V> &A, const KronSlotPlan &P, uint32_t i,
+                                       const X &x, const V *tab, dd &sink)
+{
+    if (i >= A.n) return 0.0;
+    int xs[D];
+    decode<D>(i, A.dm, xs);
+    const double xh = x(i);
+    double acc = 0.0, dsum = 0.0, out = 0.0;
+    for (int u = 0; u < P.U; ++u) {
+        const KronSlot q = P.u[u];  // the whole record in scalar registers
+        bool in = true, leaves = false;
+        int k = 0;
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            if (q.nu[s] != 0) {
+                const int src = xs[s] - q.nu[s], tgt = xs[s] + q.nu[s];
+                in = in && src >= 0 && src < P.radix[s];
+                leaves = leaves || tgt < 0 || tgt >= P.radix[s];
+            }
+            if (s == q.sp) k = xs[s];
+        }
+        const int ks = q.sp >= 0 ? min(max(k - q.nu_sp, 0), P.radix[q.sp < 0 ? 0 : q.sp] - 1) : 0;
+        const double hx = (double)tab[q.toff + k];
+        const double hs = (double)tab[q.toff + ks];
+        const double xsrc = x(in ? (uint32_t)((int64_t)i - q.delta) : i);
+        acc += (in ? hs : 0.0) * xsrc;
+        dsum += hx;
+        if (A.with_sink && leaves) out += hx;
+    }
+    if (A.with_sink) sink = dd_add_prod(sink, out, xh);
+    return acc - dsum * xh;
+}
+
+// Slot plan of a model (false: some slot's members vary with different
+// species, more than kKronSlots slots, or no factor tables).
+inline bool kron_slot_plan(const HostModel &hm, KronSlotPlan &P)
+{
+    const DevModel &dm = hm.dm;
+    if (!hm.has_factors || dm.U > kKronSlots) return false;
+    P = KronSlotPlan{};
+    P.U = dm.U;
+    for (int s = 0; s < kMaxSpecies; ++s) P.radix[s] = s < dm.d ? dm.radix[s] : 1;
+    int off = 0;
+    for (int u = 0; u < dm.U; ++u) {
+        KronSlot &q = P.u[u];
+        q.delta = (int)dm.slot_delta[u];
+        q.sp = -1;
+        for (int m = dm.slot_ptr[u]; m < dm.slot_ptr[u + 1]; ++m) {
+            const RxnDev &rx = dm.rx[dm.slot_rxn[m]];
+            for (int k = 0; k < rx.nsp; ++k) {
+                q.nu[rx.sp[k]] = rx.nu[k];
+                if (rx.foff[k] < 0) continue;
+                if (q.sp >= 0 && q.sp != rx.sp[k]) return false;
+                q.sp = rx.sp[k];
+            }
+        }
+        q.nu_sp = q.sp >= 0 ? q.nu[q.sp] : 0;
+        q.toff = off;
+        off += q.sp >= 0 ? dm.radix[q.sp] : 1;
+    }
+    P.tab_size = off;
+    return true;
+}
+
+// h_u(k) = sum over the slot's members in reaction order of c_r f_{r,s_u}(k)
+// (f = 1 for a member without a factor in s_u), from 0.0.
+inline void kron_slot_tables(const HostModel &hm, const KronSlotPlan &P, const double *coefs,
+                             std::vector<double> &tab)
+{
+    const DevModel &dm = hm.dm;
+    tab.assign((size_t)P.tab_size, 0.0);
+    for (int u = 0; u < P.U; ++u) {
+        const KronSlot &q = P.u[u];
+        const int len = q.sp >= 0 ? dm.radix[q.sp] : 1;
+        for (int m = dm.slot_ptr[u]; m < dm.slot_ptr[u + 1]; ++m) {
+            const int r = dm.slot_rxn[m];
+            const RxnDev &rx = dm.rx[r];
+            int fo = -1;
+            for (int k = 0; k < rx.nsp; ++k)
+                if (rx.sp[k] == q.sp) fo = rx.foff[k];
+            for (int k = 0; k < len; ++k) {
+                const double f = fo >= 0 ? hm.compact[(size_t)fo + k] : 1.0;
+                tab[(size_t)q.toff + k] += coefs[r] * f;
+            }
+        }
+    }
+}
+
 // Transpose (adjoint) row: (A^T x)_j = sum_r c_r a_r(x_j) (x_{j+delta_r}
 // [target in box] - x_j).  No sink column (the sink is absorbing).
 template <int D, typename V, class X>

@@ -137,6 +137,11 @@ struct MatImpl {
     DevModel *d_model = nullptr;
     double *d_compact = nullptr;     // compact factor table (FP64)
     float *d_compact_f = nullptr;    // same, rounded to FP32 (mixed KRON)
+    // KRON slot form (one coefficient-weighted factor table per unique net
+    // change), when the model allows it; rebuilt with the coefficients
+    int kron_slots = 0;
+    double *d_slot_tab = nullptr;
+    float *d_slot_tab_f = nullptr;   // same, rounded to FP32 (mixed)
     double *d_coefs = nullptr;       // R coefficients (device copy)
     // CSR (rows 0..n-1) + sink row stored separately in block order
     int *rp = nullptr;

@@ -436,6 +436,30 @@ __global__ void __launch_bounds__(kBlock) k_arn_kron_plan(KronF<V> A, KronPlan P
     arn_finish<B>(a, nrm, sink, red);
 }
 
+// k_arn_kron with the slot form (kron_row_slot); slot tables in LDS.
+template <int D, typename V, typename B>
+__global__ void __launch_bounds__(kBlock) k_arn_kron_slot(KronF<V> A, KronSlotPlan P,
+                                                          const V *__restrict__ tab_g, ArnArgs a)
+{
+    if (arn_skip<B>(a)) return;
+    extern __shared__ double smem[];
+    double *red = smem;
+    V *tabl = (V *)(red + kArnRedLen);
+    stage_table(tabl, tab_g, P.tab_size);
+    const V *tab = tabl;
+    double sig;
+    if (!arn_sigma(a, red, sig)) return;
+    dd nrm = dd_zero();
+    const KryX<B> X{a.in, sig, a.j};
+    dd sink = dd_zero();
+    for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
+        const uint32_t i = w.t * kBlock + threadIdx.x;
+        const double y = kron_row_slot<D, V>(A, P, i, X, tab, sink);
+        arn_epi<B>(a, i, y, i < a.n ? X(i) : 0.0, nrm);
+    }
+    arn_finish<B>(a, nrm, sink, red);
+}
+
 #define FSP_SWITCH_NK(nk, CALL)                                                \
     if ((nk) <= 8) {                                                           \
         constexpr int NK_ = 8;                                                 \
@@ -882,8 +906,20 @@ static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
             launch_arn_hyb<double, B>(hyb_view<double>(M), a, E.G, red_b, E.s);
     } else {
         KronPlan P;
+        KronSlotPlan SP;
         const bool plan0 = !M.hm.prop_table_dev && kron_plan(M.hm, P);
-        if (mixed && M.hm.has_factors) {
+        const bool slots = M.kron_slots && kron_slot_plan(M.hm, SP);
+        if (slots && mixed) {
+            KronF<float> A = kron_view<float>(M, -1, 0.0);
+            const size_t lds = red_b + sizeof(float) * (size_t)SP.tab_size;
+            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron_slot<D_, float, B><<<E.G, kBlock, lds, E.s>>>(
+                                        A, SP, M.d_slot_tab_f, a)));
+        } else if (slots) {
+            KronF<double> A = kron_view<double>(M, -1, 0.0);
+            const size_t lds = red_b + sizeof(double) * (size_t)SP.tab_size;
+            FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron_slot<D_, double, B><<<E.G, kBlock, lds, E.s>>>(
+                                        A, SP, M.d_slot_tab, a)));
+        } else if (mixed && M.hm.has_factors) {
             KronF<float> A = kron_view<float>(M, -1, 0.0);
             const size_t tb = sizeof(float) * (size_t)A.tab_size;
             const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;

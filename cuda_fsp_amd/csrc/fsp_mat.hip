@@ -106,7 +106,7 @@ fsp_status scan_counts(const int *d_counts, int *d_out, int64_t n, int64_t *h_to
     FSP_HIP(hipMemcpyAsync(&total, tmp + tiles, sizeof(long long), hipMemcpyDeviceToHost, s));
     FSP_HIP(hipStreamSynchronize(s));
     FSP_HIP(hipGetLastError());
-    hipFree(tmp);
+    (void)hipFree(tmp);
     *h_total = total;
     if (total > 2147483647LL) return fail(FSP_ERR_OVERFLOW, "more than 2^31-1 nonzeros");
     return FSP_OK;
@@ -347,6 +347,21 @@ static fsp_status fill_values(MatImpl &M, hipStream_t s)
     return FSP_OK;
 }
 
+// KRON slot tables for the current coefficients (slot form only).
+static fsp_status upload_slot_tables(MatImpl &M, hipStream_t s)
+{
+    KronSlotPlan P;
+    if (!M.kron_slots || !kron_slot_plan(M.hm, P)) return FSP_OK;
+    std::vector<double> tab;
+    kron_slot_tables(M.hm, P, M.coefs.data(), tab);
+    FSP_HIP(copy_sync(M.d_slot_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, s));
+    if (M.d_slot_tab_f) {
+        std::vector<float> f(tab.begin(), tab.end());
+        FSP_HIP(copy_sync(M.d_slot_tab_f, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice, s));
+    }
+    return FSP_OK;
+}
+
 fsp_status mat_assemble(MatImpl &M, hipStream_t s)
 {
     const DevModel &dm = M.hm.dm;
@@ -388,8 +403,8 @@ fsp_status mat_assemble(MatImpl &M, hipStream_t s)
             FSP_HIP(copy_sync(h.data(), cnt, sizeof(int) * (size_t)M.n, hipMemcpyDeviceToHost, s));
             M.max_row = M.n ? *std::max_element(h.begin(), h.end()) : 0;
         }
-        hipFree(cnt);
-        if (scnt) hipFree(scnt);
+        (void)hipFree(cnt);
+        if (scnt) (void)hipFree(scnt);
         const size_t vb = mixed ? sizeof(float) : sizeof(double);
         FSP_HIP(hipMalloc(&M.cols, sizeof(int) * (size_t)std::max<int64_t>(M.nnz_rows, 1)));
         FSP_HIP(hipMalloc(&M.vals, vb * (size_t)std::max<int64_t>(M.nnz_rows, 1)));
@@ -417,6 +432,14 @@ fsp_status mat_assemble(MatImpl &M, hipStream_t s)
             FSP_HIP(copy_sync(M.d_compact_f, f.data(), sizeof(float) * f.size(),
                               hipMemcpyHostToDevice, s));
         }
+        KronSlotPlan P;
+        if (!M.hm.prop_table_dev && kron_slot_plan(M.hm, P) &&
+            sizeof(double) * (size_t)P.tab_size <= kLdsTableMax) {
+            M.kron_slots = 1;
+            FSP_HIP(hipMalloc(&M.d_slot_tab, sizeof(double) * (size_t)P.tab_size));
+            if (mixed) FSP_HIP(hipMalloc(&M.d_slot_tab_f, sizeof(float) * (size_t)P.tab_size));
+            FSP_TRY(upload_slot_tables(M, s));
+        }
         M.nnz_rows = 0;
     }
     FSP_TRY(fill_values(M, s));
@@ -435,7 +458,7 @@ fsp_status mat_revalue(MatImpl &M, const double *coefs, hipStream_t s)
                                hipMemcpyHostToDevice, s));
         // the host copy must stay valid until the copy ran
         FSP_HIP(hipStreamSynchronize(s));
-        return FSP_OK;
+        return upload_slot_tables(M, s);
     }
     FSP_TRY(fill_values(M, s));
     FSP_HIP(hipStreamSynchronize(s));
@@ -595,6 +618,28 @@ __global__ void __launch_bounds__(kBlock) k_apply_kron_plan(KronF<V> A, KronPlan
     if (A.with_sink) put_dd_partial(partials, sink, red);
 }
 
+// Slot form of the Kronecker apply (kron_row_slot); the slot tables are
+// read from LDS.
+template <int D, typename V>
+__global__ void __launch_bounds__(kBlock) k_apply_kron_slot(KronF<V> A, KronSlotPlan P,
+                                                            const V *__restrict__ tab_g, PlainX X,
+                                                            double *__restrict__ y, int accumulate,
+                                                            uint32_t tiles,
+                                                            double *__restrict__ partials)
+{
+    extern __shared__ double smem[];
+    __shared__ double red[2 * kBlock / kWave];
+    stage_table((V *)smem, tab_g, P.tab_size);
+    const V *tab = (const V *)smem;
+    dd sink = dd_zero();
+    for (TileWalk w = tile_walk(tiles); w.t < w.end; w.t += w.step) {
+        const uint32_t i = w.t * kBlock + threadIdx.x;
+        const double v = kron_row_slot<D, V>(A, P, i, X, tab, sink);
+        if (i < A.n) y[i] = accumulate ? y[i] + v : v;
+    }
+    if (A.with_sink) put_dd_partial(partials, sink, red);
+}
+
 // Diagonal of the generator: -sum_r c_r a_r(x_i) over the reactions that
 // change the state (the whole outflow, including outflow leaving the box).
 template <int D, typename V>
@@ -653,7 +698,13 @@ static fsp_status launch_kron(const MatImpl &M, const double *x, double *y, int 
     const int with_sink = M.with_sink && !transpose && single < 0;
     A.with_sink = with_sink;
     KronPlan P;
-    if (!transpose && single < 0 && !A.prop_table && use_lds && kron_plan(M.hm, P)) {
+    KronSlotPlan SP;
+    if (!transpose && single < 0 && M.kron_slots && kron_slot_plan(M.hm, SP)) {
+        const V *st = std::is_same<V, float>::value ? (const V *)(const void *)M.d_slot_tab_f
+                                                    : (const V *)(const void *)M.d_slot_tab;
+        FSP_SWITCH_D(M.hm.dm.d, (k_apply_kron_slot<D_, V><<<grid, kBlock, sizeof(V) * SP.tab_size, s>>>(
+                                    A, SP, st, PlainX{x}, y, accumulate, tiles, M.partials)));
+    } else if (!transpose && single < 0 && !A.prop_table && use_lds && kron_plan(M.hm, P)) {
         FSP_SWITCH_D(M.hm.dm.d, (k_apply_kron_plan<D_, V><<<grid, kBlock, tb, s>>>(
                                     A, P, PlainX{x}, y, accumulate, tiles, M.partials)));
     } else {
@@ -746,9 +797,10 @@ void mat_free(MatImpl &M)
 {
     void *ptrs[] = {M.d_model, M.d_compact, M.d_compact_f, M.d_coefs, M.rp, M.cols, M.vals,
                     M.sink_cols, M.sink_vals, M.sink_block_ptr, M.diag, M.slot_vals,
-                    M.slot_off, M.sink_out, M.partials, M.ticket};
+                    M.slot_off, M.sink_out, M.partials, M.ticket, M.d_slot_tab,
+                    M.d_slot_tab_f};
     for (void *p : ptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
 }
 
 double mat_bytes_per_apply(const MatImpl &M)
@@ -813,10 +865,10 @@ fsp_status term_csr(const HostModel &hm, int r, int *d_rp, int *d_cols, double *
         FSP_HIP(hipStreamSynchronize(s));
     }
     *h_nnz = total;
-    hipFree(cnt);
-    if (rp != d_rp) hipFree(rp);
-    hipFree(d_model);
-    if (d_tab) hipFree(d_tab);
+    (void)hipFree(cnt);
+    if (rp != d_rp) (void)hipFree(rp);
+    (void)hipFree(d_model);
+    if (d_tab) (void)hipFree(d_tab);
     return st;
 }
 

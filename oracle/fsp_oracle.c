@@ -645,6 +645,158 @@ void orc_kron_apply(const orc_model *m, const double *coefs, int with_sink,
     geom_free(&g);
 }
 
+/* The same operator grouped by net change: this project's KRON slot form
+ * (the reference sums per reaction, orc_kron_apply above; the two agree to
+ * rounding).  Slot u is a unique net change (first appearance order) with
+ * its member reactions in reaction order; when every member's propensity
+ * varies with one species s_u at most (factor tables not identically 1),
+ *   h_u(k) = sum_{r in slot u} c_r f_{r,s_u}(k)      (from 0.0, member order)
+ *   y_i += acc - dsum x_i,  acc = sum_u [x_i - nu_u in box] h_u(x_{s_u} -
+ *   nu_{u,s_u}) x_{i - delta_u},  dsum = sum_u h_u(x_{s_u})   (slot order),
+ * and the sink collects sum_u [x_i + nu_u leaves the box] h_u(x_{s_u}).
+ * orc_kron_slot_form returns the number of table entries, or -1 when the
+ * form does not apply (a slot whose members vary with different species,
+ * more than 24 slots, or tables over 96 KiB), as on the device. */
+typedef struct {
+    int U;
+    int *slot;       /* R: slot of each reaction, -1 for zero net change */
+    int *sp;         /* U: table species, -1 constant */
+    int *toff;       /* U + 1 */
+    int *first;      /* U: first member (its net change is the slot's) */
+} kron_slots;
+
+static int factor_trivial(const orc_model *m, int r, int s)
+{
+    int stride = factor_stride(m), off = 0;
+    for (int t = 0; t < s; ++t) off += m->bounds[t] + 1;
+    const double *f = m->factors + (int64_t)r * stride + off;
+    for (int k = 0; k <= m->bounds[s]; ++k)
+        if (f[k] != 1.0) return 0;
+    return 1;
+}
+
+static int slots_init(kron_slots *ks, const orc_model *m, const model_geom *g)
+{
+    ks->slot = (int *)malloc(sizeof(int) * (size_t)(g->R + 1));
+    ks->U = orc_unique_shifts(m, ks->slot);
+    ks->sp = (int *)malloc(sizeof(int) * (size_t)(ks->U + 1));
+    ks->toff = (int *)malloc(sizeof(int) * (size_t)(ks->U + 2));
+    ks->first = (int *)malloc(sizeof(int) * (size_t)(ks->U + 1));
+    if (!m->factors || ks->U > 24) return -1;
+    int off = 0;
+    for (int u = 0; u < ks->U; ++u) {
+        ks->sp[u] = -1;
+        ks->first[u] = -1;
+        for (int r = 0; r < g->R; ++r) {
+            if (ks->slot[r] != u) continue;
+            if (ks->first[u] < 0) ks->first[u] = r;
+            for (int s = 0; s < g->d; ++s) {
+                if (factor_trivial(m, r, s)) continue;
+                if (ks->sp[u] >= 0 && ks->sp[u] != s) return -1;
+                ks->sp[u] = s;
+            }
+        }
+        ks->toff[u] = off;
+        off += ks->sp[u] >= 0 ? m->bounds[ks->sp[u]] + 1 : 1;
+    }
+    ks->toff[ks->U] = off;
+    if ((int64_t)off * 8 > 96 * 1024) return -1;
+    return off;
+}
+
+static void slots_free(kron_slots *ks)
+{
+    free(ks->slot); free(ks->sp); free(ks->toff); free(ks->first);
+}
+
+int orc_kron_slot_form(const orc_model *m)
+{
+    model_geom g;
+    geom_init(&g, m);
+    kron_slots ks;
+    int rc = slots_init(&ks, m, &g);
+    slots_free(&ks);
+    geom_free(&g);
+    return rc;
+}
+
+int orc_kron_slot_apply(const orc_model *m, const double *coefs, int with_sink,
+                        const double *x, double *y)
+{
+    model_geom g;
+    geom_init(&g, m);
+    kron_slots ks;
+    const int size = slots_init(&ks, m, &g);
+    if (size < 0) {
+        slots_free(&ks);
+        geom_free(&g);
+        return -1;
+    }
+    const int d = g.d, U = ks.U, stride = factor_stride(m);
+    const int64_t n = g.n;
+    /* the tables, member by member in reaction order */
+    double *tab = (double *)calloc((size_t)size + 1, sizeof(double));
+    for (int u = 0; u < U; ++u) {
+        const int s = ks.sp[u];
+        const int len = s >= 0 ? m->bounds[s] + 1 : 1;
+        int off = 0;
+        for (int t = 0; t < s; ++t) off += m->bounds[t] + 1;
+        for (int r = 0; r < g.R; ++r) {
+            if (ks.slot[r] != u) continue;
+            const int triv = s < 0 || factor_trivial(m, r, s);
+            const double *f = m->factors + (int64_t)r * stride + off;
+            for (int k = 0; k < len; ++k) tab[ks.toff[u] + k] += coefs[r] * (triv ? 1.0 : f[k]);
+        }
+    }
+    /* slot shifts */
+    int64_t *delta = (int64_t *)malloc(sizeof(int64_t) * (size_t)(U + 1));
+    for (int u = 0; u < U; ++u) {
+        const int *nu = &g.nu[ks.first[u] * d];
+        int64_t dl = 0, mul = 1;
+        for (int s = 0; s < d; ++s) { dl += (int64_t)nu[s] * mul; mul *= m->bounds[s] + 1; }
+        delta[u] = dl;
+    }
+    double *outv = with_sink ? (double *)malloc(sizeof(double) * (size_t)n) : NULL;
+#pragma omp parallel num_threads(g_threads)
+    {
+        int xi[64];
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            orc_indx2state(i, xi, d, m->bounds);
+            double acc = 0.0, dsum = 0.0, out = 0.0;
+            for (int u = 0; u < U; ++u) {
+                const int *nu = &g.nu[ks.first[u] * d];
+                int in = 1, leaves = 0;
+                for (int s = 0; s < d; ++s) {
+                    if (nu[s] == 0) continue;
+                    const int src = xi[s] - nu[s], tgt = xi[s] + nu[s];
+                    if (src < 0 || src > m->bounds[s]) in = 0;
+                    if (tgt < 0 || tgt > m->bounds[s]) leaves = 1;
+                }
+                const int s = ks.sp[u];
+                const double hx = tab[ks.toff[u] + (s >= 0 ? xi[s] : 0)];
+                if (in) {
+                    const double hs = tab[ks.toff[u] + (s >= 0 ? xi[s] - nu[s] : 0)];
+                    acc += hs * x[i - delta[u]];
+                }
+                dsum += hx;
+                if (leaves) out += hx;
+            }
+            y[i] += acc - dsum * x[i];
+            if (outv) outv[i] = out;
+        }
+    }
+    if (with_sink) {
+        y[n] += dot_dd(n, outv, x);
+        free(outv);
+    }
+    free(delta);
+    free(tab);
+    slots_free(&ks);
+    geom_free(&g);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Dense expm: Golub & Van Loan Algorithm 11.3.1 (Pade q = 6)           */
 /* ------------------------------------------------------------------ */
@@ -1031,7 +1183,10 @@ void orc_op_apply(const orc_op *op, const double *x, double *y)
     } else if (op->kind == 1) {
         orc_ell_apply(op->n_rows - (op->sink_out ? 1 : 0), op->U, op->diag, op->slot_vals,
                       op->slot_cols, op->sink_out, x, y);
-    } else {
+    } else if (op->kind == 3 ||
+               orc_kron_slot_apply(op->model, op->coefs, op->with_sink, x, y) != 0) {
+        /* kind 2, the device's KRON form: by slot where it applies, else per
+         * reaction; kind 3: per reaction (the reference's formula) */
         orc_kron_apply(op->model, op->coefs, op->with_sink, x, y);
     }
 }

@@ -105,6 +105,12 @@ void orc_hyb_ref_apply(int64_t n, int R, const double *diag,
  * length n+1 and y[n] += sum_i out_i x_i. */
 void orc_kron_apply(const orc_model *m, const double *coefs, int with_sink,
                     const double *x, double *y);
+/* The same operator grouped by net change (the device's KRON slot form, see
+ * fsp_oracle.c): returns -1 and leaves y when the form does not apply.
+ * orc_kron_slot_form: table entries, or -1. */
+int orc_kron_slot_form(const orc_model *m);
+int orc_kron_slot_apply(const orc_model *m, const double *coefs, int with_sink,
+                        const double *x, double *y);
 
 /* Dense matrix exponential, Golub & Van Loan, Matrix Computations (3rd ed.)
  * Algorithm 11.3.1: diagonal Pade q = 6 with power-of-two scaling by the
@@ -160,7 +166,8 @@ int orc_expv_csr_outputs(int64_t n, const int *row_ptrs, const int *cols,
                          int n_replay, orc_expv_stats *stats);
 /* Operators for the expv restatement (y += A x).  kind 0: CSR rows summed
  * in column order (dd_last: the last row is the sink row, summed
- * order-independently); 1: this project's ELL/HYB layout; 2: Kronecker.
+ * order-independently); 1: this project's ELL/HYB layout; 2: Kronecker in
+ * the device's form (by slot where that applies); 3: Kronecker per reaction.
  * Sums across rows (the sink element) are order-independent in every kind. */
 typedef struct {
     int kind;

@@ -64,6 +64,10 @@ def lib():
         L.orc_hyb_ref.argtypes = [vp, vp, vp, vp]
         L.orc_hyb_ref_apply.argtypes = [i64, ci, vp, vp, vp, vp, vp, vp]
         L.orc_kron_apply.argtypes = [vp, vp, ci, vp, vp]
+        L.orc_kron_slot_form.restype = ci
+        L.orc_kron_slot_form.argtypes = [vp]
+        L.orc_kron_slot_apply.restype = ci
+        L.orc_kron_slot_apply.argtypes = [vp, vp, ci, vp, vp]
         L.orc_expm.restype = ci
         L.orc_expm.argtypes = [ci, vp, vp]
         L.orc_expv_csr.restype = ci
@@ -199,6 +203,22 @@ def kron_apply(net, x, y, coefs=None, with_sink=False):
     lib().orc_kron_apply(m.ptr, _p(c), int(with_sink), _p(x), _p(y))
 
 
+def kron_slot_apply(net, x, y, coefs=None, with_sink=False) -> bool:
+    """y += A x in the slot form of the Kronecker operator (reactions grouped
+    by net change with coefficient-weighted factor tables, see fsp_oracle.c);
+    False (y untouched) where that form does not apply."""
+    m = OracleModel(net)
+    c = np.ascontiguousarray(net.coefs if coefs is None else coefs, np.float64)
+    return lib().orc_kron_slot_apply(m.ptr, _p(c), int(with_sink), _p(x), _p(y)) == 0
+
+
+def kron_apply_device_form(net, x, y, coefs=None, with_sink=False):
+    """y += A x with the row arithmetic of the device's KRON format: the slot
+    form where it applies, else the reference's per-reaction form."""
+    if not kron_slot_apply(net, x, y, coefs, with_sink):
+        kron_apply(net, x, y, coefs, with_sink)
+
+
 def expm(A: np.ndarray) -> np.ndarray:
     A = np.asfortranarray(A, np.float64)
     F = np.empty_like(A, order="F")
@@ -253,7 +273,9 @@ class _Op(C.Structure):
 class Operator:
     """Oracle operator y += A x with the same row arithmetic as one of the
     GPU formats: 'csr' (merged CSR, column order), 'hyb' (diag + ELL slots),
-    'kron' (Kronecker formula).  Keeps its arrays alive."""
+    'kron' (Kronecker, by slot where that form applies), 'kron_reaction'
+    (Kronecker per reaction, the reference's formula, as the persistent
+    batch kernel computes it).  Keeps its arrays alive."""
 
     def __init__(self, net, fmt="csr", with_sink=False, coefs=None):
         c = np.ascontiguousarray(net.coefs if coefs is None else coefs, np.float64)
@@ -273,10 +295,12 @@ class Operator:
             self.keep += [dg, sv, sc, sk]
             op.kind, op.U = 1, U
             op.diag, op.slot_vals, op.slot_cols, op.sink_out = _p(dg), _p(sv), _p(sc), _p(sk)
-        elif fmt == "kron":
+        elif fmt in ("kron", "kron_reaction"):
+            # "kron": the device's KRON row arithmetic (the slot form where it
+            # applies); "kron_reaction": the reference's per-reaction formula
             m = OracleModel(net)
             self.keep.append(m)
-            op.kind, op.with_sink = 2, int(with_sink)
+            op.kind, op.with_sink = (2 if fmt == "kron" else 3), int(with_sink)
             op.model, op.coefs = m.ptr, _p(c)
         else:
             raise ValueError(fmt)

@@ -276,6 +276,11 @@ inline void launch(void (*kernel)(P...), dim3 grid, dim3 block, size_t lds, void
 #define gridDim (::simt::grid_dim())
 
 inline void __syncthreads() { ::simt::syncthreads(); }
+// wavefront-scope fence and barrier: a rendezvous of the calling lane's wave
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
+#define __builtin_amdgcn_sched_barrier(mask) ((void)0)
+#define __builtin_amdgcn_wave_barrier() \
+    ::simt::wave_barrier(::simt::group()->cur->tid / ::simt::kWaveLanes)
 template <typename T>
 inline T __shfl_xor(T v, int mask, int width = 64)
 {

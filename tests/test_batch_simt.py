@@ -45,7 +45,7 @@ def _instances(k, seed=5):
 
 
 def _oracle(net, t, sink=False, p0=None):
-    opr = O.Operator(net, "kron", with_sink=sink)
+    opr = O.Operator(net, "kron_reaction", with_sink=sink)
     v = M.delta_p0(net, with_sink=sink) if p0 is None else p0.copy()
     rc, out, st = O.expv_outputs(opr, v, [t])
     assert rc == 0

@@ -127,9 +127,15 @@ def test_spmv_matches_oracle(mk):
     got = _apply(capi.Mat(net, "hyb", "f64"), x[:n])
     assert np.array_equal(got, w)
     w = np.zeros(n)
-    O.kron_apply(net, x[:n], w)
+    O.kron_apply_device_form(net, x[:n], w)
     got = _apply(capi.Mat(net, "kron", "f64"), x[:n])
     assert np.array_equal(got, w)
+    # with the sink state: box rows bit-exact, the sink to 1e-14
+    w = y0.copy()
+    O.kron_apply_device_form(net, x, w, with_sink=True)
+    got = _apply(capi.Mat(net, "kron", "f64", with_sink=True), x, accumulate=True, y0=y0)
+    assert np.array_equal(got[:n], w[:n])
+    assert abs(got[n] - w[n]) <= 1e-14 * max(1.0, abs(w[n]))
 
 
 def test_reference_fsp_mat_test():

@@ -149,6 +149,49 @@ def test_kronecker_shift_closed_form():
     assert np.array_equal(y.reshape(41, 1025), np.broadcast_to(want, (41, 1025)))
 
 
+SLOT_NETS = [
+    lambda: M.toggle_ref_test((31, 31)),
+    lambda: M.toggle_switch((60, 50)),
+    lambda: M.gene_expression((2, 30, 25)),
+    lambda: M.five_species_chain((3, 5, 6, 4, 7)),
+    lambda: M.birth_death((9, 0, 4)),
+    lambda: M.hog1p((3, 6, 5, 7, 6)).with_coefs(M.hog1p_coefs(5.0)),
+]
+
+
+@pytest.mark.parametrize("mk", SLOT_NETS)
+@pytest.mark.parametrize("sink", [False, True])
+def test_kron_slot_form_equals_per_reaction_form(mk, sink):
+    # the slot form regroups the reference's per-reaction sum by net change:
+    # equal to rounding, and equal to the dense CME definition
+    net = mk()
+    n = net.n_states
+    x = np.random.default_rng(4).random(n + 1)
+    y1, y2 = np.zeros(n + 1), np.zeros(n + 1)
+    assert O.kron_slot_apply(net, x, y1, with_sink=sink)
+    O.kron_apply(net, x, y2, with_sink=sink)
+    scale = np.abs(y2).max()
+    assert np.abs(y1 - y2).max() <= 4e-15 * scale
+    if n <= 2000:
+        A = dense_generator(net, sink=True)
+        want = A @ x if sink else (A[:n, :n] @ x[:n])
+        got = y1 if sink else y1[:n]
+        assert np.abs(got - want).max() <= 1e-13 * scale
+
+
+def test_kron_slot_form_refuses_mixed_species_slots():
+    # a slot whose members vary with different species (x+1 at a rate in x
+    # and at a rate in y) has no single table: the form does not apply
+    net = M.toggle_switch((6, 6))
+    nu = np.vstack([net.nu, [[1, 0]]])
+    f = np.vstack([net.factors, net.factors[1:2]])  # x -> x+1 at rate x
+    bad = M.ReactionNetwork("mixed", net.bounds, nu.astype(np.int32), f,
+                            np.append(net.coefs, 0.5))
+    y = np.zeros(bad.n_states)
+    assert not O.kron_slot_apply(bad, np.ones(bad.n_states), y)
+    assert not y.any()
+
+
 def test_planted_mistake_flipped_net_change_is_caught():
     # an oracle that used x + nu as the inflow source (a sign slip) must fail
     # the definition check above
