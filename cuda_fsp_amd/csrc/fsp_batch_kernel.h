@@ -331,7 +331,7 @@ __device__ void solve_one(const BatchArgs &a, BatchInst &I, double *sm)
 template <int BS, typename B, int NKB, int DMAX>
 __global__ void __launch_bounds__(BS) k_batch(BatchArgs a)
 {
-    HIP_DYNAMIC_SHARED(double, sm)
+    FSP_DYNAMIC_LDS(sm);
     __shared__ int q_sh;
     for (;;) {
         if (threadIdx.x == 0) q_sh = (int)atomicAdd(a.queue, 1u);

@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_gram_coef(LsArgs a)
 {
     KState *st = a.st + blockIdx.x;
     if (!st->active || st->broke) return;
-    HIP_DYNAMIC_SHARED(double, lds)
+    FSP_DYNAMIC_LDS(lds);
     gram_coefs(st, a.istart, a.j, a.M2, lds);
 }
 
@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_norm(LsArgs a)
 
 __global__ void __launch_bounds__(kBlock) k_ls_control(LsArgs a, int max_reject)
 {
-    HIP_DYNAMIC_SHARED(double, smem)
+    FSP_DYNAMIC_LDS(smem);
     kry_control(a.st + blockIdx.x, a.m, max_reject, smem);
 }
 

@@ -8,6 +8,12 @@
 
 #include "fsp_internal.h"
 
+// The dynamic LDS of a kernel as doubles at a 16-byte boundary (the static
+// LDS before it need not end on one), for 128-bit LDS accesses.
+#ifndef FSP_DYNAMIC_LDS
+#define FSP_DYNAMIC_LDS(var) extern __shared__ __attribute__((aligned(16))) double var[]
+#endif
+
 namespace fsp {
 
 constexpr int kWave = 64;

@@ -90,139 +90,220 @@ __device__ inline void back_subst_32(const double *Dm, double *Nm, const int *pe
     for (int i = 0; i < N; ++i) Nm[i + j * N] = x[i];
 }
 
-// The 32 x 32 exponential of a scaled matrix with lower bandwidth <= 1 (the
-// Krylov Hessenberg) by the first wavefront, every matrix held by columns in
-// registers: lane j and lane 32 + j hold column j of the Pade powers, lane j
-// accumulates column j of D and lane 32 + j column j of N, so that [D | N]
-// is one column per lane for the elimination.  Values another lane needs
-// (the pivot column, the triangular factor, F for a squaring) go through LDS
-// with wavefront-scope fences, no workgroup barrier.  Every value follows the
-// operation order of the oracle's orc_expm (products with l ascending from
-// 0.0, the elimination row by row, back substitution with k ascending, the
-// squarings), skipping only terms that are exact zeros: A(i, l) = 0 for
-// l < i - 1, and the rows more than 6 below the pivot (D has lower
-// bandwidth 6).  A: the scaled matrix (column-major, LDS); At, Ur, Fr: LDS
-// scratch of N x N; the result goes to Nm (column-major).
-__device__ __forceinline__ void expm_wave32(const double *A, double *At, double *Ur, double *Fr,
-                                   double *Nm, int js)
+// A pair of doubles at a 16-byte boundary: one 128-bit LDS access.
+struct alignas(16) dpair {
+    double x, y;
+};
+
+// Lane `lane`'s value of v (lane uniform across the wave): two 32-bit
+// readlanes, no memory.
+__device__ __forceinline__ double readlane_f64(double v, int lane)
 {
-    constexpr int N = 32, q = 6, BW = 6;
-    __shared__ double colbuf[2][BW + 1];
-    for (int e = threadIdx.x; e < N * N; e += blockDim.x) At[(e % N) * N + e / N] = A[e];
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// The 2 x 2 block (i0..i0+1, j0..j0+1) of the 32 x 32 product P Q with P
+// column-major (P(i, l) at P[i + 32 l]) and Q by rows of stride qs (Q(l, j)
+// at Q[qs l + j]); every element is sum_l P(i, l) Q(l, j) over all 32 l in
+// increasing order from 0.0, the oracle's matmul_cm.  t = {(i0, j0),
+// (i0, j0+1), (i0+1, j0), (i0+1, j0+1)}.
+__device__ __forceinline__ void prod32_2x2(const double *P, const double *Q, int qs, int i0, int j0,
+                                           double (&t)[4])
+{
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+#pragma unroll 8
+    for (int l = 0; l < 32; ++l) {
+        const dpair a = *reinterpret_cast<const dpair *>(P + i0 + 32 * l);
+        const dpair b = *reinterpret_cast<const dpair *>(Q + qs * l + j0);
+        t0 += a.x * b.x;
+        t1 += a.x * b.y;
+        t2 += a.y * b.x;
+        t3 += a.y * b.y;
+    }
+    t[0] = t0;
+    t[1] = t1;
+    t[2] = t2;
+    t[3] = t3;
+}
+
+// Stores a 2 x 2 block by rows of stride rs / column-major (stride 32).
+__device__ __forceinline__ void store_rows_2x2(double *Q, int rs, int i0, int j0, const double (&t)[4])
+{
+    *reinterpret_cast<dpair *>(Q + rs * i0 + j0) = dpair{t[0], t[1]};
+    *reinterpret_cast<dpair *>(Q + rs * (i0 + 1) + j0) = dpair{t[2], t[3]};
+}
+__device__ __forceinline__ void store_cols_2x2(double *P, int i0, int j0, const double (&t)[4])
+{
+    *reinterpret_cast<dpair *>(P + i0 + 32 * j0) = dpair{t[0], t[2]};
+    *reinterpret_cast<dpair *>(P + i0 + 32 * (j0 + 1)) = dpair{t[1], t[3]};
+}
+
+// Step k of the banded elimination of expm32_hess on one wavefront: lane c
+// holds rows k..k+6 of column c of [D | N] (32 x 64, by rows of 64 in XT).
+// The pivot column (lane k's window) comes by readlane; the first row of
+// maximal |D(i, k)| is exchanged with row k in every column; lanes 0..5
+// divide the six multipliers (one division each, broadcast by readlane);
+// rows k+1..k+6 are updated in every column.  Columns c <= k of D are updated
+// too: they hold the multipliers in the oracle and are never read again.
+// Row k, now final, goes to XT and row k + 7 enters the window.  kTail:
+// rows past 31 are skipped (k > 24; k is a constant under unrolling).
+template <bool kTail>
+__device__ __forceinline__ void elim_step32(int k, int lane, double (&w)[7], double *XT)
+{
+    constexpr int N = 32, BW = 6, W = 2 * N;
+    double v[BW + 1];
+#pragma unroll
+    for (int r = 0; r <= BW; ++r) v[r] = (!kTail || k + r < N) ? readlane_f64(w[r], k) : 0.0;
+    int p = 0;
+    double best = fabs(v[0]);
+#pragma unroll
+    for (int r = 1; r <= BW; ++r) {
+        if ((!kTail || k + r < N) && fabs(v[r]) > best) {
+            best = fabs(v[r]);
+            p = r;
+        }
+    }
+    double wp = w[0], num = 0.0;
+#pragma unroll
+    for (int r = 1; r <= BW; ++r) {
+        if (p == r) {
+            wp = w[r];
+            w[r] = w[0];
+        }
+        // D(k + r, k) after the exchange
+        if (lane == r - 1) num = (p == r) ? v[0] : v[r];
+    }
+    w[0] = wp;
+    const double lq = num / readlane_f64(wp, k);
+#pragma unroll
+    for (int r = 1; r <= BW; ++r) {
+        if (!kTail || k + r < N) {
+            const double l = readlane_f64(lq, r - 1);
+            w[r] = w[r] - l * w[0];
+        }
+    }
+    XT[W * k + lane] = w[0];
+#pragma unroll
+    for (int r = 0; r < BW; ++r) w[r] = w[r + 1];
+    w[BW] = (!kTail || k + BW + 1 < N) ? XT[W * (k + BW + 1) + lane] : 0.0;
+}
+
+// The 32 x 32 exponential of a scaled matrix with lower bandwidth <= 1 (the
+// Krylov Hessenberg) by a workgroup of exactly 256 threads.  Every value
+// follows the operation order of the oracle's orc_expm: products sum all 32
+// terms in increasing l from 0.0, the elimination runs row by row, back
+// substitution with k ascending, then the squarings.
+//
+//  * Pade polynomials: thread t owns the 2 x 2 block (2 (t % 16), 2 (t / 16))
+//    of every matrix; its two rows of A stay in registers for the six
+//    products, the powers X = A^k go through LDS by rows, and the blocks of N
+//    and D accumulate in registers.
+//  * D F = N: one wavefront, lane c holding column c of the 32 x 64 matrix
+//    [D | N] (row-major in LDS, so a row is one conflict-free access).  Only
+//    the 7 rows k..k+6 change in step k (D has lower bandwidth 6; the rows
+//    below hold zeros in the pivot column, their multipliers are 0 and their
+//    updates subtract zeros), so the lane keeps them in a register window
+//    that slides down one row per step.  The pivot column comes from lane k
+//    by readlane; lanes 0..5 divide the 6 multipliers, one division each.
+//  * U F = N: lanes 32..63, one column each, U read by rows from LDS.
+//  * Squarings: the 2 x 2 blocks again, F by columns and by rows in LDS.
+//
+// A: the scaled matrix (column-major); XT: 2 N^2 doubles (X then T in the
+// caller's layout) for the powers and then [D | N]; Nm: the result
+// (column-major).  All three 16-byte aligned.
+__device__ __forceinline__ void expm32_hess(const double *A, double *XT, double *Nm, int js)
+{
+    constexpr int N = 32, q = 6, BW = 6, W = 2 * N;
+    const int tid = threadIdx.x;
+    const int i0 = 2 * (tid & 15), j0 = 2 * (tid >> 4);
+    double ar[2][N];  // rows i0, i0 + 1 of A
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+        const dpair a = *reinterpret_cast<const dpair *>(A + i0 + N * l);
+        ar[0][l] = a.x;
+        ar[1][l] = a.y;
+    }
+    double nb[4], db[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const double id = (i0 + (u >> 1) == j0 + (u & 1)) ? 1.0 : 0.0;
+        nb[u] = id;
+        db[u] = id;
+    }
+    // X = I by rows
+    for (int e = tid; e < N * N; e += blockDim.x) XT[e] = ((e >> 5) == (e & 31)) ? 1.0 : 0.0;
     __syncthreads();
-    if (threadIdx.x < kWave) {
-        const int lane = threadIdx.x;
-        const int j = lane & (N - 1);
-        const bool nside = lane >= N;
-        double x[N], acc[N];
+    double c = 1.0;
+    for (int kk = 1; kk <= q; ++kk) {
+        c = c * (double)(q - kk + 1) / (double)((2 * q - kk + 1) * kk);
+        const double sgn = (kk % 2) ? -1.0 : 1.0;
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-            x[i] = (i == j) ? 1.0 : 0.0;
-            acc[i] = x[i];
+        for (int l = 0; l < N; ++l) {
+            const dpair b = *reinterpret_cast<const dpair *>(XT + N * l + j0);
+            t0 += ar[0][l] * b.x;
+            t1 += ar[0][l] * b.y;
+            t2 += ar[1][l] * b.x;
+            t3 += ar[1][l] * b.y;
         }
-        // Pade polynomials: X = A^kk, N += c X, D += (-1)^kk c X.  X <- A X
-        // in place: row i of the product reads x[i-1..], so x[i-1] takes its
-        // new value once row i is formed.
-        double c = 1.0;
-        for (int kk = 1; kk <= q; ++kk) {
-            // keeps the loads of A inside the loop (hoisted, their 559 values
-            // would not fit in registers)
-            wave_sync();
-            c = c * (double)(q - kk + 1) / (double)((2 * q - kk + 1) * kk);
-            const double sgn = (kk % 2) ? -1.0 : 1.0;
-            const double cf = nside ? c : sgn * c;
-            double prev = 0.0;
+        const double t[4] = {t0, t1, t2, t3};
+        __syncthreads();
+        store_rows_2x2(XT, N, i0, j0, t);
 #pragma unroll
-            for (int i = 0; i < N; ++i) {
-                double sum = 0.0;
-#pragma unroll
-                for (int l = (i > 0 ? i - 1 : 0); l < N; ++l) sum += At[i * N + l] * x[l];
-                if (i > 0) x[i - 1] = prev;
-                prev = sum;
-                acc[i] += cf * sum;
-                // one row's loads in flight at a time (bounded registers)
-                __builtin_amdgcn_sched_barrier(0);
-            }
-            x[N - 1] = prev;
+        for (int u = 0; u < 4; ++u) {
+            nb[u] += c * t[u];
+            db[u] += sgn * c * t[u];
         }
-        // D F = N by Gaussian elimination with partial pivoting
+        __syncthreads();
+    }
+    // [D | N] by rows of W = 64 over XT
+    store_rows_2x2(XT, W, i0, j0, db);
+    store_rows_2x2(XT + N, W, i0, j0, nb);
+    __syncthreads();
+    if (tid < kWave) {
+        const int lane = tid;
+        double w[BW + 1];  // rows k..k+6 of column `lane`
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-            const int kend = (k + BW < N) ? k + BW : N - 1;
-            double *cb = colbuf[k & 1];
-            if (lane == k) {
+        for (int r = 0; r <= BW; ++r) w[r] = XT[W * r + lane];
+        // steps with all six rows below the pivot and row k + 7 in the matrix
+        for (int k = 0; k < N - BW - 1; ++k) elim_step32<false>(k, lane, w, XT);
 #pragma unroll
-                for (int i = k; i <= kend; ++i) cb[i - k] = acc[i];
-            }
-            wave_sync();
-            // first row of maximal |D(i, k)|, k <= i <= kend (zeros below)
-            double best = fabs(cb[0]);
-            int p = k;
-#pragma unroll
-            for (int i = k + 1; i <= kend; ++i) {
-                const double v = fabs(cb[i - k]);
-                if (v > best) {
-                    best = v;
-                    p = i;
-                }
-            }
-#pragma unroll
-            for (int i = k + 1; i <= kend; ++i) {
-                if (p == i) {
-                    const double tmp = acc[k];
-                    acc[k] = acc[i];
-                    acc[i] = tmp;
-                }
-            }
-            const double pv = cb[p - k];
-#pragma unroll
-            for (int i = k + 1; i <= kend; ++i) {
-                const double dik = (i == p) ? cb[0] : cb[i - k];  // D(i, k) after the exchange
-                const double l = dik / pv;
-                if (nside || j > k) acc[i] = acc[i] - l * acc[k];
-            }
-        }
-        // back substitution U F = N with U the upper triangle of D (LDS,
-        // row-major); every lane runs it, the N lanes keep F
-        if (!nside) {
-#pragma unroll
-            for (int i = 0; i < N; ++i)
-                if (i <= j) Ur[i * N + j] = acc[i];
-        }
+        for (int k = N - BW - 1; k < N; ++k) elim_step32<true>(k, lane, w, XT);
         wave_sync();
+        if (lane >= N) {
+            double x[N];
 #pragma unroll
-        for (int i = N - 1; i >= 0; --i) {
-            double a = acc[i];
+            for (int i = 0; i < N; ++i) x[i] = XT[W * i + lane];
 #pragma unroll
-            for (int cc = i + 1; cc < N; ++cc) a -= Ur[i * N + cc] * acc[cc];
-            acc[i] = a / Ur[i * N + i];
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        // squarings F <- F F (x holds the product)
-        for (int sq = 0; sq < js; ++sq) {
-            if (nside) {
+            for (int i = N - 1; i >= 0; --i) {
+                double a = x[i];
 #pragma unroll
-                for (int i = 0; i < N; ++i) Fr[i * N + j] = acc[i];
+                for (int cc = i + 1; cc < N; ++cc) a -= XT[W * i + cc] * x[cc];
+                x[i] = a / XT[W * i + i];
             }
-            wave_sync();
 #pragma unroll
-            for (int i = 0; i < N; ++i) {
-                double sum = 0.0;
-#pragma unroll
-                for (int l = 0; l < N; ++l) sum += Fr[i * N + l] * acc[l];
-                x[i] = sum;
-                __builtin_amdgcn_sched_barrier(0);
-            }
-            wave_sync();
-#pragma unroll
-            for (int i = 0; i < N; ++i) acc[i] = x[i];
-        }
-        if (nside) {
-#pragma unroll
-            for (int i = 0; i < N; ++i) Nm[i + j * N] = acc[i];
+            for (int i = 0; i < N; ++i) XT[W * i + lane] = x[i];
         }
     }
     __syncthreads();
+    // F (the right half of XT by rows) by columns into Nm
+    double f[4];
+    f[0] = XT[W * i0 + N + j0];
+    f[1] = XT[W * i0 + N + j0 + 1];
+    f[2] = XT[W * (i0 + 1) + N + j0];
+    f[3] = XT[W * (i0 + 1) + N + j0 + 1];
+    store_cols_2x2(Nm, i0, j0, f);
+    __syncthreads();
+    for (int sq = 0; sq < js; ++sq) {
+        prod32_2x2(Nm, XT + N, W, i0, j0, f);
+        __syncthreads();
+        store_cols_2x2(Nm, i0, j0, f);
+        store_rows_2x2(XT + N, W, i0, j0, f);
+        __syncthreads();
+    }
 }
 
 // --------------------------------------------------------------- dense expm
@@ -233,7 +314,7 @@ __device__ __forceinline__ void expm_wave32(const double *A, double *At, double 
 // oracle's orc_expm, so the two agree bit for bit on equal input.  Result in
 // Nm.
 //
-// A 32 x 32 Hessenberg input (the Krylov case) goes to expm_wave32.  Work
+// A 32 x 32 Hessenberg input (the Krylov case) goes to expm32_hess.  Work
 // that only adds exact zeros is skipped.  With b the lower bandwidth
 // of A (1 for the Krylov Hessenberg), A^k has lower bandwidth k b and the
 // Pade denominator D lower bandwidth 6 b; elimination with partial pivoting
@@ -291,8 +372,10 @@ __device__ __forceinline__ void expm_block(int N, double *A, double *X, double *
     }
     for (int i = tid; i < N; i += blockDim.x) perm[0][i] = i;
     __syncthreads();
-    if (N == 32 && lb <= 1 && blockDim.x >= kWave) {
-        expm_wave32(A, T, X, Dm, Nm, js);
+    if (N == 32 && lb <= 1 && blockDim.x == 4 * kWave && T == X + NN &&
+        ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(X) |
+          reinterpret_cast<uintptr_t>(Nm)) & 15) == 0) {
+        expm32_hess(A, X, Nm, js);
         return;
     }
     const int q = 6;

@@ -320,7 +320,7 @@ template <typename V, typename B>
 __global__ void __launch_bounds__(kBlock) k_arn_csr(CsrF<V> A, ArnArgs a)
 {
     if (arn_skip<B>(a)) return;
-    extern __shared__ double smem[];
+    FSP_DYNAMIC_LDS(smem);
     double *red = smem;
     double *prod = red + kArnRedLen;
     double sig;
@@ -344,7 +344,7 @@ template <int NU, typename V, typename B>
 __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
 {
     if (arn_skip<B>(a)) return;
-    extern __shared__ double smem[];
+    FSP_DYNAMIC_LDS(smem);
     double *red = smem;
     double sig;
     if (!arn_sigma(a, red, sig)) return;
@@ -391,7 +391,7 @@ template <int D, typename V, typename B>
 __global__ void __launch_bounds__(kBlock) k_arn_kron(KronF<V> A, ArnArgs a, int use_lds)
 {
     if (arn_skip<B>(a)) return;
-    extern __shared__ double smem[];
+    FSP_DYNAMIC_LDS(smem);
     double *red = smem;
     V *tabl = (V *)(red + kArnRedLen);
     const V *tab = A.tab_g;
@@ -418,7 +418,7 @@ template <int D, typename V, typename B>
 __global__ void __launch_bounds__(kBlock) k_arn_kron_plan(KronF<V> A, KronPlan P, ArnArgs a)
 {
     if (arn_skip<B>(a)) return;
-    extern __shared__ double smem[];
+    FSP_DYNAMIC_LDS(smem);
     double *red = smem;
     V *tabl = (V *)(red + kArnRedLen);
     stage_table(tabl, A.tab_g, A.tab_size);
@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_kron_slot(KronF<V> A, KronSlotPl
                                                           const V *__restrict__ tab_g, ArnArgs a)
 {
     if (arn_skip<B>(a)) return;
-    extern __shared__ double smem[];
+    FSP_DYNAMIC_LDS(smem);
     double *red = smem;
     V *tabl = (V *)(red + kArnRedLen);
     stage_table(tabl, tab_g, P.tab_size);
@@ -539,7 +539,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2)
         // FP32 basis: every workgroup forms the Gram-corrected coefficients
         // 2h - G h from the raw dots in LDS; workgroup 0 also stores them as
         // the Hessenberg column (no workgroup reads that column here)
-        HIP_DYNAMIC_SHARED(double, lds)
+        FSP_DYNAMIC_LDS(lds);
         double *cw = lds + kGramLdsBytes / sizeof(double);
         gram_coefs_lds(st, a.istart, a.j, lds, cw);
         if (blockIdx.x == 0)
@@ -619,7 +619,7 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(KState *st, const double 
 // Error estimate, reject loop and step update for one state: kry_control.
 __global__ void __launch_bounds__(kBlock) k_expm_control(KState *st, int m, int max_reject)
 {
-    extern __shared__ double smem[];
+    FSP_DYNAMIC_LDS(smem);
     kry_control(st, m, max_reject, smem);
 }
 
@@ -653,7 +653,7 @@ __global__ void __launch_bounds__(kBlock) k_solution(const KState *st, const dou
 __global__ void __launch_bounds__(kBlock) k_dense_coef(const KState *st, int m, double delta,
                                                       double *coef)
 {
-    extern __shared__ double smem[];
+    FSP_DYNAMIC_LDS(smem);
     const int M2 = m + 2, NN = M2 * M2;
     double *A = smem, *X = A + NN, *T = X + NN, *Nm = T + NN, *Dm = Nm + NN;
     __shared__ double red[kBlock / kWave];
@@ -1382,7 +1382,7 @@ extern "C" fsp_status fsp_expv_set_limits(fsp_expv e, int max_reject, int64_t ma
 namespace {
 __global__ void __launch_bounds__(fsp::kBlock) k_expm_dense(int N, const double *Ag, double *Fg)
 {
-    extern __shared__ double smem[];
+    FSP_DYNAMIC_LDS(smem);
     const int NN = N * N;
     double *A = smem, *X = A + NN, *T = X + NN, *Nm = T + NN, *Dm = Nm + NN;
     __shared__ double red[fsp::kBlock / fsp::kWave];

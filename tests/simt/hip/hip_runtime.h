@@ -37,6 +37,8 @@
 #define __launch_bounds__(...)
 #define HIP_KERNEL_NAME(...) __VA_ARGS__
 #define HIP_DYNAMIC_SHARED(type, var) type *var = (type *)::simt::dynamic_lds();
+// the library's 16-byte aligned dynamic LDS (operator new storage: aligned)
+#define FSP_DYNAMIC_LDS(var) double *var = (double *)::simt::dynamic_lds()
 
 struct dim3 {
     unsigned x, y, z;
@@ -137,6 +139,18 @@ inline T shfl_xor(T v, int mask, int width)
     T r;
     std::memcpy(&r, &got, sizeof(T));
     return r;
+}
+
+// v of lane `src` of the calling lane's wave (every lane of the wave calls)
+inline int readlane(int v, int src)
+{
+    Group *g = group();
+    const unsigned t = g->cur->tid, w = t / kWaveLanes;
+    g->xbuf[t] = (uint64_t)(uint32_t)v;
+    wave_barrier(w);
+    const uint64_t got = g->xbuf[w * kWaveLanes + (unsigned)src];
+    wave_barrier(w);
+    return (int)(uint32_t)got;
 }
 
 inline void lane_entry()
@@ -281,6 +295,7 @@ inline void __syncthreads() { ::simt::syncthreads(); }
 #define __builtin_amdgcn_sched_barrier(mask) ((void)0)
 #define __builtin_amdgcn_wave_barrier() \
     ::simt::wave_barrier(::simt::group()->cur->tid / ::simt::kWaveLanes)
+#define __builtin_amdgcn_readlane(v, lane) ::simt::readlane((v), (lane))
 template <typename T>
 inline T __shfl_xor(T v, int mask, int width = 64)
 {

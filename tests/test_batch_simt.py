@@ -206,3 +206,46 @@ def test_simt_lockstep_initial_vector_and_autotune(sim, tmp_path):
     opr = O.Operator(net, "hyb")
     rc, out, st = O.expv_outputs(opr, p0.copy(), [1.5])
     assert np.array_equal(B.solution(1), out[0])
+
+
+def _expm_cases():
+    """32 x 32 upper Hessenberg matrices (the Krylov shape) over infinity
+    norms that give 0 to 11 squarings, the Expokit-augmented Krylov matrix,
+    and dense and smaller inputs that take the general path."""
+    rng = np.random.default_rng(21)
+    out = []
+    for norm in (0.0, 1e-3, 0.4, 0.99, 3.0, 40.0, 900.0, 1500.0):
+        H = np.triu(rng.standard_normal((32, 32)), -1)
+        if norm > 0:
+            H *= norm / np.abs(H).sum(axis=1).max()
+        else:
+            H[:] = 0.0
+        out.append(H)
+    # tau * [H_m, e_1 column block]: H(m+1, m) = h, H(m, m+1) = 1, H(m+1, m+2) = 1
+    m = 30
+    H = np.zeros((32, 32))
+    H[:m + 1, :m] = np.triu(rng.standard_normal((m + 1, m)), -1)
+    H[0, m] = 1.0
+    H[m + 1, m] = 1.0
+    out.append(5.0 * H)
+    out.append(rng.standard_normal((32, 32)) * 0.3)          # dense: general path
+    out.append(np.triu(rng.standard_normal((12, 12)), -1))    # N = 12
+    out.append(np.triu(rng.standard_normal((32, 32)), -2))    # lower bandwidth 2
+    return out
+
+
+@pytest.mark.parametrize("block", [256, 128, 64])
+def test_simt_expm_bit_exact(simlib, block):
+    """The device Pade exponential against the oracle's orc_expm, bit for
+    bit: at 256 lanes the 32 x 32 Hessenberg inputs take the blocked path,
+    at 128 and 64 lanes (and for the other inputs) the general one."""
+    f = simlib.simt_expm_dense
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+    for A in _expm_cases():
+        A = np.asfortranarray(A)
+        F = np.zeros_like(A, order="F")
+        assert f(A.shape[0], A.ctypes.data, F.ctypes.data, block) == 0
+        want = O.expm(A)
+        assert np.array_equal(F.view(np.uint64), want.view(np.uint64)), (
+            A.shape, np.abs(A).sum(axis=1).max(), np.abs(F - want).max())
