@@ -48,6 +48,17 @@ def compile_cpp_program(src: str, out: str) -> str:
     return out
 
 
+def compile_hip_program(srcs: list, out: str) -> str:
+    """Builds a HIP program against the facade with -fgpu-rdc (the device
+    PropFun test: its __device__ propensity lives in another translation
+    unit, as with the reference's separable compilation)."""
+    _run([HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-fgpu-rdc",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-I", os.path.join(ROOT, "include"), "-I", CPP_INCLUDE, *srcs,
+          "-o", out, "-L", LIBDIR, "-lcufsp", "-lfsp_hip", f"-Wl,-rpath,{LIBDIR}"])
+    return out
+
+
 def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

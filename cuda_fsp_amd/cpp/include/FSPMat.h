@@ -22,8 +22,11 @@ enum Precision { FP64, MIXED };
 
 class FSPMat {
 public:
-    // src/FSPMat.cu:10-45 (PropFun: evaluated on the host over the box once,
-    // the table lives on the GPU)
+    // src/FSPMat.cu:10-45.  prop_func is either a host function, evaluated
+    // on the host over the box a chunk at a time, or (as in the reference)
+    // the address of a __device__ function fetched with hipMemcpyFromSymbol,
+    // evaluated on the GPU by the tabulator at the end of this header.  The
+    // table [R][n] lives on the GPU.
     explicit FSPMat(int n_reactions, int n_species, int *fsp_dim, CSRMatInt stoich,
                     TcoefFun t_func, PropFun prop_func, MatrixFormat format = CUDA_CSR,
                     Precision precision = FP64, bool with_sink = false);
@@ -59,3 +62,72 @@ private:
 };
 
 }  // namespace cuFSP
+
+#if defined(__HIPCC__)
+// ------------------------------------------------------ device propensities
+// The reference's PropFun is a __device__ function pointer, fetched with
+// cudaMemcpyFromSymbol and called inside its assembly kernels
+// (src/cme_util.h:25, src/fspmat_csr_kernels.cu:79,83,
+// tests/test_fsp_mat.cu:86,123-124).  The same code works here with
+// hipMemcpyFromSymbol: the FSPMat constructor sees an address that is no
+// host code and calls cufsp_tabulate_device_propensity below, which
+// evaluates a_r(x) on the GPU over the box into the device table
+// [R][n] (one thread per state, the R reactions in order; each call gets
+// its own copy of the state).
+//
+// The tabulator is compiled into every hipcc translation unit that
+// includes this header (a weak definition: the program keeps one).  An
+// indirect call needs the propensity and the calling kernel in one code
+// object, so a program whose device propensity lives in another
+// translation unit is built with -fgpu-rdc, as the reference builds with
+// separable compilation.
+#include <hip/hip_runtime.h>
+
+namespace cuFSP {
+namespace device_propensity {
+struct Box {
+    int n_species, n_reactions, n_states;
+    int bounds[kMaxSpeciesFacade];
+};
+
+static __global__ void __launch_bounds__(256) k_tabulate(PropFun f, Box box, double *table)
+{
+    const long long n = box.n_states;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        // index -> state, species 0 fastest (src/cme_util.cu:5-11)
+        int x[kMaxSpeciesFacade];
+        long long rem = i;
+        for (int s = 0; s < box.n_species; ++s) {
+            x[s] = (int)(rem % (box.bounds[s] + 1));
+            rem /= box.bounds[s] + 1;
+        }
+        for (int r = 0; r < box.n_reactions; ++r) {
+            int xc[kMaxSpeciesFacade];
+            for (int s = 0; s < box.n_species; ++s) xc[s] = x[s];
+            table[(size_t)r * (size_t)n + (size_t)i] = f(xc, r);
+        }
+    }
+}
+}  // namespace device_propensity
+}  // namespace cuFSP
+
+// Fills table_dev[r * n_states + i] = f(x_i, r) on the GPU; 0 on success.
+// The caller has checked 1 <= n_species <= kMaxSpeciesFacade.
+extern "C" __attribute__((weak)) int cufsp_tabulate_device_propensity(
+    cuFSP::PropFun f, int n_species, int n_reactions, const int *bounds, int n_states,
+    double *table_dev)
+{
+    cuFSP::device_propensity::Box box{};
+    box.n_species = n_species;
+    box.n_reactions = n_reactions;
+    box.n_states = n_states;
+    for (int s = 0; s < n_species; ++s) box.bounds[s] = bounds[s];
+    const long long blocks = ((long long)n_states + 255) / 256;
+    const unsigned grid = (unsigned)(blocks < 16384 ? (blocks > 0 ? blocks : 1) : 16384);
+    hipLaunchKernelGGL(cuFSP::device_propensity::k_tabulate, dim3(grid), dim3(256), 0, nullptr,
+                       f, box, table_dev);
+    if (hipGetLastError() != hipSuccess) return 1;
+    return hipDeviceSynchronize() == hipSuccess ? 0 : 2;
+}
+#endif  // __HIPCC__

@@ -1,10 +1,18 @@
 // cuFSP.cpp -- C++ facade (namespace cuFSP) over the engine's C ABI.
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 
 #include "FSPMat.h"
 #include "KryExpvFSP.h"
+
+// Defined (weak) in every hipcc translation unit that includes FSPMat.h:
+// evaluates a __device__ PropFun on the GPU.  Null when the program has none.
+extern "C" int cufsp_tabulate_device_propensity(cuFSP::PropFun f, int n_species, int n_reactions,
+                                                const int *bounds, int n_states,
+                                                double *table_dev) __attribute__((weak));
 
 namespace cuFSP {
 
@@ -185,24 +193,31 @@ void FSPMat::init(int n_reactions, int n_species, int *fsp_dim, CSRMatInt stoich
     rows_ = (int)info.n_rows;
 }
 
-FSPMat::FSPMat(int n_reactions, int n_species, int *fsp_dim, CSRMatInt stoich, TcoefFun t_func,
-               PropFun prop_func, MatrixFormat format, Precision precision, bool with_sink)
-    : ns_(n_species), nr_(n_reactions), tcoeffunc_(t_func)
+// a_r(x) over the box into the device table [R][n].
+static void tabulate_propensity(PropFun prop_func, int n_species, int n_reactions, int *fsp_dim,
+                                int n, double *table_dev)
 {
-    if (!prop_func) throw std::runtime_error("FSPMat: propensity function is NULL");
-    if (n_species < 1 || n_species > kMaxSpeciesFacade)
-        throw std::runtime_error("FSPMat: n_species must be in 1..8");
-    if (n_reactions < 1) throw std::runtime_error("FSPMat: n_reactions must be >= 1");
-    const int n = rect_fsp_num_states(n_species, fsp_dim);
-    // Tabulate a_r(x) over the box on the host (the reference calls its
-    // device PropFun inside the assembly kernel, src/fspmat_csr_kernels.cu:79)
-    // into the device table [R][n], a chunk of states at a time, so the host
-    // holds R * chunk values, not R * n.  prop_func is called from several
-    // OpenMP threads at once: like the reference's device propensity (called
-    // by every GPU thread), it must be a pure function of its arguments.
-    void *p = nullptr;
-    throw_on(fsp_malloc(&p, (size_t)n_reactions * (size_t)n * sizeof(double)), "FSPMat::FSPMat");
-    table_dev_ = (double *)p;
+    // A pointer into no loaded host object is the address of a __device__
+    // function (hipMemcpyFromSymbol, as tests/test_fsp_mat.cu:123-124 of the
+    // reference): evaluated on the GPU, where the reference calls it inside
+    // its assembly kernel (src/fspmat_csr_kernels.cu:79).
+    Dl_info where;
+    if (dladdr(reinterpret_cast<void *>(prop_func), &where) == 0) {
+        if (!cufsp_tabulate_device_propensity)
+            throw std::runtime_error(
+                "FSPMat: prop_func is no host function and the program holds no device "
+                "propensity tabulator (compile the caller with hipcc, including FSPMat.h)");
+        const int rc = cufsp_tabulate_device_propensity(prop_func, n_species, n_reactions,
+                                                        fsp_dim, n, table_dev);
+        if (rc != 0)
+            throw std::runtime_error("FSPMat: device propensity tabulation failed (code " +
+                                     std::to_string(rc) + ")");
+        return;
+    }
+    // A host function: a chunk of states at a time, so the host holds
+    // R * chunk values, not R * n.  prop_func is called from several OpenMP
+    // threads at once: like the reference's device propensity (called by
+    // every GPU thread), it must be a pure function of its arguments.
     constexpr int kChunk = 1 << 20;
     std::vector<double> buf((size_t)n_reactions * (size_t)std::min(n, kChunk));
     for (int c0 = 0; c0 < n; c0 += kChunk) {
@@ -218,12 +233,33 @@ FSPMat::FSPMat(int n_reactions, int n_species, int *fsp_dim, CSRMatInt stoich, T
             }
         }
         for (int r = 0; r < n_reactions; ++r)
-            throw_on(fsp_memcpy_h2d(table_dev_ + (size_t)r * n + c0, buf.data() + (size_t)r * cn,
+            throw_on(fsp_memcpy_h2d(table_dev + (size_t)r * n + c0, buf.data() + (size_t)r * cn,
                                     (size_t)cn * sizeof(double)),
                      "FSPMat::FSPMat");
     }
-    init(n_reactions, n_species, fsp_dim, stoich, nullptr, table_dev_, format, precision,
-         with_sink);
+}
+
+FSPMat::FSPMat(int n_reactions, int n_species, int *fsp_dim, CSRMatInt stoich, TcoefFun t_func,
+               PropFun prop_func, MatrixFormat format, Precision precision, bool with_sink)
+    : ns_(n_species), nr_(n_reactions), tcoeffunc_(t_func)
+{
+    if (!prop_func) throw std::runtime_error("FSPMat: propensity function is NULL");
+    if (n_species < 1 || n_species > kMaxSpeciesFacade)
+        throw std::runtime_error("FSPMat: n_species must be in 1..8");
+    if (n_reactions < 1) throw std::runtime_error("FSPMat: n_reactions must be >= 1");
+    const int n = rect_fsp_num_states(n_species, fsp_dim);
+    void *p = nullptr;
+    throw_on(fsp_malloc(&p, (size_t)n_reactions * (size_t)n * sizeof(double)), "FSPMat::FSPMat");
+    table_dev_ = (double *)p;
+    try {
+        tabulate_propensity(prop_func, n_species, n_reactions, fsp_dim, n, table_dev_);
+        init(n_reactions, n_species, fsp_dim, stoich, nullptr, table_dev_, format, precision,
+             with_sink);
+    } catch (...) {
+        fsp_free(table_dev_);
+        table_dev_ = nullptr;
+        throw;
+    }
 }
 
 FSPMat::FSPMat(int n_reactions, int n_species, int *fsp_dim, CSRMatInt stoich, TcoefFun t_func,

@@ -1,5 +1,7 @@
 """The C++ drop-in facade (cuFSP::, cme::): test programs that mirror the
-reference's own tests, compiled against libcufsp.so and run on the GPU."""
+reference's own tests, compiled against libcufsp.so and run on the GPU,
+including the reference's device PropFun route (a __device__ propensity
+fetched with hipMemcpyFromSymbol, built with hipcc -fgpu-rdc)."""
 import os
 import subprocess
 
@@ -33,6 +35,29 @@ def test_cpp_reference_tests(name, tmp_path_factory):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
+
+
+def _device_propfun_exe(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("hip") / "test_fsp_mat_device")
+    B.compile_hip_program([os.path.join(HERE, "cpp", "test_fsp_mat_device.hip"),
+                           os.path.join(HERE, "cpp", "toggle_device.hip")], out)
+    return out
+
+
+def test_device_propfun_program_compiles(tmp_path_factory):
+    # the reference's device PropFun route, the propensity in a second
+    # translation unit: hipcc -fgpu-rdc against the facade
+    assert os.path.exists(_device_propfun_exe(tmp_path_factory))
+
+
+@pytest.mark.gpu
+def test_device_propfun_route(tmp_path_factory):
+    exe = _device_propfun_exe(tmp_path_factory)
+    print("[case] test_fsp_mat_device", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "device PropFun test done" in r.stdout
 
 
 @pytest.mark.gpu
