@@ -22,7 +22,7 @@ PRECISIONS = {"f64": FSP_F64, "mixed": FSP_MIXED}
 
 STATUS = {0: "FSP_OK", 1: "FSP_ERR_INVALID", 2: "FSP_ERR_UNSUPPORTED", 3: "FSP_ERR_OVERFLOW",
           4: "FSP_ERR_HIP", 5: "FSP_ERR_NOMEM", 6: "FSP_ERR_NOT_CONVERGED",
-          7: "FSP_ERR_NO_DEVICE"}
+          7: "FSP_ERR_NO_DEVICE", 8: "FSP_ERR_TIMEOUT"}
 
 
 class FspError(RuntimeError):
@@ -62,11 +62,13 @@ class FspExpvProfile(C.Structure):
 class FspBatchOptions(C.Structure):
     _fields_ = [("m", C.c_int), ("tol", C.c_double), ("anorm", C.c_double),
                 ("precision", C.c_int), ("iop", C.c_int), ("q_iop", C.c_int),
-                ("block", C.c_int), ("wg_per_cu", C.c_int), ("max_reject", C.c_int),
-                ("max_nstep", C.c_int64), ("path", C.c_int)]
+                ("lockstep_bpc", C.c_int), ("wg_per_cu", C.c_int), ("max_reject", C.c_int),
+                ("max_nstep", C.c_int64), ("path", C.c_int), ("split_states", C.c_int64),
+                ("time_limit_s", C.c_double)]
 
 
-BATCH_PATHS = {"auto": 0, "lockstep": 1, "streams": 2, "persistent": 3}
+BATCH_PATHS = {"auto": 0, "lockstep": 1, "streams": 2, "persistent": 3, "hybrid": 4}
+BATCH_PATH_NAMES = {v: k for k, v in BATCH_PATHS.items()}
 
 
 MATVEC_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p)
@@ -144,6 +146,7 @@ _SIGS = {
     "fsp_batch_get_solution": (C.c_int, [_vp, _i, _vp, _i64]),
     "fsp_batch_get_stats": (C.c_int, [_vp, _i, _vp, _vp, _vp]),
     "fsp_batch_get_config": (C.c_int, [_vp, _vp, _vp]),
+    "fsp_batch_get_tuning": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "fsp_batch_autotune": (C.c_int, [_vp, C.c_char_p, _i, _vp]),
     "fsp_loglik_indices": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
     "fsp_loglik_indices_noalloc": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
@@ -611,14 +614,16 @@ class Expv:
 
 class Batch:
     """fsp_batch handle: many independent solves v_i <- exp(t_i A_i) v_i.
-    path: "auto", "lockstep" (one graph-captured batch step advances every
-    running instance), "streams" (single-solve engine, up to four instances
-    at a time) or "persistent" (one workgroup per instance; refused on GPU
-    devices)."""
+    path: "auto" (the autotuner's choice, else "hybrid"), "lockstep" (one
+    graph-captured batch step advances every running instance),
+    "persistent" (one persistent work-queue kernel, each workgroup solving
+    whole instances), "hybrid" (the persistent kernel for instances of at
+    most split_states entries, lockstep for the rest, concurrently) or
+    "streams" (single-solve engine, up to four instances at a time)."""
 
     def __init__(self, nets, t_final, with_sink=None, m=30, tol=1e-8, anorm=1.0,
-                 precision="f64", iop=False, q_iop=2, block=0, wg_per_cu=0,
-                 max_reject=0, max_nstep=0, path="auto"):
+                 precision="f64", iop=False, q_iop=2, lockstep_bpc=0, wg_per_cu=0,
+                 max_reject=0, max_nstep=0, path="auto", split_states=0, time_limit_s=0.0):
         self.nets = list(nets)
         count = len(self.nets)
         arr = (FspModel * count)()
@@ -630,8 +635,9 @@ class Batch:
         tf = np.ascontiguousarray(t_final, np.float64)
         ws = None if with_sink is None else np.ascontiguousarray([1 if s else 0 for s in with_sink], np.int32)
         self._keep = [coefs, tf, ws]
-        opt = FspBatchOptions(m, tol, anorm, PRECISIONS[precision], int(iop), int(q_iop), int(block),
-                              int(wg_per_cu), int(max_reject), int(max_nstep), BATCH_PATHS[path])
+        opt = FspBatchOptions(m, tol, anorm, PRECISIONS[precision], int(iop), int(q_iop),
+                              int(lockstep_bpc), int(wg_per_cu), int(max_reject), int(max_nstep),
+                              BATCH_PATHS[path], int(split_states), float(time_limit_s))
         h = C.c_void_p()
         check(lib().fsp_batch_create(C.cast(arr, C.c_void_p), coefs.ctypes.data, tf.ctypes.data,
                                      None if ws is None else ws.ctypes.data, count, C.byref(opt),
@@ -662,6 +668,13 @@ class Batch:
         b, w = C.c_int(0), C.c_int(0)
         check(lib().fsp_batch_get_config(self.h, C.byref(b), C.byref(w)))
         return b.value, w.value
+
+    def tuning(self):
+        """(path name, lockstep workgroups per CU, persistent workgroups per
+        CU, split in states) of the next solve."""
+        p, b, w, sp = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int64(0)
+        check(lib().fsp_batch_get_tuning(self.h, C.byref(p), C.byref(b), C.byref(w), C.byref(sp)))
+        return BATCH_PATH_NAMES[p.value], b.value, w.value, sp.value
 
     def autotune(self, cache_path=None, force=False) -> float:
         ms = C.c_double(0.0)

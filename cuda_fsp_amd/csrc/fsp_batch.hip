@@ -1,17 +1,20 @@
-// fsp_batch.hip -- many uneven FSP solves: set-up, path dispatch, autotuner
-// and the persistent-kernel path.
+// fsp_batch.hip -- many uneven FSP solves: set-up, path dispatch and the
+// autotuner.
 //
-// Persistent path: each workgroup pulls instances from a work queue
-// (largest estimated cost first) and runs the whole Krylov expv of that
-// instance inside the workgroup: matrix-free Kronecker-sum SpMV from
-// LDS-staged factor tables, fused dot products, classical Gram-Schmidt, the
-// Pade expm of the Hessenberg matrix and the step control of
-// KryExpvFSP::step (src/KryExpvFSP.cu:38-182), with workgroup barriers only.
-// The launch (queue reset + persistent kernel) is captured once in a HIP
-// graph and replayed per solve.  The lockstep path (fsp_batch_lockstep.hip)
-// and the stream scheduler (fsp_batch_streams.hip) are the GPU-device paths.
-// The autotuner (fsp_batch_autotune) times the candidates of the path in use
-// on the device and caches the choice in a JSON file.
+// Paths (all on the device; fsp_batch_lockstep.hip holds their kernels):
+//   LOCKSTEP    one graph-captured batch step advances every running
+//               instance by one Krylov step of its own;
+//   PERSISTENT  one persistent work-queue kernel: each workgroup solves
+//               whole instances (largest estimated cost first), the launch
+//               (instance reset, queue reset, kernel, stats export) captured
+//               once in a HIP graph and replayed per solve;
+//   HYBRID      the persistent kernel takes the instances of at most
+//               `split` states, the lockstep steps take the rest, both at
+//               once on their own streams;
+//   STREAMS     each instance through the single-solve engine, up to four
+//               at a time (fsp_batch_streams.hip).
+// The autotuner (fsp_batch_autotune) times the candidates on the device and
+// caches the choice in a JSON file.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -27,145 +30,21 @@
 namespace fsp {
 
 
-static size_t batch_lds_bytes(const BatchImpl &B, int bs)
-{
-    const int M2 = B.m + 2;
-    return sizeof(double) * ((size_t)6 * M2 * M2 + 2 * (bs / kWave) + M2 + (size_t)B.tab_lds);
-}
-
-template <int BS, typename Bt, int NKB, int DMAX>
-static fsp_status launch_batch_t(BatchImpl &B, const BatchArgs &a, int grid, hipStream_t s)
-{
-    const size_t lds = batch_lds_bytes(B, BS);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_batch<BS, Bt, NKB, DMAX>), dim3(grid), dim3(BS), lds, s, a);
-    FSP_HIP(hipGetLastError());
-    return FSP_OK;
-}
-
-template <int BS>
-static fsp_status launch_batch_bs(BatchImpl &B, const BatchArgs &a, int grid, hipStream_t s)
-{
-    const bool big = B.m + 1 > 32;
-    if (B.dmax <= 4) {
-        if (B.mixed)
-            return big ? launch_batch_t<BS, float, kMaxKrylov + 1, 4>(B, a, grid, s)
-                       : launch_batch_t<BS, float, 32, 4>(B, a, grid, s);
-        return big ? launch_batch_t<BS, double, kMaxKrylov + 1, 4>(B, a, grid, s)
-                   : launch_batch_t<BS, double, 32, 4>(B, a, grid, s);
-    }
-    if (B.mixed)
-        return big ? launch_batch_t<BS, float, kMaxKrylov + 1, 8>(B, a, grid, s)
-                   : launch_batch_t<BS, float, 32, 8>(B, a, grid, s);
-    return big ? launch_batch_t<BS, double, kMaxKrylov + 1, 8>(B, a, grid, s)
-               : launch_batch_t<BS, double, 32, 8>(B, a, grid, s);
-}
-
-static BatchArgs batch_args(const BatchImpl &B)
-{
-    BatchArgs a;
-    a.inst = B.d_inst;
-    a.order = B.d_order;
-    a.count = B.count;
-    a.queue = B.d_queue;
-    a.ws = B.ws;
-    a.ws32 = B.ws32;
-    a.ws_stride = B.ws_stride;
-    a.ws32_stride = B.ws32_stride;
-    a.nv_max = B.nv_max;
-    a.m = B.m;
-    a.iop = B.iop;
-    a.q_iop = B.q_iop;
-    a.max_reject = B.max_reject;
-    a.tab_lds = B.tab_lds;
-    a.max_nstep = B.max_nstep;
-    a.tol = B.tol;
-    a.anorm = B.anorm;
-    a.fact = std::pow((B.m + 1) / std::exp(1.0), B.m + 1) * std::sqrt(2 * (3.1416) * (B.m + 1));
-    a.p10 = B.d_p10;
-    return a;
-}
-
-static fsp_status enqueue_batch(BatchImpl &B, int block, int wg_per_cu, hipStream_t s)
-{
-    const int grid = std::max(1, std::min<int>(B.count, B.cus * wg_per_cu));
-    if ((size_t)grid > B.ws_slots) return fail(FSP_ERR_INVALID, "batch grid exceeds workspace");
-    const BatchArgs a = batch_args(B);
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, B.d_queue);
-    switch (block) {
-    case 256: return launch_batch_bs<256>(B, a, grid, s);
-    case 512: return launch_batch_bs<512>(B, a, grid, s);
-    case 1024: return launch_batch_bs<1024>(B, a, grid, s);
-    default: return fail(FSP_ERR_INVALID, "batch block must be 256, 512 or 1024");
-    }
-}
-
-// The persistent kernel matches the oracle bit for bit in the CPU (SIMT)
-// build of this file, but one run on a gfx950 device stalled without an
-// identified cause; until that cause is found it is never launched on a GPU
-// device, and batches there go through the stream scheduler instead.
-static fsp_status persistent_allowed(bool *ok)
-{
-    int dev = 0;
-    FSP_HIP(hipGetDevice(&dev));
-    hipDeviceProp_t prop;
-    FSP_HIP(hipGetDeviceProperties(&prop, dev));
-    *ok = std::strncmp(prop.gcnArchName, "gfx", 3) != 0;
-    return FSP_OK;
-}
-
-static fsp_status batch_graph(BatchImpl &B, hipStream_t s)
-{
-    bool ok = false;
-    FSP_TRY(persistent_allowed(&ok));
-    if (!ok) return fail(FSP_ERR_UNSUPPORTED, "the persistent batch kernel is not launched on GPU devices");
-    if (B.gexec && B.graph_block == B.block && B.graph_wgcu == B.wg_per_cu) {
-        FSP_HIP(hipGraphLaunch(B.gexec, s));
-        return FSP_OK;
-    }
-    if (B.gexec) (void)hipGraphExecDestroy(B.gexec);
-    if (B.graph) (void)hipGraphDestroy(B.graph);
-    B.gexec = nullptr;
-    B.graph = nullptr;
-    if (!B.cap) FSP_HIP(hipStreamCreateWithFlags(&B.cap, hipStreamNonBlocking));
-    FSP_HIP(hipStreamSynchronize(s));
-    FSP_HIP(hipStreamBeginCapture(B.cap, hipStreamCaptureModeThreadLocal));
-    fsp_status st = enqueue_batch(B, B.block, B.wg_per_cu, B.cap);
-    hipGraph_t g = nullptr;
-    hipError_t ce = hipStreamEndCapture(B.cap, &g);
-    FSP_TRY(st);
-    if (ce != hipSuccess) return fail(FSP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-    B.graph = g;
-    FSP_HIP(hipGraphInstantiate(&B.gexec, g, nullptr, nullptr, 0));
-    B.graph_block = B.block;
-    B.graph_wgcu = B.wg_per_cu;
-    FSP_HIP(hipGraphLaunch(B.gexec, s));
-    return FSP_OK;
-}
-
 // Path that fsp_batch_solve takes: the requested one, else the autotuner's
-// choice, else LOCKSTEP on GPU devices and PERSISTENT elsewhere.
-static fsp_status batch_path(const BatchImpl &B, int *path)
+// choice, else HYBRID.
+static int batch_path(const BatchImpl &B)
 {
-    bool persistent = false;
-    FSP_TRY(persistent_allowed(&persistent));
     int p = B.path_req;
     if (p == FSP_BATCH_AUTO) p = B.path_tuned;
-    if (p == FSP_BATCH_AUTO) p = persistent ? FSP_BATCH_PERSISTENT : FSP_BATCH_LOCKSTEP;
-    if (p == FSP_BATCH_PERSISTENT && !persistent)
-        return fail(FSP_ERR_UNSUPPORTED, "the persistent batch kernel is not launched on GPU devices");
-    *path = p;
-    return FSP_OK;
+    if (p == FSP_BATCH_AUTO) p = FSP_BATCH_HYBRID;
+    return p;
 }
 
 static void batch_free(BatchImpl &B)
 {
     batch_streams_free(B);
     batch_lockstep_free(B);
-    if (B.gexec) (void)hipGraphExecDestroy(B.gexec);
-    if (B.graph) (void)hipGraphDestroy(B.graph);
-    if (B.cap) (void)hipStreamDestroy(B.cap);
-    void *ptrs[] = {B.d_models, B.d_tabs, B.d_coefs, B.d_vecs, B.d_p10, B.d_inst,
-                    B.d_order, B.d_queue, B.ws, B.ws32};
+    void *ptrs[] = {B.d_models, B.d_tabs, B.d_coefs, B.d_vecs, B.d_p10, B.d_inst};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -228,13 +107,15 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
         B.tol = opt->tol > 0 ? opt->tol : 1e-8;
         B.anorm = opt->anorm > 0 ? opt->anorm : 1.0;
         B.mixed = opt->precision == FSP_MIXED;
-        if (opt->block) B.block = opt->block;
-        if (opt->wg_per_cu) B.wg_per_cu = opt->wg_per_cu;
+        if (opt->wg_per_cu > 0) B.pq_wg = std::min(opt->wg_per_cu, 8);
+        if (opt->lockstep_bpc > 0) B.ls_bpc = opt->lockstep_bpc;
+        if (opt->split_states != 0) B.pq_split = opt->split_states;
+        if (opt->time_limit_s > 0) B.pq_budget_s = opt->time_limit_s;
         if (opt->max_reject > 0) B.max_reject = opt->max_reject;
         if (opt->max_nstep > 0) B.max_nstep = opt->max_nstep;
         B.iop = opt->iop ? 1 : 0;
         if (opt->q_iop > 0) B.q_iop = opt->q_iop;
-        if (opt->path < FSP_BATCH_AUTO || opt->path > FSP_BATCH_PERSISTENT)
+        if (opt->path < FSP_BATCH_AUTO || opt->path > FSP_BATCH_HYBRID)
             return bail(fail(FSP_ERR_INVALID, "unknown batch path"));
         B.path_req = opt->path;
     }
@@ -254,7 +135,6 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
     B.h_inst.resize((size_t)count);
     std::vector<double> cost((size_t)count);
     B.dmax = 1;
-    int max_tab = 0;
     for (int i = 0; i < count; ++i) {
         fsp_status st = build_host_model(&models[i], B.hms[i]);
         if (st != FSP_OK) return bail(st);
@@ -263,7 +143,6 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
         B.dmax = std::max(B.dmax, hm.dm.d);
         tab_off[i] = (int64_t)tabs.size();
         tabs.insert(tabs.end(), hm.compact.begin(), hm.compact.end());
-        max_tab = std::max(max_tab, hm.dm.compact_size);
         coef_off[i] = (int64_t)cvec.size();
         int64_t cprev = 0;
         for (int j = 0; j < i; ++j) cprev += B.hms[j].dm.R;
@@ -300,9 +179,6 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
         I.tab_size = hm.dm.compact_size;
         cost[i] = (double)nv * (1.0 + t_final[i] * outflow_bound(hm, coefs + cprev) / B.m);
     }
-    // LDS factor tables when they fit next to the Pade workspace
-    const size_t base = batch_lds_bytes(B, 1024);
-    B.tab_lds = (base + sizeof(double) * max_tab <= 150 * 1024) ? max_tab : 0;
     FSP_BATCH_HIP(hipMalloc(&B.d_models, sizeof(DevModel) * (size_t)count));
     for (int i = 0; i < count; ++i)
         FSP_BATCH_HIP(copy_sync(B.d_models + i, &B.hms[i].dm, sizeof(DevModel), hipMemcpyHostToDevice, nullptr));
@@ -335,9 +211,6 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
     B.order = order;
     B.cost = cost;
-    FSP_BATCH_HIP(hipMalloc(&B.d_order, sizeof(int) * (size_t)count));
-    FSP_BATCH_HIP(copy_sync(B.d_order, order.data(), sizeof(int) * (size_t)count, hipMemcpyHostToDevice, nullptr));
-    FSP_BATCH_HIP(hipMalloc(&B.d_queue, sizeof(unsigned int)));
     std::vector<double> p10(kP10Hi - kP10Lo + 1);
     for (int k = kP10Lo; k <= kP10Hi; ++k) {
         char buf[16];
@@ -346,25 +219,6 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
     }
     FSP_BATCH_HIP(hipMalloc(&B.d_p10, sizeof(double) * p10.size()));
     FSP_BATCH_HIP(copy_sync(B.d_p10, p10.data(), sizeof(double) * p10.size(), hipMemcpyHostToDevice, nullptr));
-    // workspace for the largest grid any configuration may use (2 per CU);
-    // none where the persistent kernel is not launched
-    bool persistent = false;
-    {
-        const fsp_status pst = persistent_allowed(&persistent);
-        if (pst != FSP_OK) return bail(pst);
-    }
-    B.ws_slots = persistent ? (size_t)std::min(count, B.cus * 2) : 0;
-    const size_t nvm = (size_t)B.nv_max;
-    B.ws_stride = (B.mixed ? 3 : (size_t)(B.m + 3)) * nvm;
-    B.ws32_stride = B.mixed ? (size_t)B.m * nvm : 0;
-    hipError_t e = hipSuccess;
-    if (B.ws_slots) e = hipMalloc(&B.ws, sizeof(double) * B.ws_stride * B.ws_slots);
-    if (e != hipSuccess) return bail(fail(FSP_ERR_NOMEM, "batch workspace"));
-    if (B.mixed && B.ws_slots) {
-        e = hipMalloc(&B.ws32, sizeof(float) * B.ws32_stride * B.ws_slots);
-        if (e != hipSuccess) return bail(fail(FSP_ERR_NOMEM, "batch workspace"));
-    }
-    if (B.wg_per_cu > 2) B.wg_per_cu = 2;
     (void)stream;
     *out = h;
     return FSP_OK;
@@ -393,14 +247,15 @@ extern "C" fsp_status fsp_batch_solve(fsp_batch b, void *stream)
 {
     using namespace fsp;
     if (!b) return fail(FSP_ERR_INVALID, "NULL batch");
-    int path = 0;
-    FSP_TRY(batch_path(b->impl, &path));
-    b->impl.solve_stream = (hipStream_t)stream;
-    if (path == FSP_BATCH_PERSISTENT) return batch_graph(b->impl, (hipStream_t)stream);
-    // lockstep and stream scheduler: synchronous, on their own streams
+    BatchImpl &B = b->impl;
+    const int path = batch_path(B);
+    B.solve_stream = (hipStream_t)stream;
+    if (path == FSP_BATCH_PERSISTENT) return batch_launch_persistent(B, B.pq_wg, (hipStream_t)stream);
+    // lockstep, hybrid and stream scheduler: synchronous, on their own streams
     FSP_HIP(hipStreamSynchronize((hipStream_t)stream));
-    if (path == FSP_BATCH_LOCKSTEP) return batch_solve_lockstep(b->impl, b->impl.ls_bpc);
-    return batch_solve_streams(b->impl);
+    if (path == FSP_BATCH_LOCKSTEP) return batch_solve_lockstep(B, B.ls_bpc);
+    if (path == FSP_BATCH_HYBRID) return batch_solve_hybrid(B, B.ls_bpc, B.pq_wg, B.pq_split);
+    return batch_solve_streams(B);
 }
 
 extern "C" fsp_status fsp_batch_get_solution(fsp_batch b, int i, double *h_p, int64_t cap)
@@ -435,37 +290,114 @@ extern "C" fsp_status fsp_batch_get_config(fsp_batch b, int *block, int *wg_per_
 {
     using namespace fsp;
     if (!b) return fail(FSP_ERR_INVALID, "NULL batch");
-    int path = 0;
-    FSP_TRY(batch_path(b->impl, &path));
     const BatchImpl &B = b->impl;
+    const int path = batch_path(B);
     // persistent: (workgroup size, workgroups per CU); lockstep: (-1,
-    // row-kernel workgroups per CU); streams: (0, concurrent streams)
-    const int bs = path == FSP_BATCH_PERSISTENT ? B.block : (path == FSP_BATCH_LOCKSTEP ? -1 : 0);
-    const int wg = path == FSP_BATCH_PERSISTENT ? B.wg_per_cu
-                                                : (path == FSP_BATCH_LOCKSTEP ? B.ls_bpc : B.workers);
+    // row-kernel workgroups per CU); hybrid: (-2, row-kernel workgroups per
+    // CU); streams: (0, concurrent streams)
+    int bs = 0, wg = B.workers;
+    if (path == FSP_BATCH_PERSISTENT) {
+        bs = kBlock;
+        wg = B.pq_wg;
+    } else if (path == FSP_BATCH_LOCKSTEP) {
+        bs = -1;
+        wg = B.ls_bpc;
+    } else if (path == FSP_BATCH_HYBRID) {
+        bs = -2;
+        wg = B.ls_bpc;
+    }
     if (block) *block = bs;
     if (wg_per_cu) *wg_per_cu = wg;
     return FSP_OK;
 }
 
-// Autotuner: times each candidate configuration on the batch itself and
-// keeps the fastest: (workgroup size, workgroups per CU) for the persistent
-// kernel, row-kernel workgroups per CU for the lockstep path, the number of
-// concurrent streams for the stream scheduler.  The choice is cached in a
-// JSON file keyed by device, instance count, longest vector, species, m,
-// precision and path family.
-static void apply_tuned(BatchImpl &B, int path, int blk, int wg, int wk)
+extern "C" fsp_status fsp_batch_get_tuning(fsp_batch b, int *path, int *lockstep_bpc, int *persistent_wg,
+                                           int64_t *split_states)
 {
-    if (path == FSP_BATCH_PERSISTENT) {
-        B.block = blk;
-        B.wg_per_cu = wg;
-    } else if (path == FSP_BATCH_LOCKSTEP) {
-        B.ls_bpc = wg;
-    } else {
-        B.workers = wk;
-    }
-    if (B.path_req == FSP_BATCH_AUTO) B.path_tuned = path;
+    using namespace fsp;
+    if (!b) return fail(FSP_ERR_INVALID, "NULL batch");
+    const BatchImpl &B = b->impl;
+    if (path) *path = batch_path(B);
+    if (lockstep_bpc) *lockstep_bpc = B.ls_bpc;
+    if (persistent_wg) *persistent_wg = B.pq_wg;
+    if (split_states) *split_states = B.pq_split;
+    return FSP_OK;
 }
+
+// Autotuner: times candidate configurations on the batch itself (one whole
+// solve each, after one untimed solve of the path) and keeps the fastest:
+//   LOCKSTEP    row-kernel workgroups per CU: one per tile, 4 or 16;
+//   PERSISTENT  persistent workgroups per CU: 1, 2 or 4;
+//   HYBRID      the lockstep grid as above (all instances on the lockstep
+//               path), then the split at 4,096 / 16,384 / 65,536 states;
+//   AUTO        every candidate above, and the path with them;
+//   STREAMS     1, 2 or 4 concurrent streams.
+// The choice is cached in a JSON file keyed by device, instance count,
+// longest vector, species, m, precision and path family.
+namespace {
+
+struct Tuned {
+    int path = FSP_BATCH_LOCKSTEP, bpc = 0, wg = 2, workers = 4;
+    long long split = 16384;
+    double ms = 1e300;
+};
+
+fsp_status time_solve(fsp::BatchImpl &B, int path, int bpc, int wg, long long split, double *ms)
+{
+    using namespace fsp;
+    auto run = [&]() -> fsp_status {
+        if (path == FSP_BATCH_LOCKSTEP) return batch_solve_lockstep(B, bpc);
+        if (path == FSP_BATCH_HYBRID) return batch_solve_hybrid(B, bpc, wg, split);
+        return batch_solve_hybrid(B, bpc, wg, -1);  // every instance in the persistent kernel
+    };
+    FSP_TRY(run());  // allocations, graph capture, warm caches
+    const auto t0 = std::chrono::steady_clock::now();
+    FSP_TRY(run());
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return FSP_OK;
+}
+
+fsp_status tune_family(fsp::BatchImpl &B, int fam, Tuned &best)
+{
+    using namespace fsp;
+    auto consider = [&](int path, int bpc, int wg, long long split) -> fsp_status {
+        double ms = 0;
+        FSP_TRY(time_solve(B, path, bpc, wg, split, &ms));
+        if (ms < best.ms) {
+            best.path = path;
+            best.bpc = bpc;
+            best.wg = wg;
+            best.split = split;
+            best.ms = ms;
+        }
+        return FSP_OK;
+    };
+    const int bpcs[3] = {0, 4, 16};
+    const int wgs[3] = {1, 2, 4};
+    const long long splits[3] = {4096, 16384, 65536};
+    if (fam == FSP_BATCH_LOCKSTEP || fam == FSP_BATCH_HYBRID || fam == FSP_BATCH_AUTO)
+        for (int bpc : bpcs) FSP_TRY(consider(FSP_BATCH_LOCKSTEP, bpc, B.pq_wg, 0));
+    if (fam == FSP_BATCH_PERSISTENT || fam == FSP_BATCH_AUTO)
+        for (int wg : wgs) FSP_TRY(consider(FSP_BATCH_PERSISTENT, B.ls_bpc, wg, -1));
+    if (fam == FSP_BATCH_HYBRID || fam == FSP_BATCH_AUTO) {
+        const Tuned ls = best;
+        const int bpc = ls.path == FSP_BATCH_LOCKSTEP ? ls.bpc : B.ls_bpc;
+        const int wg = ls.path == FSP_BATCH_PERSISTENT ? ls.wg : B.pq_wg;
+        for (long long sp : splits) FSP_TRY(consider(FSP_BATCH_HYBRID, bpc, wg, sp));
+    }
+    return FSP_OK;
+}
+
+void apply_tuned(fsp::BatchImpl &B, const Tuned &t)
+{
+    B.ls_bpc = t.bpc;
+    B.pq_wg = t.wg;
+    B.pq_split = t.split;
+    B.workers = t.workers;
+    if (B.path_req == FSP_BATCH_AUTO) B.path_tuned = t.path;
+}
+
+}  // namespace
 
 extern "C" fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, int force,
                                          double *best_ms)
@@ -473,14 +405,7 @@ extern "C" fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, in
     using namespace fsp;
     if (!b) return fail(FSP_ERR_INVALID, "NULL batch");
     BatchImpl &B = b->impl;
-    bool persistent = false;
-    FSP_TRY(persistent_allowed(&persistent));
-    // family of candidates: the requested path, or with AUTO the persistent
-    // kernel where it runs and the lockstep path on GPU devices
-    int fam = B.path_req;
-    if (fam == FSP_BATCH_AUTO) fam = persistent ? FSP_BATCH_PERSISTENT : FSP_BATCH_LOCKSTEP;
-    if (fam == FSP_BATCH_PERSISTENT && !persistent)
-        return fail(FSP_ERR_UNSUPPORTED, "the persistent batch kernel is not launched on GPU devices");
+    const int fam = B.path_req;
     hipDeviceProp_t prop;
     int dev = 0;
     FSP_HIP(hipGetDevice(&dev));
@@ -497,67 +422,35 @@ extern "C" fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, in
         const std::string k = "\"" + key.str() + "\": {";
         const size_t pos = cache.find(k);
         if (pos != std::string::npos && !force) {
-            int path = 0, blk = 0, wg = 0, wk = 0;
-            double ms = 0;
+            Tuned t;
             if (std::sscanf(cache.c_str() + pos + k.size(),
-                            "\"path\": %d, \"block\": %d, \"wg_per_cu\": %d, \"workers\": %d, \"ms\": %lf",
-                            &path, &blk, &wg, &wk, &ms) == 5) {
-                apply_tuned(B, path, blk, wg, wk);
-                if (best_ms) *best_ms = ms;
+                            "\"path\": %d, \"bpc\": %d, \"wg\": %d, \"split\": %lld, \"workers\": %d, "
+                            "\"ms\": %lf",
+                            &t.path, &t.bpc, &t.wg, &t.split, &t.workers, &t.ms) == 6) {
+                apply_tuned(B, t);
+                if (best_ms) *best_ms = t.ms;
                 return FSP_OK;
             }
         }
     }
-    double best = 1e300;
-    int bb = B.block, bw = B.wg_per_cu, bk = B.workers;
-    if (fam == FSP_BATCH_PERSISTENT) {
-        const int blocks[3] = {256, 512, 1024};
-        hipEvent_t e0, e1;
-        FSP_HIP(hipEventCreate(&e0));
-        FSP_HIP(hipEventCreate(&e1));
-        for (int bs : blocks) {
-            for (int wg = 1; wg <= 2; ++wg) {
-                if (batch_lds_bytes(B, bs) * wg > 160 * 1024) continue;
-                FSP_TRY(enqueue_batch(B, bs, wg, nullptr));  // warm-up
-                FSP_HIP(hipEventRecord(e0, nullptr));
-                FSP_TRY(enqueue_batch(B, bs, wg, nullptr));
-                FSP_HIP(hipEventRecord(e1, nullptr));
-                FSP_HIP(hipEventSynchronize(e1));
-                float ms = 0;
-                FSP_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) {
-                    best = ms;
-                    bb = bs;
-                    bw = wg;
-                }
-            }
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-    } else if (fam == FSP_BATCH_LOCKSTEP) {
-        const int bpcs[3] = {0, 4, 16};  // 0: one workgroup per tile
-        for (int bpc : bpcs) {
-            FSP_TRY(batch_lockstep_prepare(B, bpc));
-            const auto t0 = std::chrono::steady_clock::now();
-            FSP_TRY(batch_solve_lockstep(B, bpc));
-            const double ms =
-                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            if (ms < best) {
-                best = ms;
-                bw = bpc;
-            }
-        }
-        bb = -1;
+    Tuned best;
+    best.bpc = B.ls_bpc;
+    best.wg = B.pq_wg;
+    best.split = B.pq_split;
+    best.workers = B.workers;
+    if (fam == FSP_BATCH_STREAMS) {
+        FSP_TRY(batch_autotune_streams(B, &best.ms, &best.workers));
+        best.path = FSP_BATCH_STREAMS;
     } else {
-        FSP_TRY(batch_autotune_streams(B, &best, &bk));
-        bb = 0;
+        FSP_TRY(tune_family(B, fam, best));
     }
-    apply_tuned(B, fam, bb, bw, bk);
-    if (best_ms) *best_ms = best;
+    apply_tuned(B, best);
+    if (best_ms) *best_ms = best.ms;
     if (cache_path) {
         std::ostringstream ent;
-        ent << "\"" << key.str() << "\": {\"path\": " << fam << ", \"block\": " << bb
-            << ", \"wg_per_cu\": " << bw << ", \"workers\": " << bk << ", \"ms\": " << best << "}";
+        ent << "\"" << key.str() << "\": {\"path\": " << best.path << ", \"bpc\": " << best.bpc
+            << ", \"wg\": " << best.wg << ", \"split\": " << best.split << ", \"workers\": "
+            << best.workers << ", \"ms\": " << best.ms << "}";
         std::string body;
         const std::string k = "\"" + key.str() + "\": {";
         const size_t pos = cache.find(k);

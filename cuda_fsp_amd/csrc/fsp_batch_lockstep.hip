@@ -54,6 +54,7 @@ struct LsArgs {
     int *atiles;            // tiles of the running instances
     int *n_atiles;          // their count; [1]: running instances
     double *partials;       // 2 max(2m + 4, kLsRpt) T doubles
+    const int *owner;       // per instance: 1 when the persistent kernel solves it
     int T, count, m, M2;
     int j, istart, mode, nk; // Arnoldi iteration, first vector orthogonalized against,
                              // mode (0: dots, 1: avnorm), dots this iteration
@@ -140,7 +141,7 @@ __global__ void __launch_bounds__(1024) k_ls_compact(LsArgs a)
         int f = 0, nt = 0;
         if (i < a.count) {
             const KState *st = a.st + i;
-            f = st->t_now < st->t_final && st->status == 0;
+            f = st->t_now < st->t_final && st->status == 0 && !a.owner[i];
             nt = a.inst[i].ntiles;
         }
         flag[threadIdx.x] = f;
@@ -198,6 +199,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_step_begin(LsArgs a, long long ma
 {
     __shared__ double red[2 * kBlock / kWave];
     const int i = blockIdx.x;
+    if (a.owner[i]) return;
     const LsInst &I = a.inst[i];
     kry_step_begin(a.st + i, a.partials + 2 * (size_t)I.tile0, I.ntiles, a.m, a.M2, max_nstep, red);
 }
@@ -363,6 +365,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_dots(LsArgs a)
 {
     __shared__ double red[2 * kBlock / kWave];
     const int i = blockIdx.x / a.nk, k = blockIdx.x % a.nk;
+    if (a.owner[i]) return;
     KState *st = a.st + i;
     if (!st->active || st->broke || (a.mode == 1 && st->k1 == 0)) return;
     const LsInst &I = a.inst[i];
@@ -412,6 +415,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_dots(LsArgs a)
 // FP32 basis: per instance, H(., j) <- 2h - G h (gram_coefs).
 __global__ void __launch_bounds__(kBlock) k_ls_gram_coef(LsArgs a)
 {
+    if (a.owner[blockIdx.x]) return;
     KState *st = a.st + blockIdx.x;
     if (!st->active || st->broke) return;
     FSP_DYNAMIC_LDS(lds);
@@ -471,6 +475,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_norm(LsArgs a)
 {
     __shared__ double red[2 * kBlock / kWave];
     const int i = blockIdx.x;
+    if (a.owner[i]) return;
     KState *st = a.st + i;
     if (!st->active || st->broke) return;
     const LsInst &I = a.inst[i];
@@ -492,6 +497,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_norm(LsArgs a)
 
 __global__ void __launch_bounds__(kBlock) k_ls_control(LsArgs a, int max_reject)
 {
+    if (a.owner[blockIdx.x]) return;
     FSP_DYNAMIC_LDS(smem);
     kry_control(a.st + blockIdx.x, a.m, max_reject, smem);
 }
@@ -599,12 +605,310 @@ __global__ void __launch_bounds__(kBlock) k_ls_running(LsArgs a, int *out)
     int c = 0;
     for (int i = threadIdx.x; i < a.count; i += blockDim.x) {
         const KState *st = a.st + i;
-        c += st->t_now < st->t_final && st->status == 0;
+        c += st->t_now < st->t_final && st->status == 0 && !a.owner[i];
     }
     atomicAdd(&cnt, c);
     __syncthreads();
     if (threadIdx.x == 0) *out = cnt;
 }
+
+// ------------------------------------------------------ persistent kernel
+// One workgroup solves one instance at a time, from t = 0 to its final
+// time, with the lockstep path's per-instance data (HYB generator, basis,
+// work vectors, KState) and its arithmetic (row sums, double-double dots
+// and norms, Gram-corrected coefficients, kry_control), so an FP64 instance
+// takes the same steps on either path.  Workgroups take instances from a
+// work queue in decreasing order of estimated cost.  No workgroup ever
+// waits for another (the queue's atomic increment is the only
+// cross-workgroup operation), every loop is bounded, and a workgroup that
+// has run for `budget` wall-clock ticks takes no further step: the instance
+// it holds and every one it takes afterwards end with FSP_ERR_TIMEOUT, so a
+// launch always returns.
+struct PqArgs {
+    const int *list;        // instances of the kernel, decreasing estimated cost
+    int count;
+    unsigned int *queue;    // next list position; 0 at launch
+    int max_reject, iop, q_iop;
+    long long max_nstep;
+    long long budget;       // wall-clock ticks
+};
+
+constexpr int kPqChunk = 8;  // dots per pass over the instance's rows
+
+// Slot k of red (kLsRedLen layout), merged over the waves in wave order.
+__device__ inline dd pq_slot(const double *red, int k)
+{
+    constexpr int NW = kBlock / kWave;
+    dd s = dd_zero();
+#pragma unroll
+    for (int q = 0; q < NW; ++q) s = dd_merge(s, dd{red[2 * (k * NW + q)], red[2 * (k * NW + q) + 1]});
+    return s;
+}
+
+// Step prologue of one instance (k_ls_norm_partials + kry_step_begin):
+// returns whether the instance takes a step (uniform over the workgroup).
+__device__ inline bool pq_step_begin(const LsInst &I, KState *st, int m, int M2, const PqArgs &q,
+                                     long long t0, double *red, int *flag)
+{
+    dd s2 = dd_zero();
+    for (int r = threadIdx.x; r < I.nv; r += kBlock) s2 = dd_add_prod(s2, I.p[r], I.p[r]);
+    const double s = dd_value(block_sum_dd(s2, red));
+    if (threadIdx.x == 0) {
+        int a = st->t_now < st->t_final && st->status == 0;
+        if (a && (long long)(wall_clock64() - t0) > q.budget) {
+            st->status = FSP_ERR_TIMEOUT;
+            a = 0;
+        }
+        if (a && st->i_step >= q.max_nstep) {
+            st->status = FSP_ERR_NOT_CONVERGED;
+            a = 0;
+        }
+        st->active = a;
+        if (a) {
+            st->beta = sqrt(s);
+            if (st->i_step == 0) {
+                st->xm = 1.0 / (double)m;
+                st->t_new = (1.0 / st->anorm) *
+                            pow((st->fact * st->tol) / (4.0 * st->beta * st->anorm), st->xm);
+                st->btol = st->anorm * st->tol;
+            }
+            st->mb = m;
+            st->tau = fmin(st->t_final - st->t_now, st->t_new);
+            st->broke = 0;
+            st->sigma[0] = 1.0 / st->beta;
+        }
+        *flag = a;
+    }
+    __syncthreads();
+    const bool a = *flag != 0;
+    if (a)
+        for (int k = threadIdx.x; k < M2 * M2; k += kBlock) st->H[k] = 0.0;
+    __syncthreads();
+    return a;
+}
+
+// Arnoldi iteration j of one instance (k_ls_arn + k_ls_dots): V_j, w_{j+1}
+// and column j of H (mode 0), or avnorm (mode 1).
+template <typename B>
+__device__ inline void pq_arnoldi(const LsInst &I, KState *st, int j, int istart, int mode, int M2,
+                                  double *red)
+{
+    const int nk = mode == 0 ? j - istart + 1 : 1;
+    const double *in = ls_in(I, j);
+    double *out = ls_out(I, j);
+    const double sig = st->sigma[j];
+    const KryX<B> X{in, sig, j};
+    B *Vb = ls_vb<B>(I);
+    const size_t nv = (size_t)I.nv;
+    const HybF<B> A = ls_hyb<B>(I, false);
+    dd sink = dd_zero(), nrm = dd_zero();
+    for (int r = threadIdx.x; r < I.n; r += kBlock) {
+        dd unused = dd_zero();
+        const double y = hyb_row<B>(A, (uint32_t)r, X, unused);
+        const double xr = X((uint32_t)r);
+        if (I.with_sink) sink = dd_add_prod(sink, I.hsink[r], xr);
+        store_basis<B>(I.V64, Vb, nv, j, (uint32_t)r, xr);
+        if (mode == 0)
+            out[r] = y;
+        else
+            nrm = dd_add_prod(nrm, y, y);
+    }
+    constexpr bool GRAM = gram_basis<B>();
+    if (mode == 0) {
+        // each thread re-reads the rows it wrote above
+        for (int c0 = 0; c0 < nk; c0 += kPqChunk) {
+            if (!GRAM) {
+                dd acc[kPqChunk];
+#pragma unroll
+                for (int v = 0; v < kPqChunk; ++v) acc[v] = dd_zero();
+                for (int r = threadIdx.x; r < I.n; r += kBlock) {
+                    const double y = out[r];
+                    double vk[kPqChunk];
+#pragma unroll
+                    for (int v = 0; v < kPqChunk; ++v)
+                        vk[v] = basis<B>(I.V64, Vb, nv, min(istart + c0 + v, j), (uint32_t)r);
+#pragma unroll
+                    for (int v = 0; v < kPqChunk; ++v)
+                        if (c0 + v < nk) acc[v] = dd_add_prod(acc[v], y, vk[v]);
+                }
+#pragma unroll
+                for (int v = 0; v < kPqChunk; ++v)
+                    if (c0 + v < nk) ls_put_wave(red, c0 + v, acc[v]);
+            } else {
+                double acc[kPqChunk], accg[kPqChunk];
+#pragma unroll
+                for (int v = 0; v < kPqChunk; ++v) acc[v] = accg[v] = 0.0;
+                for (int r = threadIdx.x; r < I.n; r += kBlock) {
+                    const double y = out[r];
+                    const double xj = basis<B>(I.V64, Vb, nv, j, (uint32_t)r);
+                    double vk[kPqChunk];
+#pragma unroll
+                    for (int v = 0; v < kPqChunk; ++v)
+                        vk[v] = basis<B>(I.V64, Vb, nv, min(istart + c0 + v, j), (uint32_t)r);
+#pragma unroll
+                    for (int v = 0; v < kPqChunk; ++v) {
+                        acc[v] = fma(y, vk[v], acc[v]);
+                        accg[v] = fma(xj, vk[v], accg[v]);
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < kPqChunk; ++v) {
+                    if (c0 + v < nk) {
+                        ls_put_wave_f(red, c0 + v, acc[v]);
+                        ls_put_wave_f(red, nk + 1 + c0 + v, accg[v]);
+                    }
+                }
+            }
+        }
+    } else {
+        ls_put_wave(red, 0, nrm);
+    }
+    ls_put_wave(red, nk, sink);
+    __syncthreads();
+    const int k = threadIdx.x;
+    if (k < nk) {
+        double ys = 0.0, xs = 0.0;
+        if (I.with_sink) {
+            ys = dd_value(pq_slot(red, nk));
+            xs = kry_scale<B>(in[I.n], sig, j);
+        }
+        dd sd = pq_slot(red, k);
+        if (mode == 0) {
+            dd sg = GRAM ? pq_slot(red, nk + 1 + k) : dd_zero();
+            const int kk = istart + k;
+            if (I.with_sink) {
+                const double vk = (kk == j) ? xs : basis<B>(I.V64, Vb, nv, kk, (uint32_t)I.n);
+                sd = dd_add_prod(sd, ys, vk);
+                if (GRAM) sg = dd_add_prod(sg, xs, vk);
+            }
+            st->H[kk + j * M2] = dd_value(sd);
+            if (GRAM) {
+                const double g = dd_value(sg);
+                st->G[kk + j * kGramLd] = g;
+                st->G[j + kk * kGramLd] = g;
+            }
+        } else {
+            st->avnorm = sqrt(dd_value(dd_add_prod(sd, ys, ys)));
+            st->H[(j + 1) + j * M2] = 1.0;  // src/KryExpvFSP.cu:115
+        }
+        if (k == 0) {
+            st->n_matvec += 1;
+            if (I.with_sink) {
+                if (mode == 0) out[I.n] = ys;
+                store_basis<B>(I.V64, Vb, nv, j, (uint32_t)I.n, xs);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// w_{j+1} -= V[istart..j] H(., j) and |w_{j+1}| (k_ls_update + k_ls_norm);
+// returns whether the Krylov space broke down (uniform).
+template <typename B>
+__device__ inline bool pq_update(const LsInst &I, KState *st, int j, int istart, int M2, double *red)
+{
+    const double *h = st->H + (size_t)j * M2;
+    double *out = ls_out(I, j);
+    const B *Vb = ls_vb<B>(I);
+    dd nrm = dd_zero();
+    for (int r = threadIdx.x; r < I.nv; r += kBlock) {
+        // k ascending as one sum; 8 basis loads issued together
+        double acc = 0.0;
+        int k = istart;
+        for (; k + kLsChunk * 2 <= j + 1; k += kLsChunk * 2) {
+            double v[kLsChunk * 2];
+#pragma unroll
+            for (int q = 0; q < kLsChunk * 2; ++q) v[q] = basis<B>(I.V64, Vb, (size_t)I.nv, k + q, (uint32_t)r);
+#pragma unroll
+            for (int q = 0; q < kLsChunk * 2; ++q) acc += v[q] * h[k + q];
+        }
+        for (; k <= j; ++k) acc += basis<B>(I.V64, Vb, (size_t)I.nv, k, (uint32_t)r) * h[k];
+        const double w = -1.0 * acc + out[r];
+        out[r] = w;
+        nrm = dd_add_prod(nrm, w, w);
+    }
+    const double s = sqrt(dd_value(block_sum_dd(nrm, red)));
+    const bool broke = s < st->btol;
+    if (threadIdx.x == 0) {
+        if (broke) {
+            st->k1 = 0;
+            st->mb = j + 1;
+            st->tau = st->t_final - st->t_now;
+            st->broke = 1;
+            st->n_breakdown += 1;
+        } else {
+            st->H[(j + 1) + j * M2] = s;
+            st->sigma[j + 1] = 1.0 / s;
+        }
+    }
+    __syncthreads();
+    return broke;
+}
+
+// p = beta V[:, 0:mx] F0 (k_ls_solution).
+template <typename B>
+__device__ inline void pq_solution(const LsInst &I, const KState *st)
+{
+    const B *Vb = ls_vb<B>(I);
+    const int mx = st->mx;
+    const double beta = st->beta;
+    for (int r = threadIdx.x; r < I.nv; r += kBlock) {
+        double acc = 0.0;
+        for (int k = 0; k < mx; ++k) acc += basis<B>(I.V64, Vb, (size_t)I.nv, k, (uint32_t)r) * st->F0[k];
+        I.p[r] = beta * acc;
+    }
+}
+
+// The whole solve of instance ii.  smem: kLsRedLen doubles of reduction
+// slots, then the control / Gram workspace.
+template <typename B>
+__device__ __forceinline__ void pq_solve(const LsArgs &a, const PqArgs &q, int ii, double *smem, long long t0)
+{
+    __shared__ int flag;
+    double *red = smem;
+    double *ctl = smem + kLsRedLen;
+    KState *st = a.st + ii;
+    const LsInst &I = a.inst[ii];
+    const int m = a.m, M2 = a.M2;
+    while (pq_step_begin(I, st, m, M2, q, t0, red, &flag)) {
+        int istart = 0;
+        for (int j = 0; j <= m; ++j) {
+            const bool last = (j == m);
+            if (last && st->k1 == 0) break;
+            if (!last && q.iop) istart = (j >= q.q_iop - 1) ? j - q.q_iop + 1 : 0;
+            pq_arnoldi<B>(I, st, j, last ? 0 : istart, last ? 1 : 0, M2, red);
+            if (last) break;
+            if (gram_basis<B>()) {
+                gram_coefs(st, istart, j, M2, ctl);
+                __syncthreads();
+            }
+            if (pq_update<B>(I, st, j, istart, M2, red)) break;
+        }
+        kry_control(st, m, q.max_reject, ctl);
+        __syncthreads();
+        pq_solution<B>(I, st);
+        __syncthreads();
+    }
+}
+
+template <typename B>
+__global__ void __launch_bounds__(kBlock) k_pq(LsArgs a, PqArgs q)
+{
+    FSP_DYNAMIC_LDS(smem);
+    __shared__ int idx;
+    __shared__ long long t0;
+    if (threadIdx.x == 0) t0 = (long long)wall_clock64();
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) idx = (int)atomicAdd(q.queue, 1u);
+        __syncthreads();
+        const int qi = idx;
+        if (qi >= q.count) return;
+        pq_solve<B>(a, q, q.list[qi], smem, t0);
+    }
+}
+
+__global__ void k_pq_reset(unsigned int *queue) { *queue = 0u; }
 
 }  // namespace
 
@@ -623,6 +927,23 @@ struct LsHost {
     hipStream_t s = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // persistent kernel: instance ownership, its work list and queue, its
+    // stream and graphs (the kernel alone for the hybrid solve; reset,
+    // kernel and export for the all-persistent launch)
+    int *d_owner = nullptr;
+    std::vector<int> h_owner, h_list;
+    int *d_pq_list = nullptr;
+    unsigned int *d_pq_queue = nullptr;
+    int pq_count = 0;
+    hipStream_t pq_s = nullptr;
+    hipEvent_t ev_reset = nullptr;
+    hipGraph_t pq_graph = nullptr;
+    hipGraphExec_t pq_gexec = nullptr;
+    int pq_graph_grid = 0;
+    hipGraph_t full_graph = nullptr;
+    hipGraphExec_t full_gexec = nullptr;
+    int full_graph_grid = 0;
+    long long ticks_per_s = 0;
 };
 
 static LsArgs ls_args(const BatchImpl &B, const LsHost &L)
@@ -635,6 +956,7 @@ static LsArgs ls_args(const BatchImpl &B, const LsHost &L)
     a.atiles = L.d_atiles;
     a.n_atiles = L.d_cnt;
     a.partials = L.d_partials;
+    a.owner = L.d_owner;
     a.T = L.T;
     a.count = B.count;
     a.m = B.m;
@@ -652,10 +974,16 @@ void batch_lockstep_free(BatchImpl &B)
     if (!L) return;
     if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
     if (L->graph) (void)hipGraphDestroy(L->graph);
+    if (L->pq_gexec) (void)hipGraphExecDestroy(L->pq_gexec);
+    if (L->pq_graph) (void)hipGraphDestroy(L->pq_graph);
+    if (L->full_gexec) (void)hipGraphExecDestroy(L->full_gexec);
+    if (L->full_graph) (void)hipGraphDestroy(L->full_graph);
+    if (L->ev_reset) (void)hipEventDestroy(L->ev_reset);
     if (L->s) (void)hipStreamDestroy(L->s);
+    if (L->pq_s) (void)hipStreamDestroy(L->pq_s);
     void *ptrs[] = {L->d_inst, L->d_st, L->d_tile_inst, L->d_tile_r0, L->d_atiles, L->d_cnt,
                     L->d_partials, L->d_basis, L->d_work, L->d_basis32, L->d_hdiag,
-                    L->d_hsink, L->d_hvals, L->d_hoff};
+                    L->d_hsink, L->d_hvals, L->d_hoff, L->d_owner, L->d_pq_list, L->d_pq_queue};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (L->h_cnt) (void)hipHostFree(L->h_cnt);
@@ -744,6 +1072,17 @@ static fsp_status ls_build(BatchImpl &B)
     FSP_HIP(copy_sync(L.d_tile_r0, tr0.data(), sizeof(int) * (size_t)L.T, hipMemcpyHostToDevice, nullptr));
     FSP_HIP(hipMalloc(&L.d_cnt, sizeof(int) * 4));
     FSP_HIP(fill_sync(L.d_cnt, 0, sizeof(int) * 4, nullptr));
+    FSP_HIP(hipMalloc(&L.d_owner, sizeof(int) * (size_t)count));
+    FSP_HIP(fill_sync(L.d_owner, 0, sizeof(int) * (size_t)count, nullptr));
+    L.h_owner.assign((size_t)count, 0);
+    FSP_HIP(hipMalloc(&L.d_pq_list, sizeof(int) * (size_t)count));
+    FSP_HIP(hipMalloc(&L.d_pq_queue, sizeof(unsigned int)));
+    {
+        int dev = 0, khz = 0;
+        FSP_HIP(hipGetDevice(&dev));
+        FSP_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        L.ticks_per_s = (long long)std::max(khz, 1) * 1000;
+    }
     FSP_HIP(hipHostMalloc((void **)&L.h_cnt, sizeof(int) * 4, hipHostMallocDefault));
     // dot partials: 2m + 4 slots of T (dots, sink, Gram column); the
     // update's quarter-tile norms: kLsRpt T
@@ -767,6 +1106,8 @@ static fsp_status ls_build(BatchImpl &B)
     FSP_HIP(hipMalloc(&L.d_st, sizeof(KState) * (size_t)count));
     FSP_HIP(copy_sync(L.d_st, hs.data(), sizeof(KState) * (size_t)count, hipMemcpyHostToDevice, nullptr));
     FSP_HIP(hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking));
+    FSP_HIP(hipStreamCreateWithFlags(&L.pq_s, hipStreamNonBlocking));
+    FSP_HIP(hipEventCreateWithFlags(&L.ev_reset, hipEventDisableTiming));
     L.grid = 0;
     // HYB generators
     const LsArgs a = ls_args(B, L);
@@ -874,46 +1215,185 @@ fsp_status batch_lockstep_prepare(BatchImpl &B, int blocks_per_cu)
     return ls_graph(B, *B.ls, ls_grid(B, *B.ls, blocks_per_cu));
 }
 
-fsp_status batch_solve_lockstep(BatchImpl &B, int blocks_per_cu)
+// Instances of at most `split` states (split < 0: every instance) are
+// owned by the persistent kernel, listed in decreasing estimated cost; the
+// lockstep kernels skip them.
+static fsp_status pq_assign(BatchImpl &B, LsHost &L, int64_t split)
 {
-    FSP_TRY(batch_lockstep_prepare(B, blocks_per_cu));
-    LsHost &L = *B.ls;
-    LsArgs a = ls_args(B, L);
-    hipLaunchKernelGGL(k_ls_reset, dim3(B.count), dim3(kBlock), 0, L.s, a);
-    FSP_HIP(hipGetLastError());
-    // steps are launched in groups ahead of the host check; a step with no
-    // running instance does nothing
-    const int ahead = 4;
-    for (long long guard = 0;; guard += ahead) {
-        for (int k = 0; k < ahead; ++k) {
-            if (L.gexec && ls_use_graph())
-                FSP_HIP(hipGraphLaunch(L.gexec, L.s));
-            else
-                FSP_TRY(ls_enqueue_step(B, L, ls_grid(B, L, blocks_per_cu), L.s));
+    std::vector<int> own((size_t)B.count, 0), list;
+    for (int i : B.order) {
+        if (split < 0 || (int64_t)B.h_inst[i].nv <= split) {
+            own[i] = 1;
+            list.push_back(i);
         }
-        hipLaunchKernelGGL(k_ls_running, dim3(1), dim3(kBlock), 0, L.s, a, L.d_cnt + 2);
-        FSP_HIP(hipGetLastError());
-        FSP_HIP(hipMemcpyAsync(L.h_cnt, L.d_cnt, sizeof(int) * 4, hipMemcpyDeviceToHost, L.s));
-        FSP_HIP(hipStreamSynchronize(L.s));
-        if (L.h_cnt[2] == 0) break;
-        if (guard > B.max_nstep + 8) return fail(FSP_ERR_NOT_CONVERGED, "batch steps exceeded max_nstep");
     }
-    hipLaunchKernelGGL(k_ls_export, dim3((B.count + kBlock - 1) / kBlock), dim3(kBlock), 0, L.s, a, B.d_inst);
+    if (own != L.h_owner) {
+        FSP_HIP(copy_sync(L.d_owner, own.data(), sizeof(int) * own.size(), hipMemcpyHostToDevice, nullptr));
+        L.h_owner = own;
+    }
+    if (list != L.h_list) {
+        if (!list.empty())
+            FSP_HIP(copy_sync(L.d_pq_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice,
+                              nullptr));
+        L.h_list = list;
+        L.pq_graph_grid = L.full_graph_grid = 0;  // graphs hold the count
+    }
+    L.pq_count = (int)list.size();
+    return FSP_OK;
+}
+
+static size_t pq_lds_bytes(const BatchImpl &B)
+{
+    const size_t M2 = (size_t)B.m + 2;
+    const size_t ctl = std::max(5 * M2 * M2, kGramLdsBytes / sizeof(double));
+    return sizeof(double) * (kLsRedLen + ctl);
+}
+
+static int pq_grid(const BatchImpl &B, const LsHost &L, int wg_per_cu)
+{
+    return std::max(1, std::min(L.pq_count, B.cus * std::max(1, wg_per_cu)));
+}
+
+template <typename Bt>
+static fsp_status pq_enqueue_t(BatchImpl &B, LsHost &L, int grid, hipStream_t s)
+{
+    const LsArgs a = ls_args(B, L);
+    PqArgs q;
+    q.list = L.d_pq_list;
+    q.count = L.pq_count;
+    q.queue = L.d_pq_queue;
+    q.max_reject = B.max_reject;
+    q.iop = B.iop;
+    q.q_iop = B.q_iop;
+    q.max_nstep = B.max_nstep;
+    q.budget = (long long)(B.pq_budget_s * (double)L.ticks_per_s);
+    hipLaunchKernelGGL(k_pq_reset, dim3(1), dim3(1), 0, s, L.d_pq_queue);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pq<Bt>), dim3(grid), dim3(kBlock), pq_lds_bytes(B), s, a, q);
     FSP_HIP(hipGetLastError());
-    FSP_HIP(hipStreamSynchronize(L.s));
-    // report the first failed instance
+    return FSP_OK;
+}
+
+static fsp_status pq_enqueue(BatchImpl &B, LsHost &L, int grid, hipStream_t s)
+{
+    return B.mixed ? pq_enqueue_t<float>(B, L, grid, s) : pq_enqueue_t<double>(B, L, grid, s);
+}
+
+// Captures `body` on a private stream into (*g, *ge).
+template <class F>
+static fsp_status capture(hipGraph_t *g, hipGraphExec_t *ge, F body)
+{
+    if (*ge) (void)hipGraphExecDestroy(*ge);
+    if (*g) (void)hipGraphDestroy(*g);
+    *ge = nullptr;
+    *g = nullptr;
+    hipStream_t cap = nullptr;
+    FSP_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    FSP_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    fsp_status st = body(cap);
+    hipGraph_t gg = nullptr;
+    hipError_t ce = hipStreamEndCapture(cap, &gg);
+    (void)hipStreamDestroy(cap);
+    FSP_TRY(st);
+    if (ce != hipSuccess) return fail(FSP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+    *g = gg;
+    FSP_HIP(hipGraphInstantiate(ge, gg, nullptr, nullptr, 0));
+    return FSP_OK;
+}
+
+static fsp_status instance_failures(BatchImpl &B)
+{
     std::vector<BatchInst> h((size_t)B.count);
     FSP_HIP(copy_sync(h.data(), B.d_inst, sizeof(BatchInst) * (size_t)B.count, hipMemcpyDeviceToHost, nullptr));
     for (int i = 0; i < B.count; ++i) {
         if (h[i].status != 0) {
-            char buf[160];
+            char buf[200];
+            const char *why = h[i].status == FSP_ERR_TIMEOUT ? "persistent-kernel wall-clock limit reached"
+                              : h[i].steps >= B.max_nstep   ? "step count reached max_nstep"
+                                                            : "Krylov reject loop exhausted max_reject";
             std::snprintf(buf, sizeof buf, "batch instance %d (%d states, t = %f): %s", i, h[i].nv,
-                          h[i].t_final,
-                          h[i].steps >= B.max_nstep ? "step count reached max_nstep"
-                                                    : "Krylov reject loop exhausted max_reject");
+                          h[i].t_final, why);
             return fail((fsp_status)h[i].status, buf);
         }
     }
+    return FSP_OK;
+}
+
+fsp_status batch_solve_hybrid(BatchImpl &B, int blocks_per_cu, int wg_per_cu, int64_t split)
+{
+    FSP_TRY(ls_build(B));
+    LsHost &L = *B.ls;
+    FSP_TRY(pq_assign(B, L, split));
+    const int n_pq = L.pq_count, n_ls = B.count - L.pq_count;
+    if (n_ls > 0) FSP_TRY(batch_lockstep_prepare(B, blocks_per_cu));
+    const int pgrid = pq_grid(B, L, wg_per_cu);
+    if (n_pq > 0 && ls_use_graph() && (!L.pq_gexec || L.pq_graph_grid != pgrid)) {
+        FSP_TRY(capture(&L.pq_graph, &L.pq_gexec, [&](hipStream_t c) { return pq_enqueue(B, L, pgrid, c); }));
+        L.pq_graph_grid = pgrid;
+    }
+    LsArgs a = ls_args(B, L);
+    hipLaunchKernelGGL(k_ls_reset, dim3(B.count), dim3(kBlock), 0, L.s, a);
+    FSP_HIP(hipGetLastError());
+    if (n_pq > 0) {
+        // the persistent kernel on its own stream, beside the lockstep steps
+        FSP_HIP(hipEventRecord(L.ev_reset, L.s));
+        FSP_HIP(hipStreamWaitEvent(L.pq_s, L.ev_reset, 0));
+        if (ls_use_graph())
+            FSP_HIP(hipGraphLaunch(L.pq_gexec, L.pq_s));
+        else
+            FSP_TRY(pq_enqueue(B, L, pgrid, L.pq_s));
+    }
+    if (n_ls > 0) {
+        // steps are launched in groups ahead of the host check; a step with
+        // no running instance does nothing
+        const int ahead = 4;
+        for (long long guard = 0;; guard += ahead) {
+            for (int k = 0; k < ahead; ++k) {
+                if (L.gexec && ls_use_graph())
+                    FSP_HIP(hipGraphLaunch(L.gexec, L.s));
+                else
+                    FSP_TRY(ls_enqueue_step(B, L, ls_grid(B, L, blocks_per_cu), L.s));
+            }
+            hipLaunchKernelGGL(k_ls_running, dim3(1), dim3(kBlock), 0, L.s, a, L.d_cnt + 2);
+            FSP_HIP(hipGetLastError());
+            FSP_HIP(hipMemcpyAsync(L.h_cnt, L.d_cnt, sizeof(int) * 4, hipMemcpyDeviceToHost, L.s));
+            FSP_HIP(hipStreamSynchronize(L.s));
+            if (L.h_cnt[2] == 0) break;
+            if (guard > B.max_nstep + 8) return fail(FSP_ERR_NOT_CONVERGED, "batch steps exceeded max_nstep");
+        }
+    }
+    if (n_pq > 0) FSP_HIP(hipStreamSynchronize(L.pq_s));
+    hipLaunchKernelGGL(k_ls_export, dim3((B.count + kBlock - 1) / kBlock), dim3(kBlock), 0, L.s, a, B.d_inst);
+    FSP_HIP(hipGetLastError());
+    FSP_HIP(hipStreamSynchronize(L.s));
+    return instance_failures(B);
+}
+
+fsp_status batch_solve_lockstep(BatchImpl &B, int blocks_per_cu)
+{
+    return batch_solve_hybrid(B, blocks_per_cu, 1, 0);
+}
+
+fsp_status batch_launch_persistent(BatchImpl &B, int wg_per_cu, hipStream_t stream)
+{
+    FSP_TRY(ls_build(B));
+    LsHost &L = *B.ls;
+    FSP_TRY(pq_assign(B, L, -1));
+    const int grid = pq_grid(B, L, wg_per_cu);
+    auto body = [&](hipStream_t c) -> fsp_status {
+        const LsArgs a = ls_args(B, L);
+        hipLaunchKernelGGL(k_ls_reset, dim3(B.count), dim3(kBlock), 0, c, a);
+        FSP_TRY(pq_enqueue(B, L, grid, c));
+        hipLaunchKernelGGL(k_ls_export, dim3((B.count + kBlock - 1) / kBlock), dim3(kBlock), 0, c, a,
+                           B.d_inst);
+        FSP_HIP(hipGetLastError());
+        return FSP_OK;
+    };
+    if (!ls_use_graph()) return body(stream);
+    if (!L.full_gexec || L.full_graph_grid != grid) {
+        FSP_TRY(capture(&L.full_graph, &L.full_gexec, body));
+        L.full_graph_grid = grid;
+    }
+    FSP_HIP(hipGraphLaunch(L.full_gexec, stream));
     return FSP_OK;
 }
 

@@ -170,7 +170,7 @@ __device__ inline void kry_step_begin(KState *st, const double *partials, int G,
 
 // Error estimate, reject loop and step update (src/KryExpvFSP.cu:122-174),
 // one workgroup; smem: 5 (m+2)^2 doubles of LDS.
-__device__ inline void kry_control(KState *st, int m, int max_reject, double *smem)
+__device__ __forceinline__ void kry_control(KState *st, int m, int max_reject, double *smem)
 {
     if (!st->active) return;
     const int M2 = m + 2;

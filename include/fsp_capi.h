@@ -29,7 +29,8 @@ typedef enum fsp_status {
     FSP_ERR_HIP = 4,           /* HIP runtime error                            */
     FSP_ERR_NOMEM = 5,         /* device allocation failed                     */
     FSP_ERR_NOT_CONVERGED = 6, /* reject / step cap exhausted                  */
-    FSP_ERR_NO_DEVICE = 7      /* no GPU visible                               */
+    FSP_ERR_NO_DEVICE = 7,     /* no GPU visible                               */
+    FSP_ERR_TIMEOUT = 8        /* a batch's persistent kernel hit its time limit */
 } fsp_status;
 
 /* src/FSPMat.h:17-19 (enum MatrixFormat {CUDA_CSR, HYB, KRONECKER}) */
@@ -233,24 +234,33 @@ typedef struct fsp_batch_options {
     double anorm;     /* norm estimate (default 1.0, as the reference)         */
     int precision;    /* FSP_F64, or FSP_MIXED: FP32 Krylov basis              */
     int iop, q_iop;   /* incomplete orthogonalisation                          */
-    int block;        /* workgroup size 256/512/1024 (0: 512, or autotuned)    */
-    int wg_per_cu;    /* persistent workgroups per CU, 1 or 2 (0: 2)           */
+    int lockstep_bpc; /* lockstep row-kernel workgroups per CU (0: one per
+                         1024-row tile)                                        */
+    int wg_per_cu;    /* persistent-kernel workgroups per CU (0: 2, <= 8)      */
     int max_reject;   /* 0: 1000                                               */
     int64_t max_nstep;/* 0: 100000                                             */
     int path;         /* FSP_BATCH_AUTO (0) or one of the paths below          */
+    int64_t split_states; /* HYBRID: instances of at most this many entries go
+                         to the persistent kernel (0: 16384; < 0: all)         */
+    double time_limit_s;  /* wall-clock seconds a persistent workgroup may run
+                         before its instances end with FSP_ERR_TIMEOUT (0: 600) */
 } fsp_batch_options;
 
 /* Batch execution paths.  LOCKSTEP: one graph-captured batch step advances
  * every running instance by one Krylov step (row kernels over the 1024-row
- * tiles of all running instances; per-instance step control).  STREAMS:
- * each instance through the single-solve engine, up to four at a time on
- * their own streams.  PERSISTENT: one workgroup per instance in a
- * persistent work-queue kernel; refused on GPU devices.  AUTO: the
- * autotuner's choice, else LOCKSTEP on GPU devices and PERSISTENT elsewhere. */
+ * tiles of all running instances; per-instance step control).  PERSISTENT:
+ * one persistent work-queue kernel in which each workgroup solves whole
+ * instances, largest estimated cost first; the launch (reset, kernel, stats
+ * export) is one captured HIP graph.  HYBRID: the persistent kernel takes
+ * the instances of at most split_states entries and the lockstep steps the
+ * others, at the same time on two streams.  STREAMS: each instance through
+ * the single-solve engine, up to four at a time on their own streams.
+ * AUTO: the autotuner's choice, else HYBRID. */
 #define FSP_BATCH_AUTO 0
 #define FSP_BATCH_LOCKSTEP 1
 #define FSP_BATCH_STREAMS 2
 #define FSP_BATCH_PERSISTENT 3
+#define FSP_BATCH_HYBRID 4
 
 /* count independent solves v_i <- exp(t_final[i] A_i) v_i (Krylov expv,
  * KryExpvFSP semantics, each instance with its own step sizes).  models[i]
@@ -263,26 +273,32 @@ fsp_status fsp_batch_create(const fsp_model *models, const double *coefs,
                             fsp_batch *out);
 fsp_status fsp_batch_destroy(fsp_batch b);
 fsp_status fsp_batch_set_initial(fsp_batch b, int i, const double *h_p0);
-/* One solve of every instance from its initial vector.  PERSISTENT: a HIP
- * graph (queue reset + persistent kernel) launched on `stream`
- * (asynchronous).  LOCKSTEP and STREAMS wait for `stream`, run on the
- * batch's own streams and return when every instance is done; an instance
- * that fails (reject loop or step limit) makes the call return its status
- * and a message naming it, the others' results stay valid. */
+/* One solve of every instance from its initial vector.  PERSISTENT: the
+ * captured graph launched on `stream` (asynchronous; the getters wait for
+ * it, and an instance's status reports its failure).  LOCKSTEP, HYBRID and
+ * STREAMS wait for `stream`, run on the batch's own streams and return when
+ * every instance is done; an instance that fails (reject loop, step limit,
+ * time limit) makes the call return its status and a message naming it,
+ * the others' results stay valid. */
 fsp_status fsp_batch_solve(fsp_batch b, void *stream);
 fsp_status fsp_batch_get_solution(fsp_batch b, int i, double *h_p, int64_t cap);
 fsp_status fsp_batch_get_stats(fsp_batch b, int i, int64_t *steps, int64_t *rejects,
                                int *status);
 /* Configuration of the path in use: PERSISTENT (workgroup size, workgroups
  * per CU); LOCKSTEP (-1, row-kernel workgroups per CU, 0 = one per 1024-row
- * tile); STREAMS (0, concurrent streams). */
+ * tile); HYBRID (-2, row-kernel workgroups per CU); STREAMS (0, concurrent
+ * streams). */
 fsp_status fsp_batch_get_config(fsp_batch b, int *block, int *wg_per_cu);
-/* Times every candidate configuration on this batch (PERSISTENT: workgroup
- * size x workgroups per CU; on GPU devices with path AUTO or LOCKSTEP:
- * one row-kernel workgroup per tile, 4 or 16 per CU; STREAMS: 1, 2 or 4
- * streams), keeps the
- * fastest and records it in the JSON file cache_path (NULL: no cache); a
- * cached entry for the same device / shape is reused unless force. */
+/* The path a solve takes and every tuned setting (any pointer may be NULL). */
+fsp_status fsp_batch_get_tuning(fsp_batch b, int *path, int *lockstep_bpc, int *persistent_wg,
+                                int64_t *split_states);
+/* Times candidate configurations on this batch, one whole solve each after
+ * an untimed one (LOCKSTEP: row-kernel workgroups one per tile, 4 or 16 per
+ * CU; PERSISTENT: 1, 2 or 4 workgroups per CU; HYBRID: the lockstep grids,
+ * then splits at 4,096, 16,384 and 65,536 entries; AUTO: all of these and
+ * the path; STREAMS: 1, 2 or 4 streams), keeps the fastest and records it
+ * in the JSON file cache_path (NULL: no cache); a cached entry for the same
+ * device / shape is reused unless force. */
 fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, int force, double *best_ms);
 
 /* ---- time-varying right-hand side and BDF integration ---- */

@@ -308,6 +308,14 @@ inline T atomicAdd(T *p, T v)
     *p = old + v;
     return old;
 }
+// the device's constant 100 MHz clock: steady_clock in 10 ns ticks
+inline unsigned long long wall_clock64()
+{
+    return (unsigned long long)(std::chrono::duration_cast<std::chrono::nanoseconds>(
+                                    std::chrono::steady_clock::now().time_since_epoch())
+                                    .count() /
+                                10);
+}
 // clang's non-temporal load hint: a plain load on the CPU
 #define __builtin_nontemporal_load(p) (*(p))
 using std::isfinite;
@@ -327,7 +335,8 @@ enum hipMemcpyKind { hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDevice
                      hipMemcpyDeviceToDevice, hipMemcpyDefault };
 enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal,
                             hipStreamCaptureModeRelaxed };
-enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 1 };
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 1, hipDeviceAttributeWallClockRate = 2 };
+constexpr unsigned hipEventDisableTiming = 2;
 constexpr unsigned hipStreamNonBlocking = 1;
 
 struct hipDeviceProp_t {
@@ -362,7 +371,9 @@ inline hipError_t hipGetDevice(int *d)
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
 inline hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int)
 {
-    *v = a == hipDeviceAttributeMultiprocessorCount ? simt_cus() : 0;
+    *v = a == hipDeviceAttributeMultiprocessorCount ? simt_cus()
+         : a == hipDeviceAttributeWallClockRate     ? 100000  // kHz
+                                                    : 0;
     return hipSuccess;
 }
 inline hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int)
@@ -479,7 +490,10 @@ inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t)
     *e = std::chrono::steady_clock::now();
     return hipSuccess;
 }
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+// streams run their work at submission, so an event is always complete
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b)
 {
     *ms = std::chrono::duration<float, std::milli>(*b - *a).count();

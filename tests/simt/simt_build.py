@@ -17,7 +17,7 @@ LIB = os.path.join(OUT_DIR, "libfsp_batch_simt.so")
 SOURCES = [os.path.join(CSRC, "fsp_batch.hip"), os.path.join(CSRC, "fsp_batch_lockstep.hip"),
            os.path.join(CSRC, "fsp_model.hip"), os.path.join(HERE, "simt_support.cpp")]
 DEPS = SOURCES + [os.path.join(HERE, "hip", "hip_runtime.h")] + [
-    os.path.join(CSRC, h) for h in ("fsp_batch_kernel.h", "fsp_batch_impl.h", "fsp_expm.h", "fsp_formats.h",
+    os.path.join(CSRC, h) for h in ("fsp_batch_impl.h", "fsp_expm.h", "fsp_formats.h",
                                     "fsp_device.h", "fsp_internal.h", "fsp_krylov_dev.h")]
 
 

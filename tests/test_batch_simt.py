@@ -1,10 +1,12 @@
-"""The batched persistent solver (cuda_fsp_amd/csrc/fsp_batch.hip and
-fsp_batch_kernel.h, compiled unchanged for the CPU against the SIMT stand-in
-in tests/simt) against the oracle's Krylov restatement with the Kronecker
-operator: bit-identical solutions and step/reject counts, in both lane orders,
-for both workgroup sizes, with and without a sink state; the mixed form within
-1e-6 of FP64; graph replay; the autotuner cache.  Each workgroup runs as
-fibers on one host thread, so every instance here is small."""
+"""The batched solver (cuda_fsp_amd/csrc/fsp_batch.hip and
+fsp_batch_lockstep.hip, compiled unchanged for the CPU against the SIMT
+stand-in in tests/simt): the persistent work-queue kernel, the hybrid split
+and the lockstep path against the oracle's Krylov restatement with the HYB
+operator: bit-identical solutions and step/reject counts, in both lane
+orders, with and without a sink state; the mixed form within 1e-6 of FP64;
+graph replay; the persistent kernel's time limit; the autotuner cache.  Each
+workgroup runs as fibers on one host thread, so every instance here is
+small."""
 import ctypes as C
 import json
 import os
@@ -45,18 +47,22 @@ def _instances(k, seed=5):
 
 
 def _oracle(net, t, sink=False, p0=None):
-    opr = O.Operator(net, "kron_reaction", with_sink=sink)
+    # every path stores each generator in the HYB layout
+    opr = O.Operator(net, "hyb", with_sink=sink)
     v = M.delta_p0(net, with_sink=sink) if p0 is None else p0.copy()
     rc, out, st = O.expv_outputs(opr, v, [t])
     assert rc == 0
     return out[0], st
 
 
-@pytest.mark.parametrize("block", [256, 512])
-def test_simt_batch_f64_bit_exact(sim, block):
+@pytest.mark.parametrize("path,wg", [("persistent", 1), ("persistent", 2), ("hybrid", 2)])
+def test_simt_batch_f64_bit_exact(sim, path, wg):
     nets, tf = _instances(3)
     sinks = [False, True, False]
-    B = capi.Batch(nets, tf, with_sink=sinks, precision="f64", block=block)
+    # hybrid: the two smaller instances in the persistent kernel, the largest lockstep
+    split = sorted(n.n_states for n in nets)[1] + 1
+    B = capi.Batch(nets, tf, with_sink=sinks, precision="f64", path=path, wg_per_cu=wg,
+                   split_states=split)
     B.solve()
     for i, net in enumerate(nets):
         want, st = _oracle(net, tf[i], sinks[i])
@@ -68,8 +74,8 @@ def test_simt_batch_f64_bit_exact(sim, block):
 
 def test_simt_batch_mixed_and_replay(sim):
     nets, tf = _instances(2, seed=9)
-    B64 = capi.Batch(nets, tf, precision="f64")
-    B32 = capi.Batch(nets, tf, precision="mixed")
+    B64 = capi.Batch(nets, tf, precision="f64", path="persistent")
+    B32 = capi.Batch(nets, tf, precision="mixed", path="persistent")
     B64.solve()
     B32.solve()
     ref = [B64.solution(i) for i in range(len(nets))]
@@ -84,17 +90,31 @@ def test_simt_batch_initial_vector_and_autotune(sim, tmp_path):
     net = M.birth_death((20, 15), k=3.0, g=0.8)
     p0 = np.zeros(net.n_states)
     p0[5 + 21 * 7] = 1.0
-    B = capi.Batch([net, net], [1.5, 1.5])
+    B = capi.Batch([net, net], [1.5, 1.5], path="persistent")
     B.set_initial(1, p0)
     path = str(tmp_path / "tune.json")
     assert B.autotune(path) > 0
     cfg = B.config()
     entry = list(json.load(open(path)).values())[0]
-    assert (entry["block"], entry["wg_per_cu"]) == cfg
+    assert cfg == (256, entry["wg"]) and entry["path"] == 3 and entry["wg"] in (1, 2, 4)
     B.solve()
     want, _ = _oracle(net, 1.5, p0=p0)
     assert np.array_equal(B.solution(1), want)
     assert np.abs(B.solution(0) - B.solution(1)).sum() > 1e-3
+
+
+def test_simt_persistent_time_limit(sim):
+    """A workgroup past its wall-clock limit takes no further step: its
+    instances end with FSP_ERR_TIMEOUT and the launch returns."""
+    nets, tf = _instances(2)
+    B = capi.Batch(nets, [50.0, 50.0], path="persistent", time_limit_s=1e-9)
+    B.solve()
+    for i in range(2):
+        steps, rej, status = B.stats(i)
+        assert status == 8 and steps <= 1
+    H = capi.Batch(nets, [50.0, 50.0], path="hybrid", split_states=-1, time_limit_s=1e-9)
+    with pytest.raises(capi.FspError, match="FSP_ERR_TIMEOUT"):
+        H.solve()
 
 
 def test_simt_lane_order_independent(simlib):
@@ -110,7 +130,7 @@ def test_simt_lane_order_independent(simlib):
         "    if hasattr(L, n): f = getattr(L, n); f.restype = r; f.argtypes = a\n"
         "capi._lib = L\n"
         "net = M.toggle_switch((14, 14), rates=M.TOGGLE_RATES)\n"
-        "B = capi.Batch([net], [2.0], with_sink=[True], block=256); B.solve()\n"
+        "B = capi.Batch([net], [2.0], with_sink=[True], path='persistent'); B.solve()\n"
         "L2 = capi.Batch([net, net], [2.0, 1.0], with_sink=[True, False], path='lockstep'); L2.solve()\n"
         "np.save(sys.argv[1], np.concatenate([B.solution(0), L2.solution(0), L2.solution(1)]))\n"
     ) % (os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
@@ -134,8 +154,8 @@ TOGGLE_CGS_RATES = (7.838583757547972, 0.5659944527754909, 13.088930136958103, 6
 
 def test_simt_batch_mixed_reorthogonalizes(sim):
     net = M.toggle_switch((20, 20), rates=TOGGLE_CGS_RATES)
-    B64 = capi.Batch([net], [4.0], precision="f64")
-    B32 = capi.Batch([net], [4.0], precision="mixed")
+    B64 = capi.Batch([net], [4.0], precision="f64", path="persistent")
+    B32 = capi.Batch([net], [4.0], precision="mixed", path="persistent")
     B64.solve()
     B32.solve()
     assert B64.stats(0)[2] == 0 and B32.stats(0)[2] == 0
@@ -201,7 +221,7 @@ def test_simt_lockstep_initial_vector_and_autotune(sim, tmp_path):
     assert ms > 0
     cfg = B.config()
     entry = list(json.load(open(path)).values())[0]
-    assert cfg[0] == -1 and cfg[1] in (0, 4, 16) and entry["path"] == 1 and entry["wg_per_cu"] == cfg[1]
+    assert cfg[0] == -1 and cfg[1] in (0, 4, 16) and entry["path"] == 1 and entry["bpc"] == cfg[1]
     B.solve()
     opr = O.Operator(net, "hyb")
     rc, out, st = O.expv_outputs(opr, p0.copy(), [1.5])

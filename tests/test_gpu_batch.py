@@ -1,15 +1,15 @@
-"""Batched solves on the device.  On GPU devices fsp_batch_solve takes the
-lockstep path by default: one graph-captured batch step advances every
-running instance by one Krylov step of its own (HYB generator filled from
-its model, own step sizes).  The stream scheduler (each instance through the
-single-solve engine, up to four at a time) is checked as well.  (The
-persistent one-workgroup-per-instance kernel is not launched on GPU devices;
-its source, and the lockstep source, are checked bit for bit against the
-oracle on the CPU in test_batch_simt.py.)  Checks: every instance against
-the oracle's Krylov restatement with the same operator (HYB for lockstep,
-Kronecker for the stream scheduler; same step counts, <= 1e-10 L1), the mixed form against FP64 (including rates that need
-the second Gram-Schmidt pass), repeated solves, the autotuner and its cache,
-per-instance initial vectors."""
+"""Batched solves on the device: the lockstep path (one graph-captured
+batch step advances every running instance by one Krylov step of its own),
+the persistent work-queue kernel (each workgroup solves whole instances; the
+launch is one captured graph), the hybrid of the two (the default) and the
+stream scheduler (each instance through the single-solve engine, up to four
+at a time).  Checks: every instance against the oracle's Krylov
+restatement with the same operator (HYB generators on the lockstep,
+persistent and hybrid paths, Kronecker for the stream scheduler; same step
+counts, <= 1e-10 L1), the mixed form against FP64 (including rates that
+need the second Gram-Schmidt pass), repeated solves, the autotuner and its
+cache, per-instance initial vectors.  The persistent kernel runs with a
+30 s wall-clock limit per workgroup, so a launch always returns."""
 import json
 import os
 
@@ -35,20 +35,23 @@ def _oracle(net, t, sink, fmt="kron"):
     return out[0], st
 
 
-@pytest.mark.parametrize("sink,path", [(False, "auto"), (True, "auto"), (True, "streams")])
+LIMIT_S = 30.0
+
+
+@pytest.mark.parametrize("sink,path", [(False, "lockstep"), (True, "lockstep"), (True, "streams")])
 def test_batch_f64_matches_oracle(sink, path):
     O.set_threads(8)
     nets, tf, ws = _small_batch(sink=sink)
     print(f"[case] batch f64 sink={sink} path={path} n={[n.n_states for n in nets][:6]}...", flush=True)
     B = capi.Batch(nets, tf, with_sink=ws, precision="f64", path=path)
-    if path == "auto":
-        assert B.config()[0] == -1  # lockstep
+    if path == "lockstep":
+        assert B.config()[0] == -1
     B.solve()
     print("[batch] solved", flush=True)
     worst = 0.0
     for i, net in enumerate(nets):
         # lockstep: HYB generators; stream scheduler: Kronecker operator
-        want, st = _oracle(net, tf[i], ws[i], "hyb" if path == "auto" else "kron")
+        want, st = _oracle(net, tf[i], ws[i], "hyb" if path == "lockstep" else "kron")
         got = B.solution(i)
         steps, rej, status = B.stats(i)
         assert status == 0
@@ -56,6 +59,34 @@ def test_batch_f64_matches_oracle(sink, path):
         worst = max(worst, float(np.abs(got - want).sum()))
     print(f"[batch] worst L1 vs oracle {worst:.3e}", flush=True)
     assert worst <= 1e-10
+
+
+@pytest.mark.parametrize("path,sink", [("persistent", True), ("hybrid", True), ("persistent", False)])
+def test_persistent_and_hybrid_match_oracle(path, sink):
+    """The persistent kernel (every instance) and the hybrid split (the
+    instances of at most 2,000 states persistent, the rest lockstep, on two
+    streams at once): same step counts as the oracle and <= 1e-10 L1."""
+    O.set_threads(8)
+    nets, tf, ws = _small_batch(count=20, seed=7, sink=sink)
+    print(f"[case] batch {path} sink={sink} n={[n.n_states for n in nets]}", flush=True)
+    B = capi.Batch(nets, tf, with_sink=ws, precision="f64", path=path, split_states=2000,
+                   time_limit_s=LIMIT_S)
+    B.solve()
+    print("[batch] solved", flush=True)
+    worst = 0.0
+    for i, net in enumerate(nets):
+        want, st = _oracle(net, tf[i], ws[i], "hyb")
+        steps, rej, status = B.stats(i)
+        assert status == 0, (i, status)
+        assert (steps, rej) == (st.i_step, st.n_reject), (i, steps, st.i_step)
+        worst = max(worst, float(np.abs(B.solution(i) - want).sum()))
+    print(f"[batch] worst L1 vs oracle {worst:.3e}", flush=True)
+    assert worst <= 1e-10
+    # a second launch of the captured graph restarts from the initial vectors
+    first = [B.solution(i) for i in range(len(nets))]
+    B.solve()
+    for i in range(len(nets)):
+        assert np.array_equal(B.solution(i), first[i])
 
 
 TOGGLE_CGS_RATES = (7.838583757547972, 0.5659944527754909, 13.088930136958103, 6.072545924918103)
@@ -67,8 +98,8 @@ def test_batch_mixed_vs_f64_and_replay():
     nets = nets + [M.toggle_switch((60, 60), rates=TOGGLE_CGS_RATES)]
     tf = np.append(tf, 20.0)
     print(f"[case] batch mixed n={[n.n_states for n in nets][:6]}...", flush=True)
-    B64 = capi.Batch(nets, tf, precision="f64")
-    B32 = capi.Batch(nets, tf, precision="mixed")
+    B64 = capi.Batch(nets, tf, precision="f64", time_limit_s=LIMIT_S)
+    B32 = capi.Batch(nets, tf, precision="mixed", time_limit_s=LIMIT_S)
     B64.solve()
     B32.solve()
     ref = [B64.solution(i) for i in range(len(nets))]
@@ -83,22 +114,24 @@ def test_batch_mixed_vs_f64_and_replay():
 
 def test_batch_autotune_cache(tmp_path):
     nets, tf, ws = _small_batch(count=12, seed=3)
-    B = capi.Batch(nets, tf, precision="mixed")
+    print(f"[case] autotune n={[n.n_states for n in nets]}", flush=True)
+    B = capi.Batch(nets, tf, precision="mixed", time_limit_s=LIMIT_S)
     path = str(tmp_path / "tune.json")
     ms = B.autotune(path)
     assert ms > 0
-    cfg = B.config()
     data = json.load(open(path))
     assert len(data) == 1
     entry = list(data.values())[0]
-    # lockstep: (-1, row-kernel workgroups per CU)
-    assert cfg[0] == -1 and cfg[1] in (0, 4, 16)
-    assert entry["path"] == 1 and entry["wg_per_cu"] == cfg[1]
-    B2 = capi.Batch(nets, tf, precision="mixed")
+    name, bpc, wg, split = B.tuning()
+    assert capi.BATCH_PATHS[name] == entry["path"] and entry["path"] in (1, 3, 4)
+    assert (bpc, wg, split) == (entry["bpc"], entry["wg"], entry["split"])
+    print(f"[batch] tuned {B.tuning()} {ms:.1f} ms", flush=True)
+    B2 = capi.Batch(nets, tf, precision="mixed", time_limit_s=LIMIT_S)
     assert B2.autotune(path) == pytest.approx(entry["ms"])
-    assert B2.config() == cfg
+    assert B2.tuning() == B.tuning()
     B2.solve()
-    ref = capi.Batch(nets, tf, precision="mixed")
+    ref = capi.Batch(nets, tf, precision="mixed", path=name, lockstep_bpc=bpc, wg_per_cu=wg,
+                     split_states=split, time_limit_s=LIMIT_S)
     ref.solve()
     for i in range(len(nets)):
         assert np.array_equal(B2.solution(i), ref.solution(i))
@@ -108,7 +141,7 @@ def test_batch_initial_vectors():
     net = M.birth_death((40, 30), k=3.0, g=0.8)
     p0 = np.zeros(net.n_states)
     p0[5 + 41 * 7] = 1.0
-    B = capi.Batch([net, net], [1.5, 1.5])
+    B = capi.Batch([net, net], [1.5, 1.5], time_limit_s=LIMIT_S)
     B.set_initial(1, p0)
     B.solve()
     opr = O.Operator(net, "hyb")
