@@ -614,7 +614,7 @@ class Expv:
 
 class Batch:
     """fsp_batch handle: many independent solves v_i <- exp(t_i A_i) v_i.
-    path: "auto" (the autotuner's choice, else "hybrid"), "lockstep" (one
+    path: "auto" (the autotuner's choice, else "lockstep"), "lockstep" (one
     graph-captured batch step advances every running instance),
     "persistent" (one persistent work-queue kernel, each workgroup solving
     whole instances), "hybrid" (the persistent kernel for instances of at

@@ -31,12 +31,15 @@ namespace fsp {
 
 
 // Path that fsp_batch_solve takes: the requested one, else the autotuner's
-// choice, else HYBRID.
+// choice, else LOCKSTEP (on the 512-instance uneven batch of 2,048 to
+// 262,144 states the lockstep path took 28.8 s against 28.9 s for the
+// hybrid split at 4,096 states and 32.1 s at 16,384: one workgroup streams
+// a mid-size instance's basis far slower than the whole device does).
 static int batch_path(const BatchImpl &B)
 {
     int p = B.path_req;
     if (p == FSP_BATCH_AUTO) p = B.path_tuned;
-    if (p == FSP_BATCH_AUTO) p = FSP_BATCH_HYBRID;
+    if (p == FSP_BATCH_AUTO) p = FSP_BATCH_LOCKSTEP;
     return p;
 }
 

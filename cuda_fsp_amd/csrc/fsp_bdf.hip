@@ -79,39 +79,6 @@ __global__ void k_wsq_partials(const double *__restrict__ x, const double *__res
 
 constexpr int kDotChunk = 8;  // dots per pass over the rows
 
-// partials[k][b] = dd sum over rows of block b of V_k . w, k < nk; 8 dots
-// per pass, so w is read once per 8 basis vectors and their loads are
-// issued together (each dot's sum runs over the rows in the same order)
-__global__ void k_multi_dot(const double *__restrict__ V, int64_t ldv, int nk,
-                            const double *__restrict__ w, int64_t n, double *partials)
-{
-    __shared__ double red[2 * kBlock / kWave];
-    for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
-        dd s[kDotChunk];
-#pragma unroll
-        for (int q = 0; q < kDotChunk; ++q) s[q] = dd_zero();
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-             i += (int64_t)gridDim.x * blockDim.x) {
-            const double wi = w[i];
-            double v[kDotChunk];
-#pragma unroll
-            for (int q = 0; q < kDotChunk; ++q) v[q] = V[(int64_t)min(c0 + q, nk - 1) * ldv + i];
-#pragma unroll
-            for (int q = 0; q < kDotChunk; ++q) s[q] = dd_add_prod(s[q], v[q], wi);
-        }
-#pragma unroll
-        for (int q = 0; q < kDotChunk; ++q) {
-            if (c0 + q >= nk) break;
-            const dd t = block_sum_dd(s[q], red);
-            if (threadIdx.x == 0) {
-                partials[2 * ((int64_t)(c0 + q) * gridDim.x + blockIdx.x)] = t.s;
-                partials[2 * ((int64_t)(c0 + q) * gridDim.x + blockIdx.x) + 1] = t.c;
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // sum_k c_k V_k[i], k ascending, 8 loads issued together
 __device__ inline double combo(const double *__restrict__ V, int64_t ldv, int nk,
                                const double *__restrict__ c, int64_t i)
@@ -129,24 +96,143 @@ __device__ inline double combo(const double *__restrict__ V, int64_t ldv, int nk
     return acc;
 }
 
-// out[k] = value of the dd sum of partials[k][0..G), one workgroup per k
-__global__ void k_reduce_partials(const double *__restrict__ partials, int G, double *out)
+// GMRES state of the current restart cycle in device memory.  The host
+// enqueues iterations without waiting for them; every kernel of an
+// iteration returns at once when `done` is set, so iterations launched past
+// the end of a cycle do nothing (the operator product itself still runs on
+// stale input and its result is not used).
+struct GmState {
+    int done;          // the cycle ended: converged, broke down or full
+    int conv;          // converged (or broke down: the exact solution is in the space)
+    int j;             // iterations taken in this cycle
+    int m;             // subspace dimension
+    double tol, sqn;   // residual test |g_{j+1}| / sqrt(n) <= tol
+    double inv_wn;     // 1 / |w| of the newest basis vector (1 for V_0)
+    double *H;         // (m + 1) x m upper triangular after the rotations
+    double *G;         // (m + 1) x (m + 1) Gram matrix V^T V
+    double *cs, *sn;   // rotations
+    double *g;         // rotated right-hand side, m + 1
+    double *c;         // Gram-Schmidt coefficients of the current column
+    double *z;         // back-substitution result, m
+};
+
+__global__ void k_gm_init(GmState *gm, double beta, double tol, double sqn)
 {
-    __shared__ double red[2 * kBlock / kWave];
-    const dd s = sum_dd_partials(partials + 2 * (int64_t)blockIdx.x * G, G, red);
-    if (threadIdx.x == 0) out[blockIdx.x] = dd_value(s);
+    if (threadIdx.x == 0) {
+        gm->tol = tol;
+        gm->sqn = sqn;
+        gm->done = 0;
+        gm->conv = 0;
+        gm->j = 0;
+        gm->inv_wn = 1.0;
+    }
+    for (int i = threadIdx.x; i <= gm->m; i += blockDim.x) gm->g[i] = i == 0 ? beta : 0.0;
 }
 
-// w -= sum_k h_k V_k (k ascending); dd partial sums of the new w squared
-__global__ void k_gs_update(const double *__restrict__ V, int64_t ldv, int nk,
-                            const double *__restrict__ h, double *__restrict__ w, int64_t n,
-                            double *partials)
+// V_j <- V_j / |w| (the scaling of the vector the previous iteration
+// orthogonalised; V_0 is already normalised) and the preconditioned,
+// unscaled input of the operator: t = V_j / ((alpha0 - A_ii) ewt).
+__global__ void k_gm_unscale(double *__restrict__ vj, const GmState *gm, const double *__restrict__ ewt,
+                             const double *__restrict__ diag, double alpha0, double *__restrict__ t,
+                             int64_t n)
 {
+    if (gm->done) return;
+    const double s = gm->inv_wn;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = vj[i] * s;
+        vj[i] = v;
+        t[i] = v / ((alpha0 - diag[i]) * ewt[i]);
+    }
+}
+
+// w = alpha0 x / q - ewt * u (D (alpha0 I - A) D^-1 Q^-1 applied to x,
+// u = A t from k_gm_unscale)
+__global__ void k_gm_post(const double *__restrict__ x, const double *__restrict__ ewt,
+                          const double *__restrict__ diag, const double *__restrict__ u, double alpha0,
+                          double *__restrict__ out, int64_t n, const GmState *gm)
+{
+    if (gm->done) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = alpha0 * (x[i] / (alpha0 - diag[i])) - ewt[i] * u[i];
+}
+
+// Partial dd sums over the rows of block b of h_k = V_k . w (slot k) and of
+// the Gram column V_k . V_j (slot nk + k), k < nk = j + 1, from the same
+// basis loads: 8 dots per pass over the rows.
+__global__ void k_gm_dots(const double *__restrict__ V, int64_t ldv, int nk, const double *__restrict__ w,
+                          int64_t n, double *partials, const GmState *gm)
+{
+    if (gm->done) return;
     __shared__ double red[2 * kBlock / kWave];
+    const double *vj = V + (int64_t)(nk - 1) * ldv;
+    for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
+        dd s[kDotChunk], q2[kDotChunk];
+#pragma unroll
+        for (int q = 0; q < kDotChunk; ++q) s[q] = q2[q] = dd_zero();
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * blockDim.x) {
+            const double wi = w[i], xj = vj[i];
+            double v[kDotChunk];
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q) v[q] = V[(int64_t)min(c0 + q, nk - 1) * ldv + i];
+#pragma unroll
+            for (int q = 0; q < kDotChunk; ++q) {
+                s[q] = dd_add_prod(s[q], v[q], wi);
+                q2[q] = dd_add_prod(q2[q], v[q], xj);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kDotChunk; ++q) {
+            if (c0 + q >= nk) break;
+            const dd t = block_sum_dd(s[q], red);
+            const dd u = block_sum_dd(q2[q], red);
+            if (threadIdx.x == 0) {
+                const int64_t o = (int64_t)(c0 + q) * gridDim.x + blockIdx.x;
+                const int64_t og = (int64_t)(nk + c0 + q) * gridDim.x + blockIdx.x;
+                partials[2 * o] = t.s;
+                partials[2 * o + 1] = t.c;
+                partials[2 * og] = u.s;
+                partials[2 * og + 1] = u.c;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// w -= V c with the Gram-corrected coefficients c = 2 h - G h (one sweep
+// doing the work of classical Gram-Schmidt's two passes: the second pass's
+// coefficients V^T (w - V h) are h - G h), |w|^2 partials.  Every workgroup
+// forms c in LDS from the reduced dots (hv: h then the new Gram column);
+// workgroup 0 also stores the Gram column and c for the rotation kernel.
+__global__ void k_gm_update(const double *__restrict__ V, int64_t ldv, int nk, const double *__restrict__ hv,
+                            double *__restrict__ w, int64_t n, double *partials, GmState *gm)
+{
+    if (gm->done) return;
+    __shared__ double red[2 * kBlock / kWave];
+    FSP_DYNAMIC_LDS(cl);  // nk doubles
+    const int j = nk - 1, ld = gm->m + 1;
+    for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+        double acc = 0.0;
+        for (int l = 0; l < nk; ++l) {
+            const double gkl = l == j ? hv[nk + k] : (k == j ? hv[nk + l] : gm->G[k + (int64_t)l * ld]);
+            acc += gkl * hv[l];
+        }
+        cl[k] = 2.0 * hv[k] - acc;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+            gm->G[k + (int64_t)j * ld] = hv[nk + k];
+            gm->G[j + (int64_t)k * ld] = hv[nk + k];
+            gm->c[k] = cl[k];
+        }
+    }
     dd s = dd_zero();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const double acc = combo(V, ldv, nk, h, i);
+        const double acc = combo(V, ldv, nk, cl, i);
         const double v = -1.0 * acc + w[i];
         w[i] = v;
         s = dd_add_prod(s, v, v);
@@ -155,6 +241,62 @@ __global__ void k_gs_update(const double *__restrict__ V, int64_t ldv, int nk,
     if (threadIdx.x == 0) {
         partials[2 * blockIdx.x] = s.s;
         partials[2 * blockIdx.x + 1] = s.c;
+    }
+}
+
+// out[k] = value of the dd sum of partials[k][0..G), one workgroup per k
+__global__ void k_reduce_partials(const double *__restrict__ partials, int G, double *out)
+{
+    __shared__ double red[2 * kBlock / kWave];
+    const dd s = sum_dd_partials(partials + 2 * (int64_t)blockIdx.x * G, G, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = dd_value(s);
+}
+
+// Iteration j's Hessenberg column (c_0..c_j, |w|) through the previous
+// rotations and a new one, the rotated right-hand side, the residual test
+// and the scale of the next basis vector; one workgroup.  |w|^2 is summed
+// here from the update kernel's G partials.
+__global__ void k_gm_givens(GmState *gm, int j, const double *partials, int G)
+{
+    if (gm->done) return;
+    __shared__ double red[2 * kBlock / kWave];
+    const double wn = sqrt(dd_value(sum_dd_partials(partials, G, red)));
+    if (threadIdx.x != 0) return;
+    const int m = gm->m;
+    double *h = gm->H + (int64_t)j * (m + 1);
+    for (int i = 0; i <= j; ++i) h[i] = gm->c[i];
+    const double hj1 = wn;  // subdiagonal entry: no earlier rotation reaches it
+    for (int i = 0; i < j; ++i) {
+        const double a = gm->cs[i] * h[i] + gm->sn[i] * h[i + 1];
+        h[i + 1] = -gm->sn[i] * h[i] + gm->cs[i] * h[i + 1];
+        h[i] = a;
+    }
+    const double rr = hypot(h[j], hj1);
+    const double cj = rr > 0.0 ? h[j] / rr : 1.0;
+    const double sj = rr > 0.0 ? hj1 / rr : 0.0;
+    gm->cs[j] = cj;
+    gm->sn[j] = sj;
+    h[j] = rr;
+    const double gj = gm->g[j];
+    gm->g[j + 1] = -sj * gj;
+    gm->g[j] = cj * gj;
+    gm->j = j + 1;
+    gm->inv_wn = wn > 0.0 ? 1.0 / wn : 0.0;
+    const bool conv = fabs(gm->g[j + 1]) / gm->sqn <= gm->tol || wn == 0.0;
+    gm->conv = conv ? 1 : 0;
+    gm->done = (conv || j + 1 == m) ? 1 : 0;
+}
+
+// z = H(0:j, 0:j)^-1 g(0:j) for the cycle's j iterations (back
+// substitution, one thread).
+__global__ void k_gm_backsub(GmState *gm)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int j = gm->j, ld = gm->m + 1;
+    for (int i = j - 1; i >= 0; --i) {
+        double a = gm->g[i];
+        for (int c = i + 1; c < j; ++c) a -= gm->H[i + (int64_t)c * ld] * gm->z[c];
+        gm->z[i] = a / gm->H[i + (int64_t)i * ld];
     }
 }
 
@@ -167,27 +309,8 @@ __global__ void k_scale_copy(const double *__restrict__ x, double s, double *__r
 }
 
 // Right preconditioner: Jacobi, q_i = alpha0 - A_ii (> 0, the diagonal of
-// the scaled stage operator D (alpha0 I - A) D^-1).
-// t = x / (q ewt): back to the unscaled space, preconditioned
-__global__ void k_unscale(const double *__restrict__ x, const double *__restrict__ ewt,
-                          const double *__restrict__ diag, double alpha0,
-                          double *__restrict__ t, int64_t n)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        t[i] = x[i] / ((alpha0 - diag[i]) * ewt[i]);
-}
-
-// out = alpha0 x / q - ewt * u   (D (alpha0 I - A) D^-1 Q^-1 applied to x,
-// u = A t from k_unscale)
-__global__ void k_op_post(const double *__restrict__ x, const double *__restrict__ ewt,
-                          const double *__restrict__ diag, const double *__restrict__ u,
-                          double alpha0, double *__restrict__ out, int64_t n)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = alpha0 * (x[i] / (alpha0 - diag[i])) - ewt[i] * u[i];
-}
+// the scaled stage operator D (alpha0 I - A) D^-1); k_gm_unscale and
+// k_gm_post apply it around the generator product.
 
 // r = ewt * (b - alpha0 y + u), u = A y   (scaled residual)
 __global__ void k_resid(const double *__restrict__ b, const double *__restrict__ y,
@@ -265,10 +388,15 @@ struct BdfImpl {
     double *ewt = nullptr, *b = nullptr, *yp = nullptr, *ynew = nullptr, *u = nullptr, *tmp = nullptr;
     double *diag = nullptr;         // generator diagonal at the stage time
     double *V = nullptr;            // (kdim + 1) x n scaled Krylov basis
-    double *partials = nullptr;     // 2 x (kdim + 1) x kRedGrid
-    double *dred = nullptr;         // 2 (kdim + 1) + 1 reduced values
+    double *partials = nullptr;     // 2 x 2 (kdim + 1) x kRedGrid (dd)
+    double *dred = nullptr;         // 2 (kdim + 1) reduced values
     double *h_red = nullptr;        // pinned host copy
-    double *d_small = nullptr;      // kdim + 1 coefficients on the device
+    // GMRES on the device: its state, arrays, and the flags the host reads
+    GmState *d_gm = nullptr;
+    double *gm_arr = nullptr;       // H, G, cs, sn, g, c, z
+    double *gm_z = nullptr;         // z within gm_arr
+    int *h_flags = nullptr;         // pinned: (done, conv, j, m) per iteration
+    hipEvent_t ev[2] = {nullptr, nullptr};
     std::vector<double> coefs;
     fsp_bdf_stats st;
 };
@@ -319,25 +447,19 @@ fsp_status wrms(BdfImpl &B, const double *x, const double *w, double *out)
     return FSP_OK;
 }
 
-fsp_status push_small(BdfImpl &B, const double *host, int count)
-{
-    std::memcpy(B.h_red, host, sizeof(double) * (size_t)count);
-    FSP_HIP(hipMemcpyAsync(B.d_small, B.h_red, sizeof(double) * (size_t)count,
-                           hipMemcpyHostToDevice, B.s));
-    FSP_HIP(hipStreamSynchronize(B.s));
-    return FSP_OK;
-}
-
 // Solves (alpha0 I - A) y = b, y starting at the predictor (already in y),
 // by restarted GMRES on the ewt-scaled system; converged when the weighted
-// RMS of the scaled residual is <= tol.
+// RMS of the scaled residual is <= tol.  A restart cycle runs on the device:
+// per iteration the operator product, one sweep for the dots and the new
+// Gram column, one sweep for the Gram-corrected update, and the rotations in
+// one workgroup.  The host enqueues iteration it + 1 before it waits for
+// iteration it's flags, so the device never idles on the host.
 fsp_status gmres(BdfImpl &B, double alpha0, double tol, double *y, bool *converged)
 {
     const int m = B.opt.krylov_dim;
     const int64_t n = B.n;
     const int G = vec_grid(n);
     const double sqn = std::sqrt((double)n);
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hcol(m + 1);
     *converged = false;
     for (int cycle = 0; cycle <= B.opt.max_restarts; ++cycle) {
         // r0 = D (b - alpha0 y + A y)
@@ -354,75 +476,45 @@ fsp_status gmres(BdfImpl &B, double alpha0, double tol, double *y, bool *converg
             return FSP_OK;
         }
         hipLaunchKernelGGL(k_scale_copy, dim3(G), dim3(kBlock), 0, B.s, r, 1.0 / beta, r, n);
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
-        int j = 0;
-        bool done = false;
-        for (; j < m && !done; ++j) {
-            double *vj = B.V + (int64_t)j * n, *w = B.V + (int64_t)(j + 1) * n;
-            hipLaunchKernelGGL(k_unscale, dim3(G), dim3(kBlock), 0, B.s, vj, B.ewt, B.diag, alpha0,
-                               B.tmp, n);
-            FSP_TRY(mat_apply(*B.A, B.tmp, B.u, 0, B.s));
-            B.st.n_rhs++;
-            hipLaunchKernelGGL(k_op_post, dim3(G), dim3(kBlock), 0, B.s, vj, B.ewt, B.diag, B.u,
-                               alpha0, w, n);
-            FSP_HIP(hipGetLastError());
-            // classical Gram-Schmidt, two passes; the coefficients stay on
-            // the device (pass p in dred[p (j+1) ..], |w|^2 after them) and
-            // reach the host in one copy for the Givens rotations
-            std::fill(hcol.begin(), hcol.end(), 0.0);
-            for (int pass = 0; pass < 2; ++pass) {
-                double *hp = B.dred + (size_t)pass * (j + 1);
-                hipLaunchKernelGGL(k_multi_dot, dim3(kRedGrid), dim3(kBlock), 0, B.s, B.V, n, j + 1,
-                                   w, n, B.partials);
-                hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(kBlock), 0, B.s, B.partials,
-                                   kRedGrid, hp);
-                hipLaunchKernelGGL(k_gs_update, dim3(kRedGrid), dim3(kBlock), 0, B.s, B.V, n, j + 1,
-                                   hp, w, n, B.partials);
-                FSP_HIP(hipGetLastError());
-            }
-            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, B.s, B.partials, kRedGrid,
-                               B.dred + 2 * (size_t)(j + 1));
-            FSP_HIP(hipGetLastError());
-            const int nred = 2 * (j + 1) + 1;
-            FSP_HIP(hipMemcpyAsync(B.h_red, B.dred, sizeof(double) * (size_t)nred,
-                                   hipMemcpyDeviceToHost, B.s));
-            FSP_HIP(hipStreamSynchronize(B.s));
-            for (int pass = 0; pass < 2; ++pass)
-                for (int i = 0; i <= j; ++i) hcol[i] += B.h_red[(size_t)pass * (j + 1) + i];
-            const double wn = std::sqrt(B.h_red[2 * (j + 1)]);
-            hcol[j + 1] = wn;
-            if (wn > 0.0)
-                hipLaunchKernelGGL(k_scale_copy, dim3(G), dim3(kBlock), 0, B.s, w, 1.0 / wn, w, n);
-            // Givens rotations
-            for (int i = 0; i < j; ++i) {
-                const double a = cs[i] * hcol[i] + sn[i] * hcol[i + 1];
-                hcol[i + 1] = -sn[i] * hcol[i] + cs[i] * hcol[i + 1];
-                hcol[i] = a;
-            }
-            const double rr = std::hypot(hcol[j], hcol[j + 1]);
-            cs[j] = rr > 0.0 ? hcol[j] / rr : 1.0;
-            sn[j] = rr > 0.0 ? hcol[j + 1] / rr : 0.0;
-            hcol[j] = rr;
-            hcol[j + 1] = 0.0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
-            for (int i = 0; i <= j; ++i) H[(size_t)i + (size_t)j * (m + 1)] = hcol[i];
-            B.st.n_lin_iters++;
-            if (std::fabs(g[j + 1]) / sqn <= tol || wn == 0.0) done = true;
-        }
-        // back substitution H(0:j,0:j) z = g(0:j)
-        std::vector<double> z(j);
-        for (int i = j - 1; i >= 0; --i) {
-            double a = g[i];
-            for (int c = i + 1; c < j; ++c) a -= H[(size_t)i + (size_t)c * (m + 1)] * z[c];
-            z[i] = a / H[(size_t)i + (size_t)i * (m + 1)];
-        }
-        FSP_TRY(push_small(B, z.data(), j));
-        hipLaunchKernelGGL(k_sol_update, dim3(G), dim3(kBlock), 0, B.s, B.V, n, j, B.d_small,
-                           B.ewt, B.diag, alpha0, y, n);
+        hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(kBlock), 0, B.s, B.d_gm, beta, tol, sqn);
         FSP_HIP(hipGetLastError());
-        if (done) {
+        int last = -1;
+        for (int j = 0; j < m; ++j) {
+            double *vj = B.V + (int64_t)j * n, *w = B.V + (int64_t)(j + 1) * n;
+            const int nk = j + 1;
+            hipLaunchKernelGGL(k_gm_unscale, dim3(G), dim3(kBlock), 0, B.s, vj, B.d_gm, B.ewt, B.diag,
+                               alpha0, B.tmp, n);
+            FSP_TRY(mat_apply(*B.A, B.tmp, B.u, 0, B.s));
+            hipLaunchKernelGGL(k_gm_post, dim3(G), dim3(kBlock), 0, B.s, vj, B.ewt, B.diag, B.u, alpha0,
+                               w, n, B.d_gm);
+            hipLaunchKernelGGL(k_gm_dots, dim3(kRedGrid), dim3(kBlock), 0, B.s, B.V, n, nk, w, n,
+                               B.partials, B.d_gm);
+            hipLaunchKernelGGL(k_reduce_partials, dim3(2 * nk), dim3(kBlock), 0, B.s, B.partials, kRedGrid,
+                               B.dred);
+            hipLaunchKernelGGL(k_gm_update, dim3(kRedGrid), dim3(kBlock), sizeof(double) * (size_t)nk, B.s,
+                               B.V, n, nk, B.dred, w, n, B.partials, B.d_gm);
+            hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(kBlock), 0, B.s, B.d_gm, j, B.partials, kRedGrid);
+            FSP_HIP(hipGetLastError());
+            FSP_HIP(hipMemcpyAsync(B.h_flags + 4 * j, B.d_gm, sizeof(int) * 4, hipMemcpyDeviceToHost, B.s));
+            FSP_HIP(hipEventRecord(B.ev[j & 1], B.s));
+            last = j;
+            if (j >= 1) {
+                FSP_HIP(hipEventSynchronize(B.ev[(j - 1) & 1]));
+                if (B.h_flags[4 * (j - 1)]) break;  // iteration j was launched past the end
+            }
+        }
+        FSP_HIP(hipStreamSynchronize(B.s));
+        const int *fl = B.h_flags + 4 * last;
+        const int nit = fl[2];
+        const bool conv = fl[1] != 0;
+        B.st.n_lin_iters += nit;
+        B.st.n_rhs += nit;
+        // y += Q^-1 D^-1 V z, z = H^-1 g over the cycle's iterations
+        hipLaunchKernelGGL(k_gm_backsub, dim3(1), dim3(kWave), 0, B.s, B.d_gm);
+        hipLaunchKernelGGL(k_sol_update, dim3(G), dim3(kBlock), 0, B.s, B.V, n, nit,
+                           (const double *)B.gm_z, B.ewt, B.diag, alpha0, y, n);
+        FSP_HIP(hipGetLastError());
+        if (conv) {
             *converged = true;
             return FSP_OK;
         }
@@ -433,15 +525,41 @@ fsp_status gmres(BdfImpl &B, double alpha0, double tol, double *y, bool *converg
 
 fsp_status bdf_alloc(BdfImpl &B)
 {
-    const size_t n = (size_t)B.n, kd = (size_t)B.opt.krylov_dim + 1;
+    const size_t n = (size_t)B.n, kd = (size_t)B.opt.krylov_dim + 1, m = (size_t)B.opt.krylov_dim;
     for (int j = 0; j < kHist; ++j) FSP_HIP(hipMalloc(&B.Y[j], sizeof(double) * n));
     double **vs[] = {&B.ewt, &B.b, &B.yp, &B.ynew, &B.u, &B.tmp, &B.diag};
     for (double **p : vs) FSP_HIP(hipMalloc(p, sizeof(double) * n));
     FSP_HIP(hipMalloc(&B.V, sizeof(double) * n * kd));
-    FSP_HIP(hipMalloc(&B.partials, sizeof(double) * 2 * kd * kRedGrid));
-    FSP_HIP(hipMalloc(&B.dred, sizeof(double) * (2 * kd + 1)));
-    FSP_HIP(hipMalloc(&B.d_small, sizeof(double) * kd));
+    FSP_HIP(hipMalloc(&B.partials, sizeof(double) * 4 * kd * kRedGrid));
+    FSP_HIP(hipMalloc(&B.dred, sizeof(double) * 2 * kd));
     FSP_HIP(hipHostMalloc((void **)&B.h_red, sizeof(double) * (2 * kd + 1), hipHostMallocDefault));
+    // GMRES state: H ((m+1) m), G ((m+1)^2), cs, sn (m), g, c (m+1), z (m)
+    const size_t na = kd * m + kd * kd + 2 * m + 2 * kd + m;
+    FSP_HIP(hipMalloc(&B.gm_arr, sizeof(double) * na));
+    FSP_HIP(fill_sync(B.gm_arr, 0, sizeof(double) * na, B.s));
+    FSP_HIP(hipMalloc(&B.d_gm, sizeof(GmState)));
+    GmState hs;
+    std::memset(&hs, 0, sizeof(hs));
+    hs.m = (int)m;
+    hs.inv_wn = 1.0;
+    double *q = B.gm_arr;
+    hs.H = q;
+    q += kd * m;
+    hs.G = q;
+    q += kd * kd;
+    hs.cs = q;
+    q += m;
+    hs.sn = q;
+    q += m;
+    hs.g = q;
+    q += kd;
+    hs.c = q;
+    q += kd;
+    hs.z = q;
+    B.gm_z = q;
+    FSP_HIP(copy_sync(B.d_gm, &hs, sizeof(GmState), hipMemcpyHostToDevice, B.s));
+    FSP_HIP(hipHostMalloc((void **)&B.h_flags, sizeof(int) * 4 * m, hipHostMallocDefault));
+    for (hipEvent_t &e : B.ev) FSP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return FSP_OK;
 }
 
@@ -449,10 +567,14 @@ void bdf_free(BdfImpl &B)
 {
     for (int j = 0; j < kHist; ++j)
         if (B.Y[j]) (void)hipFree(B.Y[j]);
-    double *vs[] = {B.ewt, B.b, B.yp, B.ynew, B.u, B.tmp, B.diag, B.V, B.partials, B.dred, B.d_small};
+    double *vs[] = {B.ewt, B.b, B.yp, B.ynew, B.u, B.tmp, B.diag, B.V, B.partials, B.dred, B.gm_arr};
     for (double *p : vs)
         if (p) (void)hipFree(p);
+    if (B.d_gm) (void)hipFree(B.d_gm);
     if (B.h_red) (void)hipHostFree(B.h_red);
+    if (B.h_flags) (void)hipHostFree(B.h_flags);
+    for (hipEvent_t e : B.ev)
+        if (e) (void)hipEventDestroy(e);
 }
 
 // One attempted step of order k and size h from the newest history point.

@@ -547,22 +547,49 @@ __global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2)
                 st->H[(a.istart + k) + (size_t)a.j * M2] = cw[k];
         h = cw - a.istart;
     }
+    // kUpdRows rows per thread at a time (rows i + u G kBlock): the loads
+    // of all of them are issued before their sums, so a group of rows costs
+    // one memory round trip per 8 basis vectors instead of one per row; each
+    // row's sum still runs over k ascending
+    constexpr int kUpdRows = 3;
     dd nrm = dd_zero();
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.nv; i += gridDim.x * kBlock) {
-        // k ascending as one sum; the basis loads of 8 terms are issued together
-        double acc = 0.0;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i0 = blockIdx.x * kBlock + threadIdx.x; i0 < a.nv; i0 += kUpdRows * stride) {
+        uint32_t r[kUpdRows];
+        double acc[kUpdRows], o[kUpdRows];
+#pragma unroll
+        for (int u = 0; u < kUpdRows; ++u) {
+            const uint32_t iu = i0 + (uint32_t)u * stride;
+            r[u] = iu < a.nv ? iu : i0;
+            o[u] = a.out[r[u]];
+            acc[u] = 0.0;
+        }
         int k = a.istart;
         for (; k + kDotChunk <= a.j + 1; k += kDotChunk) {
-            double v[kDotChunk];
+            double v[kUpdRows][kDotChunk];
 #pragma unroll
-            for (int q = 0; q < kDotChunk; ++q) v[q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, k + q, i);
+            for (int u = 0; u < kUpdRows; ++u)
 #pragma unroll
-            for (int q = 0; q < kDotChunk; ++q) acc += v[q] * h[k + q];
+                for (int q = 0; q < kDotChunk; ++q) v[u][q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, k + q, r[u]);
+#pragma unroll
+            for (int u = 0; u < kUpdRows; ++u)
+#pragma unroll
+                for (int q = 0; q < kDotChunk; ++q) acc[u] += v[u][q] * h[k + q];
         }
-        for (; k <= a.j; ++k) acc += basis<B>(a.V64, (const B *)a.Vb, a.nv, k, i) * h[k];
-        const double w = -1.0 * acc + a.out[i];
-        a.out[i] = w;
-        nrm = dd_add_prod(nrm, w, w);
+        for (; k <= a.j; ++k) {
+            double v[kUpdRows];
+#pragma unroll
+            for (int u = 0; u < kUpdRows; ++u) v[u] = basis<B>(a.V64, (const B *)a.Vb, a.nv, k, r[u]);
+#pragma unroll
+            for (int u = 0; u < kUpdRows; ++u) acc[u] += v[u] * h[k];
+        }
+#pragma unroll
+        for (int u = 0; u < kUpdRows; ++u) {
+            if (i0 + (uint32_t)u * stride >= a.nv) break;
+            const double w = -1.0 * acc[u] + o[u];
+            a.out[r[u]] = w;
+            nrm = dd_add_prod(nrm, w, w);
+        }
     }
     const dd t = block_sum_dd(nrm, red);
     if (threadIdx.x == 0) {

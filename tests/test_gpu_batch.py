@@ -1,7 +1,7 @@
 """Batched solves on the device: the lockstep path (one graph-captured
 batch step advances every running instance by one Krylov step of its own),
 the persistent work-queue kernel (each workgroup solves whole instances; the
-launch is one captured graph), the hybrid of the two (the default) and the
+launch is one captured graph), the hybrid of the two and the
 stream scheduler (each instance through the single-solve engine, up to four
 at a time).  Checks: every instance against the oracle's Krylov
 restatement with the same operator (HYB generators on the lockstep,
@@ -92,14 +92,15 @@ def test_persistent_and_hybrid_match_oracle(path, sink):
 TOGGLE_CGS_RATES = (7.838583757547972, 0.5659944527754909, 13.088930136958103, 6.072545924918103)
 
 
-def test_batch_mixed_vs_f64_and_replay():
+@pytest.mark.parametrize("path", ["lockstep", "persistent"])
+def test_batch_mixed_vs_f64_and_replay(path):
     nets, tf, ws = _small_batch(count=16, seed=9)
     # rates under which one Gram-Schmidt pass over an FP32 basis degenerates
     nets = nets + [M.toggle_switch((60, 60), rates=TOGGLE_CGS_RATES)]
     tf = np.append(tf, 20.0)
-    print(f"[case] batch mixed n={[n.n_states for n in nets][:6]}...", flush=True)
-    B64 = capi.Batch(nets, tf, precision="f64", time_limit_s=LIMIT_S)
-    B32 = capi.Batch(nets, tf, precision="mixed", time_limit_s=LIMIT_S)
+    print(f"[case] batch mixed {path} n={[n.n_states for n in nets][:6]}...", flush=True)
+    B64 = capi.Batch(nets, tf, precision="f64", path=path, time_limit_s=LIMIT_S)
+    B32 = capi.Batch(nets, tf, precision="mixed", path=path, time_limit_s=LIMIT_S)
     B64.solve()
     B32.solve()
     ref = [B64.solution(i) for i in range(len(nets))]
