@@ -255,7 +255,7 @@ typedef struct fsp_batch_options {
  * the instances of at most split_states entries and the lockstep steps the
  * others, at the same time on two streams.  STREAMS: each instance through
  * the single-solve engine, up to four at a time on their own streams.
- * AUTO: the autotuner's choice, else HYBRID. */
+ * AUTO: the autotuner's choice, else LOCKSTEP. */
 #define FSP_BATCH_AUTO 0
 #define FSP_BATCH_LOCKSTEP 1
 #define FSP_BATCH_STREAMS 2
