@@ -92,14 +92,14 @@ def solve_outputs(E, v, p0_dev, n, t_out, out_buf):
     capi.check(capi.lib().fsp_stream_sync(None))
 
 
-def bench_solve(net, precision, steps, warmup, t_out, fmt="hyb"):
+def bench_solve(net, precision, steps, warmup, t_out, fmt="hyb", basis=None):
     A = capi.Mat(net, fmt, precision)
     n = A.n_rows
     p0 = M.delta_p0(net)
     p0_dev = capi.DeviceBuffer.from_host(p0)
     v = capi.DeviceBuffer.zeros(n)
     out = capi.DeviceBuffer.zeros(n * len(t_out))
-    E = capi.Expv(A, n, t_out[0], m=30, tol=1e-8)
+    E = capi.Expv(A, n, t_out[0], m=30, tol=1e-8, basis=basis)
     for _ in range(warmup):
         solve_outputs(E, v, p0_dev, n, t_out, out)
     t0 = time.perf_counter()
@@ -239,9 +239,12 @@ def bench_chain_mixed():
     n = net.n_states
     p0 = M.delta_p0(net)
     out, info = {}, {}
-    for prec in ("f64", "mixed"):
+    # key, generator precision, Krylov basis: FP64; FP32 values and basis;
+    # FP32 values with an FP64 basis
+    for key, prec, basis in (("f64", "f64", None), ("mixed", "mixed", None),
+                             ("mixed_values", "mixed", "f64")):
         A = capi.Mat(net, "hyb", prec)
-        E = capi.Expv(A, n, 20.0, m=30, tol=1e-8)
+        E = capi.Expv(A, n, 20.0, m=30, tol=1e-8, basis=basis)
         v = capi.DeviceBuffer.from_host(p0)
         capi.check(capi.lib().fsp_stream_sync(None))
         t0 = time.perf_counter()
@@ -249,25 +252,34 @@ def bench_chain_mixed():
         capi.check(capi.lib().fsp_stream_sync(None))
         dt = time.perf_counter() - t0
         st = E.stats()
-        out[prec] = v.to_host()
-        info[prec] = {"solve_s": round(dt, 3), "krylov_steps": int(st.i_step),
-                      "mass": float(out[prec].sum())}
-        print(f"[bench] chain t=20 {prec}: {dt:.2f} s, {st.i_step} Krylov steps", flush=True)
+        out[key] = v.to_host()
+        info[key] = {"solve_s": round(dt, 3), "krylov_steps": int(st.i_step),
+                     "mass": float(out[key].sum())}
+        print(f"[bench] chain t=20 {key}: {dt:.2f} s, {st.i_step} Krylov steps", flush=True)
         v.free()
         E.close()
         A.close()
     dims = [int(b) + 1 for b in net.bounds]
     cube = {k: v.reshape(dims[::-1]) for k, v in out.items()}
-    marg = []
-    for sp in range(len(dims)):
-        ax = tuple(a for a in range(len(dims)) if a != len(dims) - 1 - sp)
-        marg.append(float(np.abs(cube["mixed"].sum(axis=ax) - cube["f64"].sum(axis=ax)).sum()))
+
+    def marginals(key):
+        res = []
+        for sp in range(len(dims)):
+            ax = tuple(a for a in range(len(dims)) if a != len(dims) - 1 - sp)
+            res.append(float(np.abs(cube[key].sum(axis=ax) - cube["f64"].sum(axis=ax)).sum()))
+        return res
+
     return {"workload": f"five_species_chain {'x'.join(str(d) for d in dims)} ({n:,} states), "
                         "delta(0) to t=20, m=30 tol=1e-8, HYB generator",
             "f64": info["f64"], "mixed": info["mixed"],
             "speedup_mixed_vs_f64": round(info["f64"]["solve_s"] / info["mixed"]["solve_s"], 3),
             "l1_p_mixed_vs_f64": float(np.abs(out["mixed"] - out["f64"]).sum()),
-            "l1_marginals_mixed_vs_f64": marg}
+            "l1_marginals_mixed_vs_f64": marginals("mixed"),
+            "mixed_values_f64_basis": {
+                **info["mixed_values"],
+                "speedup_vs_f64": round(info["f64"]["solve_s"] / info["mixed_values"]["solve_s"], 3),
+                "l1_p_vs_f64": float(np.abs(out["mixed_values"] - out["f64"]).sum()),
+                "l1_marginals_vs_f64": marginals("mixed_values")}}
 
 
 def bench_bdf(net):
@@ -625,6 +637,10 @@ def main():
     dtm, stm, mixed = bench_solve(net, "mixed", k_side, 1, t_out)
     l1_mixed = float(np.abs(mixed - ref).sum(axis=1).max())
     print(f"[bench] mixed {k_side / dtm:.3f} solves/s, worst L1 {l1_mixed:.2e}", flush=True)
+    print("[bench] FP32 values, FP64 basis", flush=True)
+    dtv, stv, mv = bench_solve(net, "mixed", k_side, 1, t_out, basis="f64")
+    l1_mv = float(np.abs(mv - ref).sum(axis=1).max())
+    print(f"[bench] FP32 values / FP64 basis {k_side / dtv:.3f} solves/s, worst L1 {l1_mv:.2e}", flush=True)
 
     ev = HipEvents()
     big = M.five_species_chain()
@@ -695,6 +711,10 @@ def main():
                   "speedup_vs_f64": round((k_side / dtm) / solves_s, 3),
                   "worst_l1_vs_f64": l1_mixed,
                   "krylov_steps_per_solve": int(stm.i_step)},
+        "mixed_values_f64_basis": {"solves_per_s": round(k_side / dtv, 4),
+                                   "speedup_vs_f64": round((k_side / dtv) / solves_s, 3),
+                                   "worst_l1_vs_f64": l1_mv,
+                                   "krylov_steps_per_solve": int(stv.i_step)},
         **extras,
     }
     print(json.dumps(line), flush=True)

@@ -138,6 +138,7 @@ _SIGS = {
     "fsp_expv_get_tau_history": (C.c_int, [_vp, _vp, _i64, _vp]),
     "fsp_expv_set_iop": (C.c_int, [_vp, _i, _i]),
     "fsp_expv_set_profiling": (C.c_int, [_vp, _i]),
+    "fsp_expv_set_basis_precision": (C.c_int, [_vp, _i]),
     "fsp_expv_get_profile": (C.c_int, [_vp, _vp]),
     "fsp_batch_create": (C.c_int, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "fsp_batch_destroy": (C.c_int, [_vp]),
@@ -537,7 +538,7 @@ class Expv:
     """fsp_expv handle: KryExpvFSP on the device (src/KryExpvFSP.h:75-102)."""
 
     def __init__(self, A: Mat | None, n, t_final, m=30, tol=1e-8, iop=False, q_iop=2,
-                 anorm=1.0, matvec=None, stream=None):
+                 anorm=1.0, matvec=None, stream=None, basis=None):
         self._cb_keep = None
         if matvec is not None:
             def _cb(ctx, x, y):
@@ -552,6 +553,9 @@ class Expv:
                                     stream, C.byref(h)))
         self.h = h
         self.A = A
+        if basis is not None:
+            # "f64": FP64 Krylov basis over a mixed (FP32-valued) generator
+            check(lib().fsp_expv_set_basis_precision(self.h, PRECISIONS[basis]))
 
     def step(self, v):
         check(lib().fsp_expv_step(self.h, ptr(v)))

@@ -1347,6 +1347,39 @@ extern "C" fsp_status fsp_expv_get_tau_history(fsp_expv e, double *h_taus, int64
     return FSP_OK;
 }
 
+extern "C" fsp_status fsp_expv_set_basis_precision(fsp_expv e, int precision)
+{
+    using namespace fsp;
+    if (!e) return fail(FSP_ERR_INVALID, "NULL argument");
+    if (precision != FSP_F64 && precision != FSP_MIXED) return fail(FSP_ERR_INVALID, "unknown precision");
+    ExpvImpl &E = e->impl;
+    if (!E.A) return fail(FSP_ERR_UNSUPPORTED, "the basis precision needs a matrix operator");
+    const bool mixed = precision == FSP_MIXED;
+    if (mixed && E.A->prec != FSP_MIXED)
+        return fail(FSP_ERR_UNSUPPORTED, "an FP32 basis needs a FSP_MIXED matrix");
+    if (mixed == E.mixed) return FSP_OK;
+    FSP_HIP(hipStreamSynchronize(E.s));
+    const double t_final = E.h_st->t_final;
+    expv_free(E);
+    E.d_st = nullptr;
+    E.h_st = nullptr;
+    E.V64 = nullptr;
+    E.V32 = nullptr;
+    E.W[0] = E.W[1] = nullptr;
+    E.partials = nullptr;
+    E.tau_log = nullptr;
+    E.dcoef = nullptr;
+    E.graph = nullptr;
+    E.gexec = nullptr;
+    E.graph_v = nullptr;
+    E.cap = nullptr;
+    E.pev.clear();
+    E.mixed = mixed;
+    FSP_TRY(expv_alloc(E));
+    E.h_st->t_final = t_final;
+    return push_state(E);
+}
+
 extern "C" fsp_status fsp_expv_set_profiling(fsp_expv e, int enable)
 {
     using namespace fsp;

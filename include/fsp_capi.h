@@ -222,6 +222,12 @@ typedef struct fsp_expv_profile {
     double bytes[FSP_PROF_CLASSES];
 } fsp_expv_profile;
 fsp_status fsp_expv_set_profiling(fsp_expv e, int enable);
+/* Krylov basis precision for a FSP_MIXED matrix: FSP_MIXED (the default: an
+ * FP32 basis, Gram-corrected) or FSP_F64 (FP32 generator values with an FP64
+ * basis: the same Arnoldi process as FSP_F64 on the FP32-rounded generator).
+ * Call before the first step (it restarts the handle).  No reference
+ * counterpart. */
+fsp_status fsp_expv_set_basis_precision(fsp_expv e, int precision);
 fsp_status fsp_expv_get_profile(fsp_expv e, fsp_expv_profile *profile);
 
 /* ---- batched solves (no reference counterpart: the reference solves one

@@ -338,6 +338,26 @@ def test_mixed_precision_l1(fmt):
         assert abs(got[k].sum() - 1.0) <= 1e-6
 
 
+@pytest.mark.parametrize("fmt", ["csr", "hyb", "kron"])
+def test_mixed_values_with_f64_basis(fmt):
+    """FP32 generator values with an FP64 Krylov basis: the FP64 Arnoldi
+    process on the FP32-rounded generator (itself a generator: its diagonal
+    is the negated sum of the rounded rates), so the step count stays at
+    FP64's and the solution within 1e-6 L1 of FP64."""
+    net = M.gene_expression((2, 60, 60))
+    p0 = M.delta_p0(net, with_sink=True)
+    t_out = [5.0, 20.0]
+    ref, st64 = _gpu_expv(net, fmt, p0, t_out, prec="f64")
+    got, stv = _gpu_expv(net, fmt, p0, t_out, prec="mixed", basis="f64")
+    for k in range(len(t_out)):
+        err = l1(got[k], ref[k])
+        print(f"[mixed values] {fmt} t={t_out[k]} L1 vs FP64 = {err:.3e} steps {stv.i_step} vs {st64.i_step}",
+              flush=True)
+        assert err <= 1e-6, (fmt, k, err)
+        assert abs(got[k].sum() - 1.0) <= 1e-6
+    assert abs(stv.i_step - st64.i_step) <= max(2, st64.i_step // 20)
+
+
 # Rates under which one classical Gram-Schmidt pass over an FP32 basis loses
 # orthogonality completely within 30 Arnoldi steps from a point mass (the
 # Hessenberg exponential then overflows and every step size is rejected).
