@@ -516,17 +516,19 @@ def solve_only():
     print(f"[solve] {dt:.2f} s, {st.i_step} Krylov steps", flush=True)
 
 
-def batch_only(count, precision):
-    """One lockstep solve of the uneven batch (for kernel-trace runs):
+def batch_only(count, precision, path="lockstep", n_max=262144):
+    """One solve of the uneven batch by one path (for kernel-trace runs):
     prints the solve time, the Krylov step counts and the state-steps."""
-    nets, tf = M.batch_instances(count, seed=1)
-    B = capi.Batch(nets, tf, precision=precision, path="lockstep")
+    nets, tf = M.batch_instances(count, seed=1, n_max=n_max)
+    B = capi.Batch(nets, tf, precision=precision, path=path, time_limit_s=120.0)
     t0 = time.perf_counter()
     B.solve()
+    capi.check(capi.lib().fsp_stream_sync(None))
     dt = time.perf_counter() - t0
     steps = [B.stats(i)[0] for i in range(count)]
     ns = [n.n_states for n in nets]
-    print(json.dumps({"batch": count, "precision": precision, "solve_s": round(dt, 3),
+    print(json.dumps({"batch": count, "precision": precision, "path": path, "n_max": n_max,
+                      "solve_s": round(dt, 3),
                       "krylov_steps": int(sum(steps)), "batch_steps": int(max(steps)),
                       "state_steps": int(sum(s * n for s, n in zip(steps, ns))),
                       "states": int(sum(ns))}), flush=True)
@@ -553,6 +555,9 @@ def main():
     ap.add_argument("--batch-only", type=int, default=0,
                     help="only one lockstep solve of an uneven batch of this many instances")
     ap.add_argument("--batch-precision", default="f64", choices=["f64", "mixed"])
+    ap.add_argument("--batch-path", default="lockstep", choices=["lockstep", "persistent", "hybrid"])
+    ap.add_argument("--batch-nmax", type=int, default=262144,
+                    help="largest instance of the --batch-only batch (states)")
     ap.add_argument("--grad-points", type=int, default=4,
                     help="parameter points in the batched-gradient section")
     ap.add_argument("--grad-only", action="store_true",
@@ -575,7 +580,7 @@ def main():
         solve_only()
         return
     if args.batch_only:
-        batch_only(args.batch_only, args.batch_precision)
+        batch_only(args.batch_only, args.batch_precision, args.batch_path, args.batch_nmax)
         return
     if args.grad_only:
         grad_only(args.grad_points)
