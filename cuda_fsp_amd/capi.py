@@ -64,7 +64,7 @@ class FspBatchOptions(C.Structure):
                 ("precision", C.c_int), ("iop", C.c_int), ("q_iop", C.c_int),
                 ("lockstep_bpc", C.c_int), ("wg_per_cu", C.c_int), ("max_reject", C.c_int),
                 ("max_nstep", C.c_int64), ("path", C.c_int), ("split_states", C.c_int64),
-                ("time_limit_s", C.c_double)]
+                ("time_limit_s", C.c_double), ("lockstep_rows", C.c_int)]
 
 
 BATCH_PATHS = {"auto": 0, "lockstep": 1, "streams": 2, "persistent": 3, "hybrid": 4}
@@ -147,7 +147,7 @@ _SIGS = {
     "fsp_batch_get_solution": (C.c_int, [_vp, _i, _vp, _i64]),
     "fsp_batch_get_stats": (C.c_int, [_vp, _i, _vp, _vp, _vp]),
     "fsp_batch_get_config": (C.c_int, [_vp, _vp, _vp]),
-    "fsp_batch_get_tuning": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "fsp_batch_get_tuning": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "fsp_batch_autotune": (C.c_int, [_vp, C.c_char_p, _i, _vp]),
     "fsp_loglik_indices": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
     "fsp_loglik_indices_noalloc": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
@@ -627,7 +627,8 @@ class Batch:
 
     def __init__(self, nets, t_final, with_sink=None, m=30, tol=1e-8, anorm=1.0,
                  precision="f64", iop=False, q_iop=2, lockstep_bpc=0, wg_per_cu=0,
-                 max_reject=0, max_nstep=0, path="auto", split_states=0, time_limit_s=0.0):
+                 max_reject=0, max_nstep=0, path="auto", split_states=0, time_limit_s=0.0,
+                 lockstep_rows=0):
         self.nets = list(nets)
         count = len(self.nets)
         arr = (FspModel * count)()
@@ -641,7 +642,8 @@ class Batch:
         self._keep = [coefs, tf, ws]
         opt = FspBatchOptions(m, tol, anorm, PRECISIONS[precision], int(iop), int(q_iop),
                               int(lockstep_bpc), int(wg_per_cu), int(max_reject), int(max_nstep),
-                              BATCH_PATHS[path], int(split_states), float(time_limit_s))
+                              BATCH_PATHS[path], int(split_states), float(time_limit_s),
+                              int(lockstep_rows))
         h = C.c_void_p()
         check(lib().fsp_batch_create(C.cast(arr, C.c_void_p), coefs.ctypes.data, tf.ctypes.data,
                                      None if ws is None else ws.ctypes.data, count, C.byref(opt),
@@ -674,11 +676,13 @@ class Batch:
         return b.value, w.value
 
     def tuning(self):
-        """(path name, lockstep workgroups per CU, persistent workgroups per
-        CU, split in states) of the next solve."""
-        p, b, w, sp = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int64(0)
-        check(lib().fsp_batch_get_tuning(self.h, C.byref(p), C.byref(b), C.byref(w), C.byref(sp)))
-        return BATCH_PATH_NAMES[p.value], b.value, w.value, sp.value
+        """(path name, lockstep workgroups per CU, lockstep tile rows per
+        thread, persistent workgroups per CU, split in states) of the next
+        solve."""
+        p, b, r, w, sp = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0), C.c_int64(0)
+        check(lib().fsp_batch_get_tuning(self.h, C.byref(p), C.byref(b), C.byref(r), C.byref(w),
+                                         C.byref(sp)))
+        return BATCH_PATH_NAMES[p.value], b.value, r.value, w.value, sp.value
 
     def autotune(self, cache_path=None, force=False) -> float:
         ms = C.c_double(0.0)

@@ -112,6 +112,11 @@ extern "C" fsp_status fsp_batch_create(const fsp_model *models, const double *co
         B.mixed = opt->precision == FSP_MIXED;
         if (opt->wg_per_cu > 0) B.pq_wg = std::min(opt->wg_per_cu, 8);
         if (opt->lockstep_bpc > 0) B.ls_bpc = opt->lockstep_bpc;
+        if (opt->lockstep_rows > 0) {
+            if (opt->lockstep_rows != 2 && opt->lockstep_rows != 4 && opt->lockstep_rows != 8)
+                return bail(fail(FSP_ERR_INVALID, "lockstep_rows must be 2, 4 or 8"));
+            B.ls_rpt = opt->lockstep_rows;
+        }
         if (opt->split_states != 0) B.pq_split = opt->split_states;
         if (opt->time_limit_s > 0) B.pq_budget_s = opt->time_limit_s;
         if (opt->max_reject > 0) B.max_reject = opt->max_reject;
@@ -314,14 +319,15 @@ extern "C" fsp_status fsp_batch_get_config(fsp_batch b, int *block, int *wg_per_
     return FSP_OK;
 }
 
-extern "C" fsp_status fsp_batch_get_tuning(fsp_batch b, int *path, int *lockstep_bpc, int *persistent_wg,
-                                           int64_t *split_states)
+extern "C" fsp_status fsp_batch_get_tuning(fsp_batch b, int *path, int *lockstep_bpc, int *lockstep_rows,
+                                           int *persistent_wg, int64_t *split_states)
 {
     using namespace fsp;
     if (!b) return fail(FSP_ERR_INVALID, "NULL batch");
     const BatchImpl &B = b->impl;
     if (path) *path = batch_path(B);
     if (lockstep_bpc) *lockstep_bpc = B.ls_bpc;
+    if (lockstep_rows) *lockstep_rows = B.ls_rpt;
     if (persistent_wg) *persistent_wg = B.pq_wg;
     if (split_states) *split_states = B.pq_split;
     return FSP_OK;
@@ -329,10 +335,12 @@ extern "C" fsp_status fsp_batch_get_tuning(fsp_batch b, int *path, int *lockstep
 
 // Autotuner: times candidate configurations on the batch itself (one whole
 // solve each, after one untimed solve of the path) and keeps the fastest:
-//   LOCKSTEP    row-kernel workgroups per CU: one per tile, 4 or 16;
+//   LOCKSTEP    row-kernel workgroups per CU (one per tile, 4 or 16) x
+//               tile rows per thread (2, 4 or 8: tiles of 512-2,048 rows);
 //   PERSISTENT  persistent workgroups per CU: 1, 2 or 4;
-//   HYBRID      the lockstep grid as above (all instances on the lockstep
-//               path), then the split at 4,096 / 16,384 / 65,536 states;
+//   HYBRID      the lockstep configurations as above (all instances on the
+//               lockstep path), then the split at 4,096 / 16,384 / 65,536
+//               states;
 //   AUTO        every candidate above, and the path with them;
 //   STREAMS     1, 2 or 4 concurrent streams.
 // The choice is cached in a JSON file keyed by device, instance count,
@@ -340,14 +348,15 @@ extern "C" fsp_status fsp_batch_get_tuning(fsp_batch b, int *path, int *lockstep
 namespace {
 
 struct Tuned {
-    int path = FSP_BATCH_LOCKSTEP, bpc = 0, wg = 2, workers = 4;
+    int path = FSP_BATCH_LOCKSTEP, bpc = 0, rpt = 4, wg = 2, workers = 4;
     long long split = 16384;
     double ms = 1e300;
 };
 
-fsp_status time_solve(fsp::BatchImpl &B, int path, int bpc, int wg, long long split, double *ms)
+fsp_status time_solve(fsp::BatchImpl &B, int path, int bpc, int rpt, int wg, long long split, double *ms)
 {
     using namespace fsp;
+    B.ls_rpt = rpt;
     auto run = [&]() -> fsp_status {
         if (path == FSP_BATCH_LOCKSTEP) return batch_solve_lockstep(B, bpc);
         if (path == FSP_BATCH_HYBRID) return batch_solve_hybrid(B, bpc, wg, split);
@@ -363,12 +372,13 @@ fsp_status time_solve(fsp::BatchImpl &B, int path, int bpc, int wg, long long sp
 fsp_status tune_family(fsp::BatchImpl &B, int fam, Tuned &best)
 {
     using namespace fsp;
-    auto consider = [&](int path, int bpc, int wg, long long split) -> fsp_status {
+    auto consider = [&](int path, int bpc, int rpt, int wg, long long split) -> fsp_status {
         double ms = 0;
-        FSP_TRY(time_solve(B, path, bpc, wg, split, &ms));
+        FSP_TRY(time_solve(B, path, bpc, rpt, wg, split, &ms));
         if (ms < best.ms) {
             best.path = path;
             best.bpc = bpc;
+            best.rpt = rpt;
             best.wg = wg;
             best.split = split;
             best.ms = ms;
@@ -376,17 +386,21 @@ fsp_status tune_family(fsp::BatchImpl &B, int fam, Tuned &best)
         return FSP_OK;
     };
     const int bpcs[3] = {0, 4, 16};
+    const int rpts[3] = {2, 4, 8};
     const int wgs[3] = {1, 2, 4};
     const long long splits[3] = {4096, 16384, 65536};
+    const int rpt0 = B.ls_rpt;
     if (fam == FSP_BATCH_LOCKSTEP || fam == FSP_BATCH_HYBRID || fam == FSP_BATCH_AUTO)
-        for (int bpc : bpcs) FSP_TRY(consider(FSP_BATCH_LOCKSTEP, bpc, B.pq_wg, 0));
+        for (int rpt : rpts)
+            for (int bpc : bpcs) FSP_TRY(consider(FSP_BATCH_LOCKSTEP, bpc, rpt, B.pq_wg, 0));
     if (fam == FSP_BATCH_PERSISTENT || fam == FSP_BATCH_AUTO)
-        for (int wg : wgs) FSP_TRY(consider(FSP_BATCH_PERSISTENT, B.ls_bpc, wg, -1));
+        for (int wg : wgs) FSP_TRY(consider(FSP_BATCH_PERSISTENT, B.ls_bpc, rpt0, wg, -1));
     if (fam == FSP_BATCH_HYBRID || fam == FSP_BATCH_AUTO) {
         const Tuned ls = best;
         const int bpc = ls.path == FSP_BATCH_LOCKSTEP ? ls.bpc : B.ls_bpc;
+        const int rpt = ls.path == FSP_BATCH_LOCKSTEP ? ls.rpt : rpt0;
         const int wg = ls.path == FSP_BATCH_PERSISTENT ? ls.wg : B.pq_wg;
-        for (long long sp : splits) FSP_TRY(consider(FSP_BATCH_HYBRID, bpc, wg, sp));
+        for (long long sp : splits) FSP_TRY(consider(FSP_BATCH_HYBRID, bpc, rpt, wg, sp));
     }
     return FSP_OK;
 }
@@ -394,6 +408,7 @@ fsp_status tune_family(fsp::BatchImpl &B, int fam, Tuned &best)
 void apply_tuned(fsp::BatchImpl &B, const Tuned &t)
 {
     B.ls_bpc = t.bpc;
+    B.ls_rpt = t.rpt;
     B.pq_wg = t.wg;
     B.pq_split = t.split;
     B.workers = t.workers;
@@ -427,9 +442,9 @@ extern "C" fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, in
         if (pos != std::string::npos && !force) {
             Tuned t;
             if (std::sscanf(cache.c_str() + pos + k.size(),
-                            "\"path\": %d, \"bpc\": %d, \"wg\": %d, \"split\": %lld, \"workers\": %d, "
-                            "\"ms\": %lf",
-                            &t.path, &t.bpc, &t.wg, &t.split, &t.workers, &t.ms) == 6) {
+                            "\"path\": %d, \"bpc\": %d, \"rpt\": %d, \"wg\": %d, \"split\": %lld, "
+                            "\"workers\": %d, \"ms\": %lf",
+                            &t.path, &t.bpc, &t.rpt, &t.wg, &t.split, &t.workers, &t.ms) == 7) {
                 apply_tuned(B, t);
                 if (best_ms) *best_ms = t.ms;
                 return FSP_OK;
@@ -438,6 +453,7 @@ extern "C" fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, in
     }
     Tuned best;
     best.bpc = B.ls_bpc;
+    best.rpt = B.ls_rpt;
     best.wg = B.pq_wg;
     best.split = B.pq_split;
     best.workers = B.workers;
@@ -452,7 +468,7 @@ extern "C" fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, in
     if (cache_path) {
         std::ostringstream ent;
         ent << "\"" << key.str() << "\": {\"path\": " << best.path << ", \"bpc\": " << best.bpc
-            << ", \"wg\": " << best.wg << ", \"split\": " << best.split << ", \"workers\": "
+            << ", \"rpt\": " << best.rpt << ", \"wg\": " << best.wg << ", \"split\": " << best.split << ", \"workers\": "
             << best.workers << ", \"ms\": " << best.ms << "}";
         std::string body;
         const std::string k = "\"" + key.str() + "\": {";

@@ -53,7 +53,7 @@ struct LsArgs {
     const int *tile_r0;     // first row (within its instance) of tile t
     int *atiles;            // tiles of the running instances
     int *n_atiles;          // their count; [1]: running instances
-    double *partials;       // 2 max(2m + 4, kLsRpt) T doubles
+    double *partials;       // 2 max(2m + 4, RPT) T doubles
     const int *owner;       // per instance: 1 when the persistent kernel solves it
     int T, count, m, M2;
     int j, istart, mode, nk; // Arnoldi iteration, first vector orthogonalized against,
@@ -67,11 +67,15 @@ namespace {
 constexpr int kLsSlots = 2 * kMaxKrylov + 4;
 constexpr int kLsRedLen = 2 * kLsSlots * (kBlock / kWave);
 constexpr int kLsChunk = 4;  // dots per pass over a tile's rows
-// Tiles of kLsRows rows: each of the kBlock threads owns rows r0 + tid +
-// u kBlock (u < kLsRpt), so a tile's set-up and its dot reductions are
-// shared by kLsRpt rows per thread.
-constexpr int kLsRpt = 4;
-constexpr int kLsRows = kBlock * kLsRpt;
+// Tiles of RPT kBlock rows (RPT = 2, 4 or 8, the autotuner's tile height):
+// each of the kBlock threads owns rows r0 + tid + u kBlock (u < RPT), so a
+// tile's set-up and its dot reductions are shared by RPT rows per thread.
+#define FSP_LS_RPT_SWITCH(rpt, CALL)                    \
+    switch (rpt) {                                      \
+    case 2: { constexpr int RPT_ = 2; CALL; } break;    \
+    case 8: { constexpr int RPT_ = 8; CALL; } break;    \
+    default: { constexpr int RPT_ = 4; CALL; } break;   \
+    }
 
 __device__ inline double *ls_in(const LsInst &I, int j) { return j == 0 ? I.p : ((j & 1) ? I.W1 : I.W0); }
 __device__ inline double *ls_out(const LsInst &I, int j) { return ((j + 1) & 1) ? I.W1 : I.W0; }
@@ -174,6 +178,7 @@ __global__ void __launch_bounds__(1024) k_ls_compact(LsArgs a)
 }
 
 // |p|^2 per running tile (slot 0).
+template <int RPT>
 __global__ void __launch_bounds__(kBlock) k_ls_norm_partials(LsArgs a)
 {
     __shared__ double red[2 * kBlock / kWave];
@@ -183,7 +188,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_norm_partials(LsArgs a)
         const LsInst &I = a.inst[a.tile_inst[t]];
         dd s = dd_zero();
 #pragma unroll
-        for (int u = 0; u < kLsRpt; ++u) {
+        for (int u = 0; u < RPT; ++u) {
             const int r = a.tile_r0[t] + (int)threadIdx.x + u * kBlock;
             if (r < I.nv) s = dd_add_prod(s, I.p[r], I.p[r]);
         }
@@ -228,18 +233,20 @@ __device__ inline double ls_row(const LsInst &I, uint32_t r, const KryX<B> &X, d
     return hyb_row<B>(ls_hyb<B>(I, with_sink), r, X, sink);
 }
 
-// Row sums of the thread's kLsRpt rows of a tile (r0 + u kBlock) for a
+// Row sums of the thread's RPT rows of a tile (r0 + u kBlock) for a
 // compile-time slot count, RG rows at a time (the loads of a group issued
 // before their sums; 2 with an FP64 basis, 4 with FP32, sized to keep the
 // kernel's occupancy); rows past the instance read row 0 and are
 // discarded.  No sink terms (the caller adds them in row order).
-template <int NU, typename B>
-__device__ inline void ls_rows(const LsInst &I, int r0, const KryX<B> &X, double (&y)[kLsRpt])
+template <int NU, typename B, int RPT>
+__device__ inline void ls_rows(const LsInst &I, int r0, const KryX<B> &X, double (&y)[RPT])
 {
-    constexpr int RG = std::is_same<B, double>::value ? 2 : 4;
+    constexpr int RG0 = std::is_same<B, double>::value ? 2 : 4;
+    constexpr int RG = RG0 < RPT ? RG0 : RPT;  // a group never runs past y[RPT]
+    static_assert(RPT % RG == 0, "rows per thread must be a multiple of the group");
     const HybF<B> A = ls_hyb<B>(I, false);
 #pragma unroll 1
-    for (int u0 = 0; u0 < kLsRpt; u0 += RG) {
+    for (int u0 = 0; u0 < RPT; u0 += RG) {
 #pragma unroll
         for (int u = u0; u < u0 + RG; ++u) {
             const uint32_t r = (uint32_t)(r0 + u * kBlock);
@@ -249,7 +256,7 @@ __device__ inline void ls_rows(const LsInst &I, int r0, const KryX<B> &X, double
     }
 }
 
-template <typename B>
+template <typename B, int RPT>
 __global__ void __launch_bounds__(kBlock) k_ls_arn(LsArgs a)
 {
     __shared__ double red[kLsRedLen];
@@ -267,14 +274,14 @@ __global__ void __launch_bounds__(kBlock) k_ls_arn(LsArgs a)
         B *Vb = ls_vb<B>(I);
         const int nk = a.nk;
         dd sink = dd_zero(), nrm = dd_zero();
-        double y[kLsRpt];
+        double y[RPT];
         switch (I.U) {  // the instance's slot count, uniform over the tile
-        case 4: ls_rows<4, B>(I, r0, X, y); break;
-        case 5: ls_rows<5, B>(I, r0, X, y); break;
-        case 6: ls_rows<6, B>(I, r0, X, y); break;
+        case 4: ls_rows<4, B, RPT>(I, r0, X, y); break;
+        case 5: ls_rows<5, B, RPT>(I, r0, X, y); break;
+        case 6: ls_rows<6, B, RPT>(I, r0, X, y); break;
         default:
 #pragma unroll
-            for (int u = 0; u < kLsRpt; ++u) {
+            for (int u = 0; u < RPT; ++u) {
                 const uint32_t r = (uint32_t)(r0 + u * kBlock);
                 dd unused = dd_zero();
                 y[u] = r < (uint32_t)I.n ? ls_row<B>(I, r, X, unused, false) : 0.0;
@@ -282,7 +289,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_arn(LsArgs a)
         }
         // stores and sink terms in row order
 #pragma unroll
-        for (int u = 0; u < kLsRpt; ++u) {
+        for (int u = 0; u < RPT; ++u) {
             const uint32_t r = (uint32_t)(r0 + u * kBlock);
             if (r < (uint32_t)I.n) {
                 if (I.with_sink) sink = dd_add_prod(sink, I.hsink[r], X(r));
@@ -302,7 +309,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_arn(LsArgs a)
 #pragma unroll
                 for (int v = 0; v < kLsChunk; ++v) acc[v] = dd_zero();
 #pragma unroll
-                for (int u = 0; u < kLsRpt; ++u) {
+                for (int u = 0; u < RPT; ++u) {
                     const uint32_t r = (uint32_t)(r0 + u * kBlock);
                     if (r >= (uint32_t)I.n) continue;
                     double vk[kLsChunk];
@@ -327,7 +334,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_arn(LsArgs a)
 #pragma unroll
                 for (int v = 0; v < CH; ++v) acc[v] = accg[v] = 0.0;
 #pragma unroll
-                for (int u = 0; u < kLsRpt; ++u) {
+                for (int u = 0; u < RPT; ++u) {
                     const uint32_t r = (uint32_t)(r0 + u * kBlock);
                     if (r >= (uint32_t)I.n) continue;
                     const double xj = basis<B>(I.V64, Vb, (size_t)I.nv, a.j, r);
@@ -423,17 +430,17 @@ __global__ void __launch_bounds__(kBlock) k_ls_gram_coef(LsArgs a)
 }
 
 // w_{j+1} -= V[istart..j] H(., j) (all nv entries, the sink element
-// included), one row per thread: kLsRpt workgroups per running tile
-// (quarter u of tile t in workgroup slot kLsRpt t + u), |w|^2 partial of
-// each quarter at 2 (kLsRpt t + u).
-template <typename B>
+// included), one row per thread: RPT workgroups per running tile
+// (part u of tile t in workgroup slot RPT t + u), |w|^2 partial of each
+// part at 2 (RPT t + u).
+template <typename B, int RPT>
 __global__ void __launch_bounds__(kBlock) k_ls_update(LsArgs a)
 {
     __shared__ double red[2 * kBlock / kWave];
-    const int nq = a.n_atiles[0] * kLsRpt;
+    const int nq = a.n_atiles[0] * RPT;
     for (int q = blockIdx.x; q < nq; q += gridDim.x) {
-        const int t = a.atiles[q / kLsRpt];
-        const int sub = q % kLsRpt;
+        const int t = a.atiles[q / RPT];
+        const int sub = q % RPT;
         const int ii = a.tile_inst[t];
         const KState *st = a.st + ii;
         if (!st->active || st->broke) continue;
@@ -461,7 +468,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_update(LsArgs a)
         }
         nrm = block_sum_dd(nrm, red);
         if (threadIdx.x == 0) {
-            const size_t slot = (size_t)t * kLsRpt + sub;
+            const size_t slot = (size_t)t * RPT + sub;
             a.partials[2 * slot] = nrm.s;
             a.partials[2 * slot + 1] = nrm.c;
         }
@@ -471,6 +478,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_update(LsArgs a)
 // Per instance: s = |w_{j+1}| from the update's quarter-tile partials,
 // happy breakdown (src/KryExpvFSP.cu:90-98) or H(j+1, j) = s and
 // sigma_{j+1} = 1 / s.
+template <int RPT>
 __global__ void __launch_bounds__(kBlock) k_ls_norm(LsArgs a)
 {
     __shared__ double red[2 * kBlock / kWave];
@@ -480,7 +488,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_norm(LsArgs a)
     if (!st->active || st->broke) return;
     const LsInst &I = a.inst[i];
     const double s = sqrt(dd_value(
-        sum_dd_partials(a.partials + 2 * (size_t)I.tile0 * kLsRpt, I.ntiles * kLsRpt, red)));
+        sum_dd_partials(a.partials + 2 * (size_t)I.tile0 * RPT, I.ntiles * RPT, red)));
     if (threadIdx.x == 0) {
         if (s < st->btol) {
             st->k1 = 0;
@@ -503,7 +511,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_control(LsArgs a, int max_reject)
 }
 
 // p = beta V[:, 0:mx] F0 per running tile (src/KryExpvFSP.cu:160-166).
-template <typename B>
+template <typename B, int RPT>
 __global__ void __launch_bounds__(kBlock) k_ls_solution(LsArgs a)
 {
     const int na = a.n_atiles[0];
@@ -518,19 +526,19 @@ __global__ void __launch_bounds__(kBlock) k_ls_solution(LsArgs a)
         const int mx = st->mx;
         const double beta = st->beta;
         const uint32_t nv = (uint32_t)I.nv;
-        double acc[kLsRpt];
+        double acc[RPT];
 #pragma unroll
-        for (int u = 0; u < kLsRpt; ++u) acc[u] = 0.0;
+        for (int u = 0; u < RPT; ++u) acc[u] = 0.0;
         for (int k = 0; k < mx; ++k) {
             const double fk = st->F0[k];
 #pragma unroll
-            for (int u = 0; u < kLsRpt; ++u) {
+            for (int u = 0; u < RPT; ++u) {
                 const uint32_t r = (uint32_t)(r0 + u * kBlock);
                 if (r < nv) acc[u] += basis<B>(I.V64, Vb, (size_t)nv, k, r) * fk;
             }
         }
 #pragma unroll
-        for (int u = 0; u < kLsRpt; ++u) {
+        for (int u = 0; u < RPT; ++u) {
             const uint32_t r = (uint32_t)(r0 + u * kBlock);
             if (r < nv) I.p[r] = beta * acc[u];
         }
@@ -540,14 +548,14 @@ __global__ void __launch_bounds__(kBlock) k_ls_solution(LsArgs a)
 // ------------------------------------------------------ set-up and export
 // HYB generator of every instance from its model (the single-solve HYB
 // fill's row values: fsp_assemble.h), one row per thread over all tiles.
-template <int D, typename V>
+template <int D, typename V, int RPT>
 __global__ void __launch_bounds__(kBlock) k_ls_fill_hyb(LsArgs a, int mixed)
 {
     for (int t = blockIdx.x; t < a.T; t += gridDim.x) {
         const LsInst &I = a.inst[a.tile_inst[t]];
         const DevModel *dm = I.dm;
         V *vals = (V *)I.hvals;
-        for (int u = 0; u < kLsRpt; ++u) {
+        for (int u = 0; u < RPT; ++u) {
             const uint32_t i = (uint32_t)(a.tile_r0[t] + (int)threadIdx.x + u * kBlock);
             if (i >= (uint32_t)I.n) continue;
             int xs[D];
@@ -924,6 +932,8 @@ struct LsHost {
     void *d_hvals = nullptr;
     int *d_hoff = nullptr;
     int T = 0, grid = 0, graph_grid = 0;
+    int rpt = 0;                    // tile rows / kBlock of the tile tables
+    std::vector<LsInst> h_inst;     // host copy of d_inst
     hipStream_t s = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -991,15 +1001,58 @@ void batch_lockstep_free(BatchImpl &B)
     B.ls = nullptr;
 }
 
+// Tile tables for tiles of B.ls_rpt * kBlock rows over every instance's nv
+// entries: each instance's first tile and tile count (uploaded with the
+// instances), the running-tile list and the partials sized for them.  The
+// captured batch step holds the tile count, so it is recaptured.
+static fsp_status ls_tiles(BatchImpl &B, LsHost &L)
+{
+    const int rpt = (B.ls_rpt == 2 || B.ls_rpt == 8) ? B.ls_rpt : 4;
+    const int rows = rpt * kBlock;
+    std::vector<int> tinst, tr0;
+    for (int i = 0; i < B.count; ++i) {
+        LsInst &I = L.h_inst[i];
+        I.tile0 = (int)tinst.size();
+        I.ntiles = (I.nv + rows - 1) / rows;
+        for (int u = 0; u < I.ntiles; ++u) {
+            tinst.push_back(i);
+            tr0.push_back(u * rows);
+        }
+    }
+    void *old[] = {L.d_tile_inst, L.d_tile_r0, L.d_atiles, L.d_partials};
+    for (void *p : old)
+        if (p) (void)hipFree(p);
+    L.d_tile_inst = L.d_tile_r0 = L.d_atiles = nullptr;
+    L.d_partials = nullptr;
+    L.T = (int)tinst.size();
+    FSP_HIP(copy_sync(L.d_inst, L.h_inst.data(), sizeof(LsInst) * (size_t)B.count, hipMemcpyHostToDevice,
+                      nullptr));
+    FSP_HIP(hipMalloc(&L.d_tile_inst, sizeof(int) * (size_t)L.T));
+    FSP_HIP(hipMalloc(&L.d_tile_r0, sizeof(int) * (size_t)L.T));
+    FSP_HIP(hipMalloc(&L.d_atiles, sizeof(int) * (size_t)L.T));
+    FSP_HIP(copy_sync(L.d_tile_inst, tinst.data(), sizeof(int) * (size_t)L.T, hipMemcpyHostToDevice, nullptr));
+    FSP_HIP(copy_sync(L.d_tile_r0, tr0.data(), sizeof(int) * (size_t)L.T, hipMemcpyHostToDevice, nullptr));
+    // dot partials: 2m + 4 slots of T (dots, sink, Gram column); the
+    // update's part-tile norms: rpt T
+    const size_t pslots = (size_t)std::max(2 * B.m + 4, rpt) * (size_t)L.T;
+    FSP_HIP(hipMalloc(&L.d_partials, sizeof(double) * 2 * pslots));
+    if (L.gexec) (void)hipGraphExecDestroy(L.gexec);
+    if (L.graph) (void)hipGraphDestroy(L.graph);
+    L.gexec = nullptr;
+    L.graph = nullptr;
+    L.graph_grid = 0;
+    L.rpt = rpt;
+    return FSP_OK;
+}
+
 static fsp_status ls_build(BatchImpl &B)
 {
     if (B.ls) return FSP_OK;
     B.ls = new LsHost();
     LsHost &L = *B.ls;
     const int count = B.count, m = B.m;
-    // tiles of kLsRows rows over every instance's nv entries
-    std::vector<int> tinst, tr0;
-    std::vector<LsInst> hi((size_t)count);
+    L.h_inst.assign((size_t)count, LsInst{});
+    std::vector<LsInst> &hi = L.h_inst;
     size_t basis_total = 0, basis32_total = 0, work_total = 0, diag_total = 0, slot_total = 0;
     std::vector<int> offs;  // slot column offsets of every instance, -delta_u
     for (int i = 0; i < count; ++i) {
@@ -1011,12 +1064,6 @@ static fsp_status ls_build(BatchImpl &B)
         I.with_sink = bi.with_sink;
         I.tab_size = bi.tab_size;
         I.t_final = bi.t_final;
-        I.tile0 = (int)tinst.size();
-        I.ntiles = (bi.nv + kLsRows - 1) / kLsRows;
-        for (int u = 0; u < I.ntiles; ++u) {
-            tinst.push_back(i);
-            tr0.push_back(u * kLsRows);
-        }
         basis_total += (size_t)bi.nv * (B.mixed ? 1 : (size_t)(m + 1));
         basis32_total += B.mixed ? (size_t)bi.nv * (size_t)m : 0;
         work_total += 2 * (size_t)bi.nv;
@@ -1028,7 +1075,6 @@ static fsp_status ls_build(BatchImpl &B)
             offs.push_back((int)std::max<int64_t>(-(int64_t)bi.n, std::min<int64_t>((int64_t)bi.n, d)));
         }
     }
-    L.T = (int)tinst.size();
     FSP_HIP(hipMalloc(&L.d_basis, sizeof(double) * basis_total));
     if (B.mixed) FSP_HIP(hipMalloc(&L.d_basis32, sizeof(float) * basis32_total));
     FSP_HIP(hipMalloc(&L.d_work, sizeof(double) * work_total));
@@ -1064,12 +1110,7 @@ static fsp_status ls_build(BatchImpl &B)
         oo += (size_t)I.U;
     }
     FSP_HIP(hipMalloc(&L.d_inst, sizeof(LsInst) * (size_t)count));
-    FSP_HIP(copy_sync(L.d_inst, hi.data(), sizeof(LsInst) * (size_t)count, hipMemcpyHostToDevice, nullptr));
-    FSP_HIP(hipMalloc(&L.d_tile_inst, sizeof(int) * (size_t)L.T));
-    FSP_HIP(hipMalloc(&L.d_tile_r0, sizeof(int) * (size_t)L.T));
-    FSP_HIP(hipMalloc(&L.d_atiles, sizeof(int) * (size_t)L.T));
-    FSP_HIP(copy_sync(L.d_tile_inst, tinst.data(), sizeof(int) * (size_t)L.T, hipMemcpyHostToDevice, nullptr));
-    FSP_HIP(copy_sync(L.d_tile_r0, tr0.data(), sizeof(int) * (size_t)L.T, hipMemcpyHostToDevice, nullptr));
+    FSP_TRY(ls_tiles(B, L));
     FSP_HIP(hipMalloc(&L.d_cnt, sizeof(int) * 4));
     FSP_HIP(fill_sync(L.d_cnt, 0, sizeof(int) * 4, nullptr));
     FSP_HIP(hipMalloc(&L.d_owner, sizeof(int) * (size_t)count));
@@ -1084,10 +1125,6 @@ static fsp_status ls_build(BatchImpl &B)
         L.ticks_per_s = (long long)std::max(khz, 1) * 1000;
     }
     FSP_HIP(hipHostMalloc((void **)&L.h_cnt, sizeof(int) * 4, hipHostMallocDefault));
-    // dot partials: 2m + 4 slots of T (dots, sink, Gram column); the
-    // update's quarter-tile norms: kLsRpt T
-    const size_t pslots = (size_t)std::max(2 * m + 4, kLsRpt) * (size_t)L.T;
-    FSP_HIP(hipMalloc(&L.d_partials, sizeof(double) * 2 * pslots));
     // step states: constants once (tolerance, decimal powers), the rest per solve
     std::vector<double> p10(kP10Hi - kP10Lo + 1);
     FSP_HIP(copy_sync(p10.data(), B.d_p10, sizeof(double) * p10.size(), hipMemcpyDeviceToHost, nullptr));
@@ -1112,29 +1149,31 @@ static fsp_status ls_build(BatchImpl &B)
     // HYB generators
     const LsArgs a = ls_args(B, L);
     const int fgrid = std::max(1, std::min(L.T, 65535));
-    if (B.dmax <= 4) {
-        if (B.mixed)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_fill_hyb<4, float>), dim3(fgrid), dim3(kBlock), 0, L.s, a, 1);
-        else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_fill_hyb<4, double>), dim3(fgrid), dim3(kBlock), 0, L.s, a, 0);
-    } else {
-        if (B.mixed)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_fill_hyb<8, float>), dim3(fgrid), dim3(kBlock), 0, L.s, a, 1);
-        else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_fill_hyb<8, double>), dim3(fgrid), dim3(kBlock), 0, L.s, a, 0);
-    }
+    FSP_LS_RPT_SWITCH(L.rpt, {
+        if (B.dmax <= 4) {
+            if (B.mixed)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_fill_hyb<4, float, RPT_>), dim3(fgrid), dim3(kBlock), 0, L.s, a, 1);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_fill_hyb<4, double, RPT_>), dim3(fgrid), dim3(kBlock), 0, L.s, a, 0);
+        } else {
+            if (B.mixed)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_fill_hyb<8, float, RPT_>), dim3(fgrid), dim3(kBlock), 0, L.s, a, 1);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_fill_hyb<8, double, RPT_>), dim3(fgrid), dim3(kBlock), 0, L.s, a, 0);
+        }
+    });
     FSP_HIP(hipGetLastError());
     FSP_HIP(hipStreamSynchronize(L.s));
     return FSP_OK;
 }
 
-template <typename Bt>
+template <typename Bt, int RPT>
 static fsp_status ls_enqueue_step_t(BatchImpl &B, LsHost &L, int grid, hipStream_t s)
 {
     const int m = B.m, M2 = m + 2;
     LsArgs a = ls_args(B, L);
     hipLaunchKernelGGL(k_ls_compact, dim3(1), dim3(1024), 0, s, a);
-    hipLaunchKernelGGL(k_ls_norm_partials, dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_norm_partials<RPT>), dim3(grid), dim3(kBlock), 0, s, a);
     hipLaunchKernelGGL(k_ls_step_begin, dim3(B.count), dim3(kBlock), 0, s, a, (long long)B.max_nstep);
     FSP_HIP(hipGetLastError());
     int istart = 0;
@@ -1145,26 +1184,29 @@ static fsp_status ls_enqueue_step_t(BatchImpl &B, LsHost &L, int grid, hipStream
         a.istart = last ? 0 : istart;
         a.mode = last ? 1 : 0;
         a.nk = last ? 1 : j - istart + 1;
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_arn<Bt>), dim3(grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_arn<Bt, RPT>), dim3(grid), dim3(kBlock), 0, s, a);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_dots<Bt>), dim3(B.count * a.nk), dim3(kBlock), 0, s, a);
         FSP_HIP(hipGetLastError());
         if (last) break;
         if (B.mixed)
             hipLaunchKernelGGL(k_ls_gram_coef, dim3(B.count), dim3(kBlock), kGramLdsBytes, s, a);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update<Bt>), dim3(grid * kLsRpt), dim3(kBlock), 0, s, a);
-        hipLaunchKernelGGL(k_ls_norm, dim3(B.count), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_update<Bt, RPT>), dim3(grid * RPT), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_norm<RPT>), dim3(B.count), dim3(kBlock), 0, s, a);
         FSP_HIP(hipGetLastError());
     }
     const size_t lds = sizeof(double) * 5 * (size_t)M2 * M2;
     hipLaunchKernelGGL(k_ls_control, dim3(B.count), dim3(kBlock), lds, s, a, B.max_reject);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_solution<Bt>), dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ls_solution<Bt, RPT>), dim3(grid), dim3(kBlock), 0, s, a);
     FSP_HIP(hipGetLastError());
     return FSP_OK;
 }
 
 static fsp_status ls_enqueue_step(BatchImpl &B, LsHost &L, int grid, hipStream_t s)
 {
-    return B.mixed ? ls_enqueue_step_t<float>(B, L, grid, s) : ls_enqueue_step_t<double>(B, L, grid, s);
+    fsp_status st = FSP_OK;
+    FSP_LS_RPT_SWITCH(L.rpt, (st = B.mixed ? ls_enqueue_step_t<float, RPT_>(B, L, grid, s)
+                                           : ls_enqueue_step_t<double, RPT_>(B, L, grid, s)));
+    return st;
 }
 
 // Workgroups of the row kernels: one per tile (blocks_per_cu = 0; those
@@ -1211,6 +1253,7 @@ static bool ls_use_graph()
 fsp_status batch_lockstep_prepare(BatchImpl &B, int blocks_per_cu)
 {
     FSP_TRY(ls_build(B));
+    if (B.ls->rpt != ((B.ls_rpt == 2 || B.ls_rpt == 8) ? B.ls_rpt : 4)) FSP_TRY(ls_tiles(B, *B.ls));
     if (!ls_use_graph()) return FSP_OK;
     return ls_graph(B, *B.ls, ls_grid(B, *B.ls, blocks_per_cu));
 }

@@ -250,6 +250,8 @@ typedef struct fsp_batch_options {
                          to the persistent kernel (0: 16384; < 0: all)         */
     double time_limit_s;  /* wall-clock seconds a persistent workgroup may run
                          before its instances end with FSP_ERR_TIMEOUT (0: 600) */
+    int lockstep_rows;    /* lockstep tile rows per thread: 2, 4 or 8 (tiles of
+                         512, 1024 or 2048 rows; 0: 4)                         */
 } fsp_batch_options;
 
 /* Batch execution paths.  LOCKSTEP: one graph-captured batch step advances
@@ -296,15 +298,16 @@ fsp_status fsp_batch_get_stats(fsp_batch b, int i, int64_t *steps, int64_t *reje
  * streams). */
 fsp_status fsp_batch_get_config(fsp_batch b, int *block, int *wg_per_cu);
 /* The path a solve takes and every tuned setting (any pointer may be NULL). */
-fsp_status fsp_batch_get_tuning(fsp_batch b, int *path, int *lockstep_bpc, int *persistent_wg,
-                                int64_t *split_states);
+fsp_status fsp_batch_get_tuning(fsp_batch b, int *path, int *lockstep_bpc, int *lockstep_rows,
+                                int *persistent_wg, int64_t *split_states);
 /* Times candidate configurations on this batch, one whole solve each after
  * an untimed one (LOCKSTEP: row-kernel workgroups one per tile, 4 or 16 per
- * CU; PERSISTENT: 1, 2 or 4 workgroups per CU; HYBRID: the lockstep grids,
- * then splits at 4,096, 16,384 and 65,536 entries; AUTO: all of these and
- * the path; STREAMS: 1, 2 or 4 streams), keeps the fastest and records it
- * in the JSON file cache_path (NULL: no cache); a cached entry for the same
- * device / shape is reused unless force. */
+ * CU, times tiles of 512, 1024 or 2048 rows; PERSISTENT: 1, 2 or 4
+ * workgroups per CU; HYBRID: the lockstep configurations, then splits at
+ * 4,096, 16,384 and 65,536 entries; AUTO: all of these and the path;
+ * STREAMS: 1, 2 or 4 streams), keeps the fastest and records it in the JSON
+ * file cache_path (NULL: no cache); a cached entry for the same device /
+ * shape is reused unless force. */
 fsp_status fsp_batch_autotune(fsp_batch b, const char *cache_path, int force, double *best_ms);
 
 /* ---- time-varying right-hand side and BDF integration ---- */

@@ -177,11 +177,12 @@ def _oracle_kw(net, t, sink=False, **kw):
     return out[0], st
 
 
-@pytest.mark.parametrize("iop", [False, True])
-def test_simt_lockstep_f64_bit_exact(sim, iop):
+@pytest.mark.parametrize("iop,rows", [(False, 4), (True, 4), (False, 2), (False, 8)])
+def test_simt_lockstep_f64_bit_exact(sim, iop, rows):
     nets, tf = _instances(3)
     sinks = [False, True, False]
-    B = capi.Batch(nets, tf, with_sink=sinks, precision="f64", path="lockstep", iop=iop, q_iop=2)
+    B = capi.Batch(nets, tf, with_sink=sinks, precision="f64", path="lockstep", iop=iop, q_iop=2,
+                   lockstep_rows=rows)
     assert B.config()[0] == -1
     B.solve()
     for i, net in enumerate(nets):
@@ -197,12 +198,13 @@ def test_simt_lockstep_f64_bit_exact(sim, iop):
         assert np.array_equal(B.solution(i), first[i])
 
 
-def test_simt_lockstep_mixed(sim):
+@pytest.mark.parametrize("rows", [2, 4, 8])
+def test_simt_lockstep_mixed(sim, rows):
     nets, tf = _instances(2, seed=9)
     nets = nets + [M.toggle_switch((20, 20), rates=TOGGLE_CGS_RATES)]
     tf = tf + [4.0]
-    B64 = capi.Batch(nets, tf, precision="f64", path="lockstep")
-    B32 = capi.Batch(nets, tf, precision="mixed", path="lockstep")
+    B64 = capi.Batch(nets, tf, precision="f64", path="lockstep", lockstep_rows=rows)
+    B32 = capi.Batch(nets, tf, precision="mixed", path="lockstep", lockstep_rows=rows)
     B64.solve()
     B32.solve()
     for i in range(len(nets)):
@@ -222,6 +224,7 @@ def test_simt_lockstep_initial_vector_and_autotune(sim, tmp_path):
     cfg = B.config()
     entry = list(json.load(open(path)).values())[0]
     assert cfg[0] == -1 and cfg[1] in (0, 4, 16) and entry["path"] == 1 and entry["bpc"] == cfg[1]
+    assert entry["rpt"] in (2, 4, 8) and B.tuning()[2] == entry["rpt"]
     B.solve()
     opr = O.Operator(net, "hyb")
     rc, out, st = O.expv_outputs(opr, p0.copy(), [1.5])

@@ -38,12 +38,16 @@ def _oracle(net, t, sink, fmt="kron"):
 LIMIT_S = 30.0
 
 
-@pytest.mark.parametrize("sink,path", [(False, "lockstep"), (True, "lockstep"), (True, "streams")])
-def test_batch_f64_matches_oracle(sink, path):
+@pytest.mark.parametrize("sink,path,rows", [(False, "lockstep", 4), (True, "lockstep", 4),
+                                            (True, "lockstep", 2), (False, "lockstep", 8),
+                                            (True, "streams", 0)])
+def test_batch_f64_matches_oracle(sink, path, rows):
+    """rows: lockstep tiles of rows x 256 rows."""
     O.set_threads(8)
     nets, tf, ws = _small_batch(sink=sink)
-    print(f"[case] batch f64 sink={sink} path={path} n={[n.n_states for n in nets][:6]}...", flush=True)
-    B = capi.Batch(nets, tf, with_sink=ws, precision="f64", path=path)
+    print(f"[case] batch f64 sink={sink} path={path} rows={rows} n={[n.n_states for n in nets][:6]}...",
+          flush=True)
+    B = capi.Batch(nets, tf, with_sink=ws, precision="f64", path=path, lockstep_rows=rows)
     if path == "lockstep":
         assert B.config()[0] == -1
     B.solve()
@@ -92,15 +96,16 @@ def test_persistent_and_hybrid_match_oracle(path, sink):
 TOGGLE_CGS_RATES = (7.838583757547972, 0.5659944527754909, 13.088930136958103, 6.072545924918103)
 
 
-@pytest.mark.parametrize("path", ["lockstep", "persistent"])
-def test_batch_mixed_vs_f64_and_replay(path):
+@pytest.mark.parametrize("path,rows", [("lockstep", 4), ("lockstep", 2), ("lockstep", 8),
+                                       ("persistent", 0)])
+def test_batch_mixed_vs_f64_and_replay(path, rows):
     nets, tf, ws = _small_batch(count=16, seed=9)
     # rates under which one Gram-Schmidt pass over an FP32 basis degenerates
     nets = nets + [M.toggle_switch((60, 60), rates=TOGGLE_CGS_RATES)]
     tf = np.append(tf, 20.0)
-    print(f"[case] batch mixed {path} n={[n.n_states for n in nets][:6]}...", flush=True)
-    B64 = capi.Batch(nets, tf, precision="f64", path=path, time_limit_s=LIMIT_S)
-    B32 = capi.Batch(nets, tf, precision="mixed", path=path, time_limit_s=LIMIT_S)
+    print(f"[case] batch mixed {path} rows={rows} n={[n.n_states for n in nets][:6]}...", flush=True)
+    B64 = capi.Batch(nets, tf, precision="f64", path=path, time_limit_s=LIMIT_S, lockstep_rows=rows)
+    B32 = capi.Batch(nets, tf, precision="mixed", path=path, time_limit_s=LIMIT_S, lockstep_rows=rows)
     B64.solve()
     B32.solve()
     ref = [B64.solution(i) for i in range(len(nets))]
@@ -123,16 +128,17 @@ def test_batch_autotune_cache(tmp_path):
     data = json.load(open(path))
     assert len(data) == 1
     entry = list(data.values())[0]
-    name, bpc, wg, split = B.tuning()
+    name, bpc, rpt, wg, split = B.tuning()
     assert capi.BATCH_PATHS[name] == entry["path"] and entry["path"] in (1, 3, 4)
-    assert (bpc, wg, split) == (entry["bpc"], entry["wg"], entry["split"])
+    assert (bpc, rpt, wg, split) == (entry["bpc"], entry["rpt"], entry["wg"], entry["split"])
+    assert rpt in (2, 4, 8)
     print(f"[batch] tuned {B.tuning()} {ms:.1f} ms", flush=True)
     B2 = capi.Batch(nets, tf, precision="mixed", time_limit_s=LIMIT_S)
     assert B2.autotune(path) == pytest.approx(entry["ms"])
     assert B2.tuning() == B.tuning()
     B2.solve()
     ref = capi.Batch(nets, tf, precision="mixed", path=name, lockstep_bpc=bpc, wg_per_cu=wg,
-                     split_states=split, time_limit_s=LIMIT_S)
+                     split_states=split, time_limit_s=LIMIT_S, lockstep_rows=rpt)
     ref.solve()
     for i in range(len(nets)):
         assert np.array_equal(B2.solution(i), ref.solution(i))
