@@ -110,6 +110,10 @@ __device__ inline uint32_t hyb_col(uint32_t i, int off, uint32_t n)
     return (c >= 0 && c < (int64_t)n) ? (uint32_t)c : i;
 }
 
+// A zero slot value contributes +0 whatever x holds at its implicit column:
+// the column of an out-of-box neighbour names some other state (or row i
+// itself at the ends of the vector), and a non-finite x there must not reach
+// this row.
 // Row i with a compile-time slot count: all slot values are loaded first
 // (non-temporal: streamed once per product), with all gathers of x at the
 // implicit columns, then the sum in slot order, so a row costs one memory
@@ -126,7 +130,7 @@ __device__ inline double hyb_row_n(const HybF<V> &A, uint32_t i, const X &x, dd 
 #pragma unroll
     for (int u = 0; u < NU; ++u) xc[u] = x(hyb_col(i, A.slot_off[u], A.n));
 #pragma unroll
-    for (int u = 0; u < NU; ++u) y += (double)v[u] * xc[u];
+    for (int u = 0; u < NU; ++u) y += v[u] != (V)0 ? (double)v[u] * xc[u] : 0.0;
     if (A.sink_out) sink = dd_add_prod(sink, A.sink_out[i], xi);
     return y;
 }
@@ -154,7 +158,8 @@ __device__ inline double hyb_row(const HybF<V> &A, uint32_t i, const X &x, dd &s
     double y = A.diag[i] * xi;
     for (int u = 0; u < A.U; ++u) {
         const size_t o = (size_t)u * A.n + i;
-        y += (double)A.slot_vals[o] * x(hyb_col(i, A.slot_off[u], A.n));
+        const V v = A.slot_vals[o];
+        y += v != (V)0 ? (double)v * x(hyb_col(i, A.slot_off[u], A.n)) : 0.0;
     }
     if (A.sink_out) sink = dd_add_prod(sink, A.sink_out[i], xi);
     return y;

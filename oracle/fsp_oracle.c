@@ -531,8 +531,12 @@ void orc_ell_apply(int64_t n, int U, const double *diag,
 #pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         double acc = diag[i] * x[i];
-        for (int u = 0; u < U; ++u)
-            acc += slot_vals[(int64_t)u * n + i] * x[slot_cols[(int64_t)u * n + i]];
+        /* a zero slot (an out-of-box neighbour, or a zero rate) adds +0, as
+         * the device's HYB row does, whatever x holds at its column */
+        for (int u = 0; u < U; ++u) {
+            const double v = slot_vals[(int64_t)u * n + i];
+            acc += v != 0.0 ? v * x[slot_cols[(int64_t)u * n + i]] : 0.0;
+        }
         y[i] += acc;
     }
     if (sink_out) y[n] += dot_dd(n, sink_out, x);

@@ -138,6 +138,32 @@ def test_spmv_matches_oracle(mk):
     assert abs(got[n] - w[n]) <= 1e-14 * max(1.0, abs(w[n]))
 
 
+def _connected_rows(rp, c, n, k):
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    out = np.zeros(n, bool)
+    out[rows[c == k]] = True
+    return out
+
+
+@pytest.mark.parametrize("fmt", ["csr", "hyb", "kron"])
+def test_nonfinite_entry_reaches_only_connected_rows(fmt):
+    """A NaN in x at state k reaches row i of A x only where the stored
+    generator has an entry at (i, k) (the merged CSR structure, explicit
+    zeros included): the HYB layout's implicit columns of out-of-box
+    neighbours, which name other states, carry nothing."""
+    net = M.gene_expression((2, 20, 25))
+    n = net.n_states
+    rp, c, _ = O.merged_csr(net)
+    rng = np.random.default_rng(11)
+    A = capi.Mat(net, fmt, "f64")
+    for k in [0, 1, 2, 3, 62, n - 1] + rng.choice(n, 6, replace=False).tolist():
+        x = rng.random(n)
+        x[k] = np.nan
+        bad = ~np.isfinite(_apply(A, x))
+        assert bad[k], (fmt, k)
+        assert not (bad & ~_connected_rows(rp, c, n, k)).any(), (fmt, k, np.flatnonzero(bad))
+
+
 def test_reference_fsp_mat_test():
     # tests/test_fsp_mat.cu: toggle, bounds (31, 31); A.0 = 0 for every
     # format, CSR == HYB (L2 <= 1e-14), KRON vs HYB (L2 <= 1e-12)

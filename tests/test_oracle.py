@@ -298,3 +298,23 @@ def test_loglik_indices():
     idx = np.array([0, 1, 1, 2, 3])
     want = math.log(0.5) + 2 * math.log(0.25) + math.log(1e-44) + math.log(0.25)
     assert abs(O.loglik_indices(p, idx) - want) <= 1e-12
+
+
+def test_ell_apply_zero_slots_carry_nothing():
+    """orc_ell_apply: a NaN in x at state k reaches only the rows whose
+    stored CSR row has column k; the explicit ELL columns of out-of-box
+    neighbours point at row i itself, and zero slots add +0."""
+    net = M.gene_expression((2, 9, 11))
+    n = net.n_states
+    U, dg, sv, sc, _ = O.ell(net, with_sink=False)
+    rp, c, _ = O.merged_csr(net)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    for k in range(n):
+        x = np.ones(n)
+        x[k] = np.nan
+        y = np.zeros(n)
+        O.ell_apply(U, dg, sv, sc, None, x, y)
+        connected = np.zeros(n, bool)
+        connected[rows[c == k]] = True
+        bad = ~np.isfinite(y)
+        assert bad[k] and not (bad & ~connected).any(), k
