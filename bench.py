@@ -389,15 +389,19 @@ def grad_only(points):
     print(json.dumps(bench_adjoint(net, points)), flush=True)
 
 
-def bench_batch(count, seq_every=8):
+def bench_batch(count, seq_every=8, cpu_budget_s=15.0):
     """Uneven batch (toggle switches and gene-expression networks drawn
     50/50, rates within a decade of nominal, n log-uniform in [2,048,
     262,144], t_final uniform in [1, 100], seed 1), solved by the lockstep
-    path in mixed precision and in FP64.  The sequential plain FP64 leg (the
-    single-solve engine, HYB generator, one instance at a time) runs on every
-    `seq_every`-th instance (a fixed subset) and is scaled to the whole batch
-    by the subset's share of the state-steps of the lockstep FP64 solve; the
-    worst-instance L1 distances are over that subset."""
+    path in mixed precision and in FP64: set-up (buffers, HYB generators,
+    tile tables, step graph) untimed, then one timed solve.  The sequential
+    plain FP64 leg (the single-solve engine, HYB generator, one instance at
+    a time, set-up untimed) runs on every `seq_every`-th instance (a fixed
+    subset) and is scaled to the whole batch by the subset's share of the
+    state-steps of the lockstep FP64 solve; the worst-instance L1 distances
+    are over that subset.  The CPU leg runs the oracle (HYB row arithmetic)
+    on the subset's instances in order until `cpu_budget_s` has passed and
+    scales the same way."""
     nets, tf = M.batch_instances(count, seed=1)
     out = {"workload": f"{count} uneven instances (toggle / gene expression 50/50, n in "
                        f"[2048, 262144], t_final in [1, 100], seed 1), m=30 tol=1e-8",
@@ -407,16 +411,20 @@ def bench_batch(count, seq_every=8):
         B = capi.Batch(nets, tf, precision=prec, path="lockstep")
         capi.check(capi.lib().fsp_stream_sync(None))
         t0 = time.perf_counter()
-        B.solve()  # includes the first solve's HYB fill, allocations and graph capture
+        B.prepare()
+        t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        B.solve()
         dt = time.perf_counter() - t0
         st = [B.stats(i) for i in range(count)]
         lock[prec] = {"sol": {i: B.solution(i) for i in range(0, count, seq_every)},
                       "steps": [int(x[0]) for x in st]}
-        out[f"lockstep_{prec}"] = {"solve_s": round(dt, 3), "includes_setup": True,
+        out[f"lockstep_{prec}"] = {"solve_s": round(dt, 3), "setup_s": round(t_setup, 3),
                                    "solves_per_s": round(count / dt, 3),
+                                   "tuning": B.tuning(),
                                    "krylov_steps": int(sum(x[0] for x in st)),
                                    "failed": sum(1 for x in st if x[2])}
-        print(f"[bench] batch {count}: lockstep {prec} {dt:.2f} s", flush=True)
+        print(f"[bench] batch {count}: lockstep {prec} {dt:.2f} s (set-up {t_setup:.2f} s)", flush=True)
         B.close()
     subset = list(range(0, count, seq_every))
     t_seq, worst = 0.0, {"mixed": 0.0, "f64": 0.0}
@@ -455,6 +463,28 @@ def bench_batch(count, seq_every=8):
         o["worst_l1_vs_sequential_f64_subset"] = worst[prec]
     print(f"[bench] batch {count}: sequential FP64 subset {t_seq:.2f} s "
           f"(share {share:.3f})", flush=True)
+    # CPU oracle on the subset's instances, in order, within the budget
+    from oracle import oracle as O
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1), 16)
+    O.set_threads(threads)
+    t_cpu, done = 0.0, []
+    for i in subset:
+        if t_cpu >= cpu_budget_s:
+            break
+        opr = O.Operator(nets[i], "hyb")
+        t0 = time.perf_counter()
+        rc, res, _ = O.expv_outputs(opr, M.delta_p0(nets[i]), [float(tf[i])])
+        t_cpu += time.perf_counter() - t0
+        done.append(i)
+    cpu_share = sum(work[i] for i in done) / sum(work)
+    out["cpu_oracle"] = {"kind": "port", "cores": threads,
+                         "sample": f"instances {done[0]}, {done[1] if len(done) > 1 else ''} ... of the "
+                                   f"subset ({len(done)} instances, {cpu_share:.4f} of the batch's "
+                                   f"state-steps) in {t_cpu:.1f} s",
+                         "estimated_solve_s": round(t_cpu / cpu_share, 1),
+                         "solves_per_s": round(count * cpu_share / t_cpu, 4)}
+    print(f"[bench] batch {count}: CPU oracle {len(done)} instances {t_cpu:.1f} s "
+          f"(share {cpu_share:.4f})", flush=True)
     return out
 
 

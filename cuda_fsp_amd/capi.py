@@ -148,6 +148,7 @@ _SIGS = {
     "fsp_batch_get_stats": (C.c_int, [_vp, _i, _vp, _vp, _vp]),
     "fsp_batch_get_config": (C.c_int, [_vp, _vp, _vp]),
     "fsp_batch_get_tuning": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "fsp_batch_prepare": (C.c_int, [_vp]),
     "fsp_batch_autotune": (C.c_int, [_vp, C.c_char_p, _i, _vp]),
     "fsp_loglik_indices": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
     "fsp_loglik_indices_noalloc": (C.c_int, [_vp, _vp, _i64, _d, _vp, _vp]),
@@ -659,6 +660,11 @@ class Batch:
 
     def solve(self, stream=None):
         check(lib().fsp_batch_solve(self.h, stream))
+
+    def prepare(self):
+        """Set-up of the solve path (buffers, generators, graphs) ahead of a
+        timed solve."""
+        check(lib().fsp_batch_prepare(self.h))
 
     def solution(self, i) -> np.ndarray:
         out = np.empty(self.nv[i], np.float64)

@@ -266,6 +266,13 @@ extern "C" fsp_status fsp_batch_solve(fsp_batch b, void *stream)
     return batch_solve_streams(B);
 }
 
+extern "C" fsp_status fsp_batch_prepare(fsp_batch b)
+{
+    using namespace fsp;
+    if (!b) return fail(FSP_ERR_INVALID, "NULL batch");
+    return batch_prepare(b->impl, batch_path(b->impl));
+}
+
 extern "C" fsp_status fsp_batch_get_solution(fsp_batch b, int i, double *h_p, int64_t cap)
 {
     using namespace fsp;

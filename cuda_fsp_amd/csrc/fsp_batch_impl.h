@@ -104,6 +104,9 @@ fsp_status batch_solve_hybrid(BatchImpl &B, int blocks_per_cu, int wg_per_cu, in
 // The persistent kernel alone as one captured graph on `stream`
 // (asynchronous: instance reset, queue reset, the kernel, the stats export).
 fsp_status batch_launch_persistent(BatchImpl &B, int wg_per_cu, hipStream_t stream);
+// Allocations, tile tables and captured graphs of a path (LOCKSTEP, HYBRID,
+// PERSISTENT), so that a following solve does no set-up; STREAMS: nothing.
+fsp_status batch_prepare(BatchImpl &B, int path);
 void batch_lockstep_free(BatchImpl &B);
 fsp_status batch_autotune_streams(BatchImpl &B, double *best_ms, int *best_workers);
 void batch_streams_free(BatchImpl &B);

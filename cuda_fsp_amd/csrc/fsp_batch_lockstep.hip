@@ -234,14 +234,16 @@ __device__ inline double ls_row(const LsInst &I, uint32_t r, const KryX<B> &X, d
 }
 
 // Row sums of the thread's RPT rows of a tile (r0 + u kBlock) for a
-// compile-time slot count, RG rows at a time (the loads of a group issued
-// before their sums; 2 with an FP64 basis, 4 with FP32, sized to keep the
-// kernel's occupancy); rows past the instance read row 0 and are
-// discarded.  No sink terms (the caller adds them in row order).
+// compile-time slot count, RG = 2 rows at a time (the loads of a group
+// issued before their sums; two rows keep the kernel at 5 waves per SIMD
+// in both precisions: four FP32 rows, with each slot value held for its
+// zero test until its gather returns, took 108 VGPRs and the 512-instance
+// mixed batch from 28.3 to 30.0 s); rows past the instance read row 0 and
+// are discarded.  No sink terms (the caller adds them in row order).
 template <int NU, typename B, int RPT>
 __device__ inline void ls_rows(const LsInst &I, int r0, const KryX<B> &X, double (&y)[RPT])
 {
-    constexpr int RG0 = std::is_same<B, double>::value ? 2 : 4;
+    constexpr int RG0 = 2;
     constexpr int RG = RG0 < RPT ? RG0 : RPT;  // a group never runs past y[RPT]
     static_assert(RPT % RG == 0, "rows per thread must be a multiple of the group");
     const HybF<B> A = ls_hyb<B>(I, false);
@@ -1361,18 +1363,29 @@ static fsp_status instance_failures(BatchImpl &B)
     return FSP_OK;
 }
 
-fsp_status batch_solve_hybrid(BatchImpl &B, int blocks_per_cu, int wg_per_cu, int64_t split)
+// Allocations, tile tables and captured graphs of a hybrid solve with this
+// split (the lockstep step graph when any instance is lockstep, the
+// persistent kernel's graph when any is persistent).
+static fsp_status hybrid_prepare(BatchImpl &B, int blocks_per_cu, int wg_per_cu, int64_t split)
 {
     FSP_TRY(ls_build(B));
     LsHost &L = *B.ls;
     FSP_TRY(pq_assign(B, L, split));
-    const int n_pq = L.pq_count, n_ls = B.count - L.pq_count;
-    if (n_ls > 0) FSP_TRY(batch_lockstep_prepare(B, blocks_per_cu));
+    if (B.count - L.pq_count > 0) FSP_TRY(batch_lockstep_prepare(B, blocks_per_cu));
     const int pgrid = pq_grid(B, L, wg_per_cu);
-    if (n_pq > 0 && ls_use_graph() && (!L.pq_gexec || L.pq_graph_grid != pgrid)) {
+    if (L.pq_count > 0 && ls_use_graph() && (!L.pq_gexec || L.pq_graph_grid != pgrid)) {
         FSP_TRY(capture(&L.pq_graph, &L.pq_gexec, [&](hipStream_t c) { return pq_enqueue(B, L, pgrid, c); }));
         L.pq_graph_grid = pgrid;
     }
+    return FSP_OK;
+}
+
+fsp_status batch_solve_hybrid(BatchImpl &B, int blocks_per_cu, int wg_per_cu, int64_t split)
+{
+    FSP_TRY(hybrid_prepare(B, blocks_per_cu, wg_per_cu, split));
+    LsHost &L = *B.ls;
+    const int n_pq = L.pq_count, n_ls = B.count - L.pq_count;
+    const int pgrid = pq_grid(B, L, wg_per_cu);
     LsArgs a = ls_args(B, L);
     hipLaunchKernelGGL(k_ls_reset, dim3(B.count), dim3(kBlock), 0, L.s, a);
     FSP_HIP(hipGetLastError());
@@ -1416,27 +1429,45 @@ fsp_status batch_solve_lockstep(BatchImpl &B, int blocks_per_cu)
     return batch_solve_hybrid(B, blocks_per_cu, 1, 0);
 }
 
-fsp_status batch_launch_persistent(BatchImpl &B, int wg_per_cu, hipStream_t stream)
+// The all-persistent launch as one graph: instance reset, queue reset, the
+// kernel, the stats export.
+static fsp_status full_body(BatchImpl &B, LsHost &L, int grid, hipStream_t c)
+{
+    const LsArgs a = ls_args(B, L);
+    hipLaunchKernelGGL(k_ls_reset, dim3(B.count), dim3(kBlock), 0, c, a);
+    FSP_TRY(pq_enqueue(B, L, grid, c));
+    hipLaunchKernelGGL(k_ls_export, dim3((B.count + kBlock - 1) / kBlock), dim3(kBlock), 0, c, a, B.d_inst);
+    FSP_HIP(hipGetLastError());
+    return FSP_OK;
+}
+
+static fsp_status persistent_prepare(BatchImpl &B, int wg_per_cu)
 {
     FSP_TRY(ls_build(B));
     LsHost &L = *B.ls;
     FSP_TRY(pq_assign(B, L, -1));
     const int grid = pq_grid(B, L, wg_per_cu);
-    auto body = [&](hipStream_t c) -> fsp_status {
-        const LsArgs a = ls_args(B, L);
-        hipLaunchKernelGGL(k_ls_reset, dim3(B.count), dim3(kBlock), 0, c, a);
-        FSP_TRY(pq_enqueue(B, L, grid, c));
-        hipLaunchKernelGGL(k_ls_export, dim3((B.count + kBlock - 1) / kBlock), dim3(kBlock), 0, c, a,
-                           B.d_inst);
-        FSP_HIP(hipGetLastError());
-        return FSP_OK;
-    };
-    if (!ls_use_graph()) return body(stream);
-    if (!L.full_gexec || L.full_graph_grid != grid) {
-        FSP_TRY(capture(&L.full_graph, &L.full_gexec, body));
+    if (ls_use_graph() && (!L.full_gexec || L.full_graph_grid != grid)) {
+        FSP_TRY(capture(&L.full_graph, &L.full_gexec, [&](hipStream_t c) { return full_body(B, L, grid, c); }));
         L.full_graph_grid = grid;
     }
+    return FSP_OK;
+}
+
+fsp_status batch_launch_persistent(BatchImpl &B, int wg_per_cu, hipStream_t stream)
+{
+    FSP_TRY(persistent_prepare(B, wg_per_cu));
+    LsHost &L = *B.ls;
+    if (!ls_use_graph()) return full_body(B, L, pq_grid(B, L, wg_per_cu), stream);
     FSP_HIP(hipGraphLaunch(L.full_gexec, stream));
+    return FSP_OK;
+}
+
+fsp_status batch_prepare(BatchImpl &B, int path)
+{
+    if (path == FSP_BATCH_LOCKSTEP) return hybrid_prepare(B, B.ls_bpc, B.pq_wg, 0);
+    if (path == FSP_BATCH_HYBRID) return hybrid_prepare(B, B.ls_bpc, B.pq_wg, B.pq_split);
+    if (path == FSP_BATCH_PERSISTENT) return persistent_prepare(B, B.pq_wg);
     return FSP_OK;
 }
 

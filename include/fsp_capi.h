@@ -289,6 +289,10 @@ fsp_status fsp_batch_set_initial(fsp_batch b, int i, const double *h_p0);
  * time limit) makes the call return its status and a message naming it,
  * the others' results stay valid. */
 fsp_status fsp_batch_solve(fsp_batch b, void *stream);
+/* Does the set-up of the path a solve would take (device buffers, HYB
+ * generators, tile tables, captured graphs) so that the next solve does
+ * none; optional (a solve prepares what is missing).  STREAMS: nothing. */
+fsp_status fsp_batch_prepare(fsp_batch b);
 fsp_status fsp_batch_get_solution(fsp_batch b, int i, double *h_p, int64_t cap);
 fsp_status fsp_batch_get_stats(fsp_batch b, int i, int64_t *steps, int64_t *rejects,
                                int *status);

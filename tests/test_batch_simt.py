@@ -63,6 +63,7 @@ def test_simt_batch_f64_bit_exact(sim, path, wg):
     split = sorted(n.n_states for n in nets)[1] + 1
     B = capi.Batch(nets, tf, with_sink=sinks, precision="f64", path=path, wg_per_cu=wg,
                    split_states=split)
+    B.prepare()
     B.solve()
     for i, net in enumerate(nets):
         want, st = _oracle(net, tf[i], sinks[i])

@@ -75,6 +75,7 @@ def test_persistent_and_hybrid_match_oracle(path, sink):
     print(f"[case] batch {path} sink={sink} n={[n.n_states for n in nets]}", flush=True)
     B = capi.Batch(nets, tf, with_sink=ws, precision="f64", path=path, split_states=2000,
                    time_limit_s=LIMIT_S)
+    B.prepare()  # set-up (buffers, generators, graphs) ahead of the solve
     B.solve()
     print("[batch] solved", flush=True)
     worst = 0.0
