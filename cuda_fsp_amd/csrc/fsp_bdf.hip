@@ -158,8 +158,10 @@ __global__ void k_gm_post(const double *__restrict__ x, const double *__restrict
         out[i] = alpha0 * (x[i] / (alpha0 - diag[i])) - ewt[i] * u[i];
 }
 
-// Partial dd sums over the rows of block b of h_k = V_k . w (slot k) and of
-// the Gram column V_k . V_j (slot nk + k), k < nk = j + 1, from the same
+// Partial sums over the rows of block b of h_k = V_k . w (slot k, double-
+// double) and of the Gram column V_k . V_j (slot nk + k, plain FP64 in a
+// fixed order: its entries are of rounding size off the diagonal and only
+// correct the coefficients at that size), k < nk = j + 1, from the same
 // basis loads: 8 dots per pass over the rows.
 __global__ void k_gm_dots(const double *__restrict__ V, int64_t ldv, int nk, const double *__restrict__ w,
                           int64_t n, double *partials, const GmState *gm)
@@ -168,9 +170,13 @@ __global__ void k_gm_dots(const double *__restrict__ V, int64_t ldv, int nk, con
     __shared__ double red[2 * kBlock / kWave];
     const double *vj = V + (int64_t)(nk - 1) * ldv;
     for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
-        dd s[kDotChunk], q2[kDotChunk];
+        dd s[kDotChunk];
+        double q2[kDotChunk];
 #pragma unroll
-        for (int q = 0; q < kDotChunk; ++q) s[q] = q2[q] = dd_zero();
+        for (int q = 0; q < kDotChunk; ++q) {
+            s[q] = dd_zero();
+            q2[q] = 0.0;
+        }
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
              i += (int64_t)gridDim.x * blockDim.x) {
             const double wi = w[i], xj = vj[i];
@@ -180,21 +186,21 @@ __global__ void k_gm_dots(const double *__restrict__ V, int64_t ldv, int nk, con
 #pragma unroll
             for (int q = 0; q < kDotChunk; ++q) {
                 s[q] = dd_add_prod(s[q], v[q], wi);
-                q2[q] = dd_add_prod(q2[q], v[q], xj);
+                q2[q] = fma(v[q], xj, q2[q]);
             }
         }
 #pragma unroll
         for (int q = 0; q < kDotChunk; ++q) {
             if (c0 + q >= nk) break;
             const dd t = block_sum_dd(s[q], red);
-            const dd u = block_sum_dd(q2[q], red);
+            const double u = block_sum(q2[q], red);
             if (threadIdx.x == 0) {
                 const int64_t o = (int64_t)(c0 + q) * gridDim.x + blockIdx.x;
                 const int64_t og = (int64_t)(nk + c0 + q) * gridDim.x + blockIdx.x;
                 partials[2 * o] = t.s;
                 partials[2 * o + 1] = t.c;
-                partials[2 * og] = u.s;
-                partials[2 * og + 1] = u.c;
+                partials[2 * og] = u;
+                partials[2 * og + 1] = 0.0;
             }
         }
         __syncthreads();
