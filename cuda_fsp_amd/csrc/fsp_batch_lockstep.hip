@@ -1398,7 +1398,9 @@ fsp_status batch_solve_hybrid(BatchImpl &B, int blocks_per_cu, int wg_per_cu, in
         else
             FSP_TRY(pq_enqueue(B, L, pgrid, L.pq_s));
     }
-    if (n_ls > 0) {
+    // the lockstep steps; whatever happens to them, the persistent kernel is
+    // waited for before this call returns
+    auto lockstep_loop = [&]() -> fsp_status {
         // steps are launched in groups ahead of the host check; a step with
         // no running instance does nothing
         const int ahead = 4;
@@ -1413,11 +1415,13 @@ fsp_status batch_solve_hybrid(BatchImpl &B, int blocks_per_cu, int wg_per_cu, in
             FSP_HIP(hipGetLastError());
             FSP_HIP(hipMemcpyAsync(L.h_cnt, L.d_cnt, sizeof(int) * 4, hipMemcpyDeviceToHost, L.s));
             FSP_HIP(hipStreamSynchronize(L.s));
-            if (L.h_cnt[2] == 0) break;
+            if (L.h_cnt[2] == 0) return FSP_OK;
             if (guard > B.max_nstep + 8) return fail(FSP_ERR_NOT_CONVERGED, "batch steps exceeded max_nstep");
         }
-    }
+    };
+    const fsp_status ls_st = n_ls > 0 ? lockstep_loop() : FSP_OK;
     if (n_pq > 0) FSP_HIP(hipStreamSynchronize(L.pq_s));
+    FSP_TRY(ls_st);
     hipLaunchKernelGGL(k_ls_export, dim3((B.count + kBlock - 1) / kBlock), dim3(kBlock), 0, L.s, a, B.d_inst);
     FSP_HIP(hipGetLastError());
     FSP_HIP(hipStreamSynchronize(L.s));
