@@ -108,6 +108,21 @@ constexpr int kArnSlots = 2 * kMaxKrylov + 4;
 constexpr int kArnRedLen = 2 * kArnSlots * (kBlock / kWave);
 constexpr int kDotChunk = 8;
 constexpr int kTileGroup = 3;  // tiles whose dot loads are issued together
+// LDS of the FP64 dot phase's transposed block reduction: kDotChunk dd per
+// thread.  It follows the kArnRedLen doubles of red and may share its bytes
+// with a kernel's phase-1 LDS (a CSR product tile, a Kronecker table): the
+// dot phase starts with a barrier after the last phase-1 access.
+constexpr int kArnChunkLen = 2 * kDotChunk * kBlock;
+static_assert(kBlock % kDotChunk == 0 && kBlock / kDotChunk <= kWave &&
+                  ((kBlock / kDotChunk) & (kBlock / kDotChunk - 1)) == 0,
+              "a dot's segment must be a power-of-two part of one wave");
+
+// Dynamic LDS bytes of an Arnoldi row kernel with `extra` bytes of phase-1
+// LDS after red.
+inline size_t arn_lds_bytes(size_t extra)
+{
+    return sizeof(double) * kArnRedLen + std::max(extra, sizeof(double) * kArnChunkLen);
+}
 
 template <typename B>
 __device__ inline bool arn_skip(const ArnArgs &a)
@@ -133,13 +148,13 @@ __device__ inline void arn_epi(const ArnArgs &a, uint32_t i, double y, double xi
 
 // Merges the per-wave dd sums stored in red (slot k of wave w at
 // 2 (k NW + w)) in wave order and writes the block's partials of slots
-// 0..last: slot k of block b at 2 (k G + b).
-__device__ inline void arn_store_partials(const ArnArgs &a, int last, const double *red)
+// first..last: slot k of block b at 2 (k G + b).
+__device__ inline void arn_store_partials(const ArnArgs &a, int last, const double *red, int first = 0)
 {
     constexpr int NW = kBlock / kWave;
     __syncthreads();
-    if ((int)threadIdx.x <= last) {
-        const int kk = threadIdx.x;
+    if ((int)threadIdx.x <= last - first) {
+        const int kk = first + (int)threadIdx.x;
         dd s = dd_zero();
 #pragma unroll
         for (int q = 0; q < NW; ++q) s = dd_merge(s, dd{red[2 * (kk * NW + q)], red[2 * (kk * NW + q) + 1]});
@@ -159,27 +174,57 @@ __device__ inline void arn_put_wave(double *red, int k, dd v)
     }
 }
 
+// Block partials of slots base + q, q = 0 .. kDotChunk - 1 (those with
+// c0 + q < nk) from kDotChunk per-thread dd sums.  The sums go to LDS as [dot][thread]; thread
+// t then merges dot t / S (S = kBlock / kDotChunk threads per dot) over the
+// threads t % S + S u, u = 0, 1, ... in that order, and a butterfly over its
+// S-lane part of the wave finishes the block sum: a fixed order, and
+// kDotChunk + log2 S dd merges per thread where a 64-lane butterfly per dot
+// and wave would take 6 kDotChunk.
+__device__ inline void arn_chunk_partials(const ArnArgs &a, int c0, int nk, int base,
+                                          const dd (&acc)[kDotChunk], double *buf)
+{
+    constexpr int S = kBlock / kDotChunk;
+    __syncthreads();  // earlier reads of buf (or phase-1 LDS) are done
+#pragma unroll
+    for (int q = 0; q < kDotChunk; ++q) {
+        buf[2 * (q * kBlock + threadIdx.x)] = acc[q].s;
+        buf[2 * (q * kBlock + threadIdx.x) + 1] = acc[q].c;
+    }
+    __syncthreads();
+    const int q = threadIdx.x / S, sg = threadIdx.x % S;
+    dd v = dd_zero();
+#pragma unroll
+    for (int u = 0; u < kBlock / S; ++u) {
+        const int t = q * kBlock + sg + S * u;
+        v = dd_merge(v, dd{buf[2 * t], buf[2 * t + 1]});
+    }
+#pragma unroll
+    for (int off = S / 2; off > 0; off >>= 1) {
+        dd o;
+        o.s = __shfl_xor(v.s, off, kWave);
+        o.c = __shfl_xor(v.c, off, kWave);
+        v = dd_merge(v, o);
+    }
+    if (sg == 0 && c0 + q < nk) {
+        a.partials[2 * ((size_t)(base + q) * a.G + blockIdx.x)] = v.s;
+        a.partials[2 * ((size_t)(base + q) * a.G + blockIdx.x) + 1] = v.c;
+    }
+}
+
 // Phase 2 and the block reduction.  Mode 0: the dots V_k . w_{j+1}
 // (k = istart..j) over this block's rows, kDotChunk dots at a time: each
 // thread re-reads its own w_{j+1}[i] (written in phase 1, so cached) and a
 // chunk of independent basis loads (column index clamped to j, always
 // valid), so few registers are live and a chunk costs one memory round trip.
-// Every wave reduces each dot by butterflies into red (kArnRedLen doubles);
-// one barrier, then the block partials.
+// Each chunk's block partials come from the transposed LDS reduction
+// (arn_chunk_partials, kArnChunkLen doubles after red); the sink sum (and
+// mode 1's norm) is reduced by wave butterflies into red (kArnRedLen
+// doubles), then one barrier and its block partial.
 // With an FP32 basis the same pass also forms the Gram dots V_k . V_j
-// (slots nk+1..2nk) from the same basis loads, both in plain FP64 (a fixed
-// summation tree, so still deterministic; no bit-exact oracle match is
-// asked of the FP32 path).
-__device__ inline void arn_put_wave_f(double *red, int k, double v)
-{
-    constexpr int NW = kBlock / kWave;
-    const double t = wave_sum(v);
-    if (threadIdx.x % kWave == 0) {
-        const int w = threadIdx.x / kWave;
-        red[2 * (k * NW + w)] = t;
-        red[2 * (k * NW + w) + 1] = 0.0;
-    }
-}
+// (slots nk+1..2nk) from the same basis loads; each thread sums both in
+// plain FP64 (a fixed order, so still deterministic; no bit-exact oracle
+// match is asked of the FP32 path).
 
 template <typename B>
 __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red)
@@ -215,9 +260,7 @@ __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red
                         if (c0 + q < nk) acc[q] = dd_add_prod(acc[q], y[g], vk[g][q]);
                 }
             }
-#pragma unroll
-            for (int q = 0; q < kDotChunk; ++q)
-                if (c0 + q < nk) arn_put_wave(red, c0 + q, acc[q]);
+            arn_chunk_partials(a, c0, nk, c0, acc, red + kArnRedLen);
         }
     } else if (a.mode == 0) {
         for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
@@ -249,19 +292,21 @@ __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red
                     }
                 }
             }
+            dd da[kDotChunk], dg[kDotChunk];
 #pragma unroll
             for (int q = 0; q < kDotChunk; ++q) {
-                if (c0 + q < nk) {
-                    arn_put_wave_f(red, c0 + q, acc[q]);
-                    arn_put_wave_f(red, nk + 1 + c0 + q, accg[q]);
-                }
+                da[q] = dd{acc[q], 0.0};
+                dg[q] = dd{accg[q], 0.0};
             }
+            arn_chunk_partials(a, c0, nk, c0, da, red + kArnRedLen);
+            arn_chunk_partials(a, c0, nk, nk + 1 + c0, dg, red + kArnRedLen);
         }
     } else {
         arn_put_wave(red, 0, nrm);
     }
+    // the sink sum at slot nk (and mode 1's norm at slot 0)
     arn_put_wave(red, nk, sink);
-    arn_store_partials(a, (GRAM && a.mode == 0) ? 2 * nk : nk, red);
+    arn_store_partials(a, nk, red, a.mode == 0 ? nk : 0);
 }
 
 // Register-accumulator variant for the callback path (k_cb_dots), which
@@ -916,15 +961,15 @@ template <typename B>
 static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
 {
     MatImpl &M = *E.A;
-    const size_t red_b = sizeof(double) * kArnRedLen;
+    const size_t red_b = arn_lds_bytes(0);
     const bool mixed = M.prec == FSP_MIXED;
     if (M.format == FSP_CSR) {
         if (mixed) {
             CsrF<float> A = csr_view<float>(M);
-            k_arn_csr<float, B><<<E.G, kBlock, red_b + sizeof(double) * A.cap, E.s>>>(A, a);
+            k_arn_csr<float, B><<<E.G, kBlock, arn_lds_bytes(sizeof(double) * A.cap), E.s>>>(A, a);
         } else {
             CsrF<double> A = csr_view<double>(M);
-            k_arn_csr<double, B><<<E.G, kBlock, red_b + sizeof(double) * A.cap, E.s>>>(A, a);
+            k_arn_csr<double, B><<<E.G, kBlock, arn_lds_bytes(sizeof(double) * A.cap), E.s>>>(A, a);
         }
     } else if (M.format == FSP_HYB) {
         if (mixed)
@@ -938,19 +983,19 @@ static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
         const bool slots = M.kron_slots && kron_slot_plan(M.hm, SP);
         if (slots && mixed) {
             KronF<float> A = kron_view<float>(M, -1, 0.0);
-            const size_t lds = red_b + sizeof(float) * (size_t)SP.tab_size;
+            const size_t lds = arn_lds_bytes(sizeof(float) * (size_t)SP.tab_size);
             FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron_slot<D_, float, B><<<E.G, kBlock, lds, E.s>>>(
                                         A, SP, M.d_slot_tab_f, a)));
         } else if (slots) {
             KronF<double> A = kron_view<double>(M, -1, 0.0);
-            const size_t lds = red_b + sizeof(double) * (size_t)SP.tab_size;
+            const size_t lds = arn_lds_bytes(sizeof(double) * (size_t)SP.tab_size);
             FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron_slot<D_, double, B><<<E.G, kBlock, lds, E.s>>>(
                                         A, SP, M.d_slot_tab, a)));
         } else if (mixed && M.hm.has_factors) {
             KronF<float> A = kron_view<float>(M, -1, 0.0);
             const size_t tb = sizeof(float) * (size_t)A.tab_size;
             const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
-            const size_t lds = red_b + (use_lds ? tb : 0);
+            const size_t lds = arn_lds_bytes(use_lds ? tb : 0);
             if (plan0 && use_lds) {
                 FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron_plan<D_, float, B><<<E.G, kBlock, lds, E.s>>>(A, P, a)));
             } else {
@@ -960,7 +1005,7 @@ static fsp_status launch_arn_spmv(ExpvImpl &E, ArnArgs &a)
             KronF<double> A = kron_view<double>(M, -1, 0.0);
             const size_t tb = sizeof(double) * (size_t)A.tab_size;
             const int use_lds = (A.tab_g && tb <= kLdsTableMax) ? 1 : 0;
-            const size_t lds = red_b + (use_lds ? tb : 0);
+            const size_t lds = arn_lds_bytes(use_lds ? tb : 0);
             if (plan0 && use_lds) {
                 FSP_SWITCH_D(M.hm.dm.d, (k_arn_kron_plan<D_, double, B><<<E.G, kBlock, lds, E.s>>>(A, P, a)));
             } else {
