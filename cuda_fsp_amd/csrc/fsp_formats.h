@@ -118,20 +118,42 @@ __device__ inline uint32_t hyb_col(uint32_t i, int off, uint32_t n)
 // (non-temporal: streamed once per product), with all gathers of x at the
 // implicit columns, then the sum in slot order, so a row costs one memory
 // round trip whatever U is.
+// The loads of one such row (hyb_load) and its sum (hyb_sum), apart so that
+// a kernel can issue a later row's loads before an earlier row's sum.
+template <int NU, typename V>
+struct HybLoads {
+    double d, xi;
+    V v[NU];
+    double xc[NU];
+};
+
+template <int NU, typename V, class X>
+__device__ inline void hyb_load(const HybF<V> &A, uint32_t i, const X &x, HybLoads<NU, V> &L)
+{
+    L.xi = x(i);
+    L.d = __builtin_nontemporal_load(A.diag + i);
+#pragma unroll
+    for (int u = 0; u < NU; ++u) L.v[u] = __builtin_nontemporal_load(A.slot_vals + (size_t)u * A.n + i);
+#pragma unroll
+    for (int u = 0; u < NU; ++u) L.xc[u] = x(hyb_col(i, A.slot_off[u], A.n));
+}
+
+template <int NU, typename V>
+__device__ inline double hyb_sum(const HybLoads<NU, V> &L)
+{
+    double y = L.d * L.xi;
+#pragma unroll
+    for (int u = 0; u < NU; ++u) y += L.v[u] != (V)0 ? (double)L.v[u] * L.xc[u] : 0.0;
+    return y;
+}
+
 template <int NU, typename V, class X>
 __device__ inline double hyb_row_n(const HybF<V> &A, uint32_t i, const X &x, dd &sink)
 {
-    const double xi = x(i);
-    double y = __builtin_nontemporal_load(A.diag + i) * xi;
-    V v[NU];
-#pragma unroll
-    for (int u = 0; u < NU; ++u) v[u] = __builtin_nontemporal_load(A.slot_vals + (size_t)u * A.n + i);
-    double xc[NU];
-#pragma unroll
-    for (int u = 0; u < NU; ++u) xc[u] = x(hyb_col(i, A.slot_off[u], A.n));
-#pragma unroll
-    for (int u = 0; u < NU; ++u) y += v[u] != (V)0 ? (double)v[u] * xc[u] : 0.0;
-    if (A.sink_out) sink = dd_add_prod(sink, A.sink_out[i], xi);
+    HybLoads<NU, V> L;
+    hyb_load<NU>(A, i, x, L);
+    const double y = hyb_sum<NU>(L);
+    if (A.sink_out) sink = dd_add_prod(sink, A.sink_out[i], L.xi);
     return y;
 }
 
