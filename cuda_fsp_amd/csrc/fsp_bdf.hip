@@ -159,15 +159,17 @@ __global__ void k_gm_post(const double *__restrict__ x, const double *__restrict
 }
 
 // Partial sums over the rows of block b of h_k = V_k . w (slot k, double-
-// double) and of the Gram column V_k . V_j (slot nk + k, plain FP64 in a
-// fixed order: its entries are of rounding size off the diagonal and only
-// correct the coefficients at that size), k < nk = j + 1, from the same
-// basis loads: 8 dots per pass over the rows.
-__global__ void k_gm_dots(const double *__restrict__ V, int64_t ldv, int nk, const double *__restrict__ w,
-                          int64_t n, double *partials, const GmState *gm)
+// double) and of the Gram column V_k . V_j (slot nk + k: each thread's sum in
+// plain FP64 in a fixed order, as its entries are of rounding size off the
+// diagonal and only correct the coefficients at that size), k < nk = j + 1,
+// from the same basis loads: 8 dots per pass over the rows, each pass's
+// block sums by the transposed LDS reduction (block_sums_dd_t).
+__global__ void __launch_bounds__(kBlock) k_gm_dots(const double *__restrict__ V, int64_t ldv, int nk,
+                                                    const double *__restrict__ w, int64_t n, double *partials,
+                                                    const GmState *gm)
 {
     if (gm->done) return;
-    __shared__ double red[2 * kBlock / kWave];
+    __shared__ double buf[2 * kDotChunk * kBlock];
     const double *vj = V + (int64_t)(nk - 1) * ldv;
     for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
         dd s[kDotChunk];
@@ -189,21 +191,21 @@ __global__ void k_gm_dots(const double *__restrict__ V, int64_t ldv, int nk, con
                 q2[q] = fma(v[q], xj, q2[q]);
             }
         }
-#pragma unroll
-        for (int q = 0; q < kDotChunk; ++q) {
-            if (c0 + q >= nk) break;
-            const dd t = block_sum_dd(s[q], red);
-            const double u = block_sum(q2[q], red);
-            if (threadIdx.x == 0) {
-                const int64_t o = (int64_t)(c0 + q) * gridDim.x + blockIdx.x;
-                const int64_t og = (int64_t)(nk + c0 + q) * gridDim.x + blockIdx.x;
-                partials[2 * o] = t.s;
-                partials[2 * o + 1] = t.c;
-                partials[2 * og] = u;
-                partials[2 * og + 1] = 0.0;
-            }
+        int q;
+        dd t;
+        if (block_sums_dd_t<kDotChunk>(s, buf, q, t) && c0 + q < nk) {
+            const int64_t o = (int64_t)(c0 + q) * gridDim.x + blockIdx.x;
+            partials[2 * o] = t.s;
+            partials[2 * o + 1] = t.c;
         }
-        __syncthreads();
+        dd g[kDotChunk];
+#pragma unroll
+        for (int k = 0; k < kDotChunk; ++k) g[k] = dd{q2[k], 0.0};
+        if (block_sums_dd_t<kDotChunk>(g, buf, q, t) && c0 + q < nk) {
+            const int64_t og = (int64_t)(nk + c0 + q) * gridDim.x + blockIdx.x;
+            partials[2 * og] = t.s;
+            partials[2 * og + 1] = t.c;
+        }
     }
 }
 

@@ -129,6 +129,48 @@ __device__ inline dd block_sum_dd(dd v, double *red)
     return s;
 }
 
+// Block sums of CH dd values per thread at once (blockDim.x == kBlock),
+// transposed through LDS: the values go to buf (2 CH kBlock doubles) as
+// [value][thread]; thread x then merges value q = x / S (S = kBlock / CH
+// threads per value) over the threads x % S + S u, u = 0, 1, ... in that
+// order, and a butterfly over its S lanes finishes the sum.  Returns true in
+// the one thread per value (x % S == 0) that holds it, with q and the sum.
+// A fixed order, and CH + log2 S dd merges per thread where a 64-lane
+// butterfly per value takes 6 CH.  Starts with a barrier, so earlier reads
+// of buf (or other uses of its bytes) are done; the caller reuses buf only
+// after another barrier.
+template <int CH>
+__device__ inline bool block_sums_dd_t(const dd (&v)[CH], double *buf, int &q, dd &sum)
+{
+    constexpr int S = kBlock / CH;
+    static_assert(kBlock % CH == 0 && S <= kWave && (S & (S - 1)) == 0,
+                  "a value's threads must be a power-of-two part of one wave");
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+        buf[2 * (k * kBlock + threadIdx.x)] = v[k].s;
+        buf[2 * (k * kBlock + threadIdx.x) + 1] = v[k].c;
+    }
+    __syncthreads();
+    q = threadIdx.x / S;
+    const int sg = threadIdx.x % S;
+    dd t = dd_zero();
+#pragma unroll
+    for (int u = 0; u < kBlock / S; ++u) {
+        const int x = q * kBlock + sg + S * u;
+        t = dd_merge(t, dd{buf[2 * x], buf[2 * x + 1]});
+    }
+#pragma unroll
+    for (int off = S / 2; off > 0; off >>= 1) {
+        dd o;
+        o.s = __shfl_xor(t.s, off, kWave);
+        o.c = __shfl_xor(t.c, off, kWave);
+        t = dd_merge(t, o);
+    }
+    sum = t;
+    return sg == 0;
+}
+
 // Sum of `count` dd partials stored as (s, c) pairs, fixed order.
 __device__ inline dd sum_dd_partials(const double *__restrict__ p, int count, double *red)
 {

@@ -113,9 +113,6 @@ constexpr int kTileGroup = 3;  // tiles whose dot loads are issued together
 // with a kernel's phase-1 LDS (a CSR product tile, a Kronecker table): the
 // dot phase starts with a barrier after the last phase-1 access.
 constexpr int kArnChunkLen = 2 * kDotChunk * kBlock;
-static_assert(kBlock % kDotChunk == 0 && kBlock / kDotChunk <= kWave &&
-                  ((kBlock / kDotChunk) & (kBlock / kDotChunk - 1)) == 0,
-              "a dot's segment must be a power-of-two part of one wave");
 
 // Dynamic LDS bytes of an Arnoldi row kernel with `extra` bytes of phase-1
 // LDS after red.
@@ -175,38 +172,14 @@ __device__ inline void arn_put_wave(double *red, int k, dd v)
 }
 
 // Block partials of slots base + q, q = 0 .. kDotChunk - 1 (those with
-// c0 + q < nk) from kDotChunk per-thread dd sums.  The sums go to LDS as [dot][thread]; thread
-// t then merges dot t / S (S = kBlock / kDotChunk threads per dot) over the
-// threads t % S + S u, u = 0, 1, ... in that order, and a butterfly over its
-// S-lane part of the wave finishes the block sum: a fixed order, and
-// kDotChunk + log2 S dd merges per thread where a 64-lane butterfly per dot
-// and wave would take 6 kDotChunk.
+// c0 + q < nk) from kDotChunk per-thread dd sums, by the transposed block
+// reduction (block_sums_dd_t; buf: kArnChunkLen doubles).
 __device__ inline void arn_chunk_partials(const ArnArgs &a, int c0, int nk, int base,
                                           const dd (&acc)[kDotChunk], double *buf)
 {
-    constexpr int S = kBlock / kDotChunk;
-    __syncthreads();  // earlier reads of buf (or phase-1 LDS) are done
-#pragma unroll
-    for (int q = 0; q < kDotChunk; ++q) {
-        buf[2 * (q * kBlock + threadIdx.x)] = acc[q].s;
-        buf[2 * (q * kBlock + threadIdx.x) + 1] = acc[q].c;
-    }
-    __syncthreads();
-    const int q = threadIdx.x / S, sg = threadIdx.x % S;
-    dd v = dd_zero();
-#pragma unroll
-    for (int u = 0; u < kBlock / S; ++u) {
-        const int t = q * kBlock + sg + S * u;
-        v = dd_merge(v, dd{buf[2 * t], buf[2 * t + 1]});
-    }
-#pragma unroll
-    for (int off = S / 2; off > 0; off >>= 1) {
-        dd o;
-        o.s = __shfl_xor(v.s, off, kWave);
-        o.c = __shfl_xor(v.c, off, kWave);
-        v = dd_merge(v, o);
-    }
-    if (sg == 0 && c0 + q < nk) {
+    int q;
+    dd v;
+    if (block_sums_dd_t<kDotChunk>(acc, buf, q, v) && c0 + q < nk) {
         a.partials[2 * ((size_t)(base + q) * a.G + blockIdx.x)] = v.s;
         a.partials[2 * ((size_t)(base + q) * a.G + blockIdx.x) + 1] = v.c;
     }
