@@ -138,14 +138,32 @@ __device__ inline void hyb_load(const HybF<V> &A, uint32_t i, const X &x, HybLoa
     for (int u = 0; u < NU; ++u) L.xc[u] = x(hyb_col(i, A.slot_off[u], A.n));
 }
 
+// hyb_sum of loads whose x entries are still to be mapped by xs (e.g. the
+// Krylov scaling, applied to raw loads issued before the scale was known).
+template <int NU, typename V, class XS>
+__device__ inline double hyb_sum_x(const HybLoads<NU, V> &L, const XS &xs)
+{
+    double y = L.d * xs(L.xi);
+#pragma unroll
+    for (int u = 0; u < NU; ++u) y += L.v[u] != (V)0 ? (double)L.v[u] * xs(L.xc[u]) : 0.0;
+    return y;
+}
+
+struct XIdentity {
+    __device__ double operator()(double v) const { return v; }
+};
+
 template <int NU, typename V>
 __device__ inline double hyb_sum(const HybLoads<NU, V> &L)
 {
-    double y = L.d * L.xi;
-#pragma unroll
-    for (int u = 0; u < NU; ++u) y += L.v[u] != (V)0 ? (double)L.v[u] * L.xc[u] : 0.0;
-    return y;
+    return hyb_sum_x<NU>(L, XIdentity{});
 }
+
+// Raw entries of a vector (for hyb_load before a scale is known).
+struct RawX {
+    const double *__restrict__ in;
+    __device__ double operator()(uint32_t c) const { return in[c]; }
+};
 
 template <int NU, typename V, class X>
 __device__ inline double hyb_row_n(const HybF<V> &A, uint32_t i, const X &x, dd &sink)

@@ -366,40 +366,45 @@ __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
     if (arn_skip<B>(a)) return;
     FSP_DYNAMIC_LDS(smem);
     double *red = smem;
+    constexpr int K = NU > 0 ? NU : 1;
+    // the pipeline holds two rows' loads: above 8 slots that would cost
+    // occupancy (registers), so one tile at a time there
+    constexpr bool kPipe = NU > 0 && K <= 8;
+    const TileWalk w = tile_walk(a.tiles);
+    // rows past the box read row 0 and are discarded
+    auto row = [&](uint32_t t) {
+        const uint32_t i = t * kBlock + threadIdx.x;
+        return i < A.n ? i : 0u;
+    };
+    // the first tile's loads go out before the scale sigma_j is known (its
+    // x entries raw, scaled at the sum by the same kry_scale), so that they
+    // overlap the norm prologue's loads of the update's partials
+    HybLoads<K, V> cur, nxt;
+    if (kPipe && w.t < w.end) hyb_load<K>(A, row(w.t), RawX{a.in}, cur);
     double sig;
     if (!arn_sigma(a, red, sig)) return;
     dd nrm = dd_zero();
     const KryX<B> X{a.in, sig, a.j};
+    const int jj = a.j;
+    auto xs = [sig, jj](double v) { return kry_scale<B>(v, sig, jj); };
     dd sink = dd_zero();
     if (NU > 0) {
-        constexpr int K = NU > 0 ? NU : 1;
-        const TileWalk w = tile_walk(a.tiles);
-        // rows past the box read row 0 and are discarded
-        auto row = [&](uint32_t t) {
-            const uint32_t i = t * kBlock + threadIdx.x;
-            return i < A.n ? i : 0u;
-        };
-        // the pipeline holds two rows' loads: above 8 slots that would cost
-        // occupancy (registers), so one tile at a time there
-        constexpr bool kPipe = K <= 8;
-        HybLoads<K, V> cur, nxt;
-        if (kPipe && w.t < w.end) hyb_load<K>(A, row(w.t), X, cur);
         for (uint32_t t = w.t; t < w.end; t += w.step) {
             const uint32_t tn = t + w.step;
             if (!kPipe)
-                hyb_load<K>(A, row(t), X, cur);
+                hyb_load<K>(A, row(t), RawX{a.in}, cur);
             else if (tn < w.end)
-                hyb_load<K>(A, row(tn), X, nxt);
+                hyb_load<K>(A, row(tn), RawX{a.in}, nxt);
             const uint32_t i = t * kBlock + threadIdx.x;
             const bool in = i < A.n;
-            const double y = hyb_sum<K>(cur);
+            const double y = hyb_sum_x<K>(cur, xs);
             if (A.sink_out && in) sink = dd_add_prod(sink, A.sink_out[i], X(i));
             arn_epi<B>(a, i, in ? y : 0.0, i < a.n ? X(i) : 0.0, nrm);
             if (kPipe) cur = nxt;
         }
     } else {
-        for (TileWalk w = tile_walk(a.tiles); w.t < w.end; w.t += w.step) {
-            const uint32_t i = w.t * kBlock + threadIdx.x;
+        for (TileWalk v = w; v.t < v.end; v.t += v.step) {
+            const uint32_t i = v.t * kBlock + threadIdx.x;
             const double y = hyb_row<V>(A, i, X, sink);
             arn_epi<B>(a, i, y, i < a.n ? X(i) : 0.0, nrm);
         }
