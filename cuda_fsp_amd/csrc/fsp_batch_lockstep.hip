@@ -463,22 +463,7 @@ __global__ void __launch_bounds__(kBlock) k_ls_update(LsArgs a)
 #pragma unroll
                 for (int q = 0; q < kLsChunk * 2; ++q) acc += v[q] * h[k + q];
             }
-            if (k <= a.j) {
-                // the last vectors in one round trip: column index clamped to
-                // j (always valid), and past j the term is +0 * +0, which
-                // leaves acc's bits unchanged (acc starts at +0 and never
-                // holds -0 in round-to-nearest); a select, not a branch, so
-                // the loads are issued together
-                double v[kLsChunk * 2];
-#pragma unroll
-                for (int q = 0; q < kLsChunk * 2; ++q)
-                    v[q] = basis<B>(I.V64, Vb, (size_t)I.nv, min(k + q, a.j), r);
-#pragma unroll
-                for (int q = 0; q < kLsChunk * 2; ++q) {
-                    const bool in = k + q <= a.j;
-                    acc += (in ? v[q] : 0.0) * (in ? h[min(k + q, a.j)] : 0.0);
-                }
-            }
+            for (; k <= a.j; ++k) acc += basis<B>(I.V64, Vb, (size_t)I.nv, k, r) * h[k];
             const double w = -1.0 * acc + out[r];
             out[r] = w;
             nrm = dd_add_prod(nrm, w, w);
