@@ -199,13 +199,15 @@ __device__ inline void arn_chunk_partials(const ArnArgs &a, int c0, int nk, int 
 // plain FP64 (a fixed order, so still deterministic; no bit-exact oracle
 // match is asked of the FP32 path).
 
+// c_begin: the first chunk of FP64 dots still to form (a kernel that formed
+// the first chunk with its rows passes kDotChunk).
 template <typename B>
-__device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red)
+__device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red, int c_begin = 0)
 {
     constexpr bool GRAM = gram_basis<B>();
     const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
     if (a.mode == 0 && !GRAM) {
-        for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
+        for (int c0 = c_begin; c0 < nk; c0 += kDotChunk) {
             dd acc[kDotChunk];
 #pragma unroll
             for (int q = 0; q < kDotChunk; ++q) acc[q] = dd_zero();
@@ -388,6 +390,15 @@ __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
     const int jj = a.j;
     auto xs = [sig, jj](double v) { return kry_scale<B>(v, sig, jj); };
     dd sink = dd_zero();
+    // FP64 basis, dots mode: the first chunk of dots is formed tile by tile
+    // with the rows (w_{j+1} and V_j from registers, the basis loads of the
+    // vectors below j issued with the next tile's generator loads), in the
+    // same per-thread order as arn_finish's pass
+    const bool fuse0 = kPipe && !gram_basis<B>() && a.mode == 0;
+    const int nk = a.j - a.istart + 1;
+    dd acc0[kDotChunk];
+#pragma unroll
+    for (int q = 0; q < kDotChunk; ++q) acc0[q] = dd_zero();
     if (NU > 0) {
         for (uint32_t t = w.t; t < w.end; t += w.step) {
             const uint32_t tn = t + w.step;
@@ -397,9 +408,24 @@ __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
                 hyb_load<K>(A, row(tn), RawX{a.in}, nxt);
             const uint32_t i = t * kBlock + threadIdx.x;
             const bool in = i < A.n;
+            // vectors istart + q below j (any valid index otherwise: such a
+            // load is not used)
+            double vk[kDotChunk];
+            if (fuse0) {
+#pragma unroll
+                for (int q = 0; q < kDotChunk; ++q)
+                    vk[q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + q, max(a.j - 1, 0)),
+                                     row(t));
+            }
             const double y = hyb_sum_x<K>(cur, xs);
             if (A.sink_out && in) sink = dd_add_prod(sink, A.sink_out[i], X(i));
             arn_epi<B>(a, i, in ? y : 0.0, i < a.n ? X(i) : 0.0, nrm);
+            if (fuse0 && i < a.n) {
+                const double xj = X(i);
+#pragma unroll
+                for (int q = 0; q < kDotChunk; ++q)
+                    if (q < nk) acc0[q] = dd_add_prod(acc0[q], y, a.istart + q < a.j ? vk[q] : xj);
+            }
             if (kPipe) cur = nxt;
         }
     } else {
@@ -409,7 +435,8 @@ __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
             arn_epi<B>(a, i, y, i < a.n ? X(i) : 0.0, nrm);
         }
     }
-    arn_finish<B>(a, nrm, sink, red);
+    if (fuse0) arn_chunk_partials(a, 0, nk, 0, acc0, red + kArnRedLen);
+    arn_finish<B>(a, nrm, sink, red, fuse0 ? kDotChunk : 0);
 }
 
 template <int D, typename V, typename B>
