@@ -276,12 +276,19 @@ __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red
             arn_chunk_partials(a, c0, nk, c0, da, red + kArnRedLen);
             arn_chunk_partials(a, c0, nk, nk + 1 + c0, dg, red + kArnRedLen);
         }
+    }
+    // the sink sum at slot nk (mode 1: its norm at slot 0, the sink at 1);
+    // without a sink entry (nv == n) the finalizer reads no sink slot
+    const bool has_sink = a.nv > a.n;
+    if (a.mode == 0) {
+        if (!has_sink) return;
+        arn_put_wave(red, nk, sink);
+        arn_store_partials(a, nk, red, nk);
     } else {
         arn_put_wave(red, 0, nrm);
+        if (has_sink) arn_put_wave(red, 1, sink);
+        arn_store_partials(a, has_sink ? 1 : 0, red);
     }
-    // the sink sum at slot nk (and mode 1's norm at slot 0)
-    arn_put_wave(red, nk, sink);
-    arn_store_partials(a, nk, red, a.mode == 0 ? nk : 0);
 }
 
 // Register-accumulator variant for the callback path (k_cb_dots), which
