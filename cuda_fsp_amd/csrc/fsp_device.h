@@ -171,6 +171,45 @@ __device__ inline bool block_sums_dd_t(const dd (&v)[CH], double *buf, int &q, d
     return sg == 0;
 }
 
+// Wave sums of CH dd values per lane at once, transposed through the wave's
+// own LDS (wbuf: 2 CH kWave doubles, used by this wave only): lane l merges
+// value q = l / S (S = kWave / CH lanes per value) over the lanes l % S + S u,
+// u = 0, 1, ... in that order, and a butterfly over its S lanes finishes the
+// wave's sum.  Returns true in the one lane per value (l % S == 0) that holds
+// it, with q and the sum.  A fixed order, CH + log2 S dd merges per lane, and
+// only wave-level synchronisation (no block barrier).
+template <int CH>
+__device__ inline bool wave_sums_dd_t(const dd (&v)[CH], double *wbuf, int &q, dd &sum)
+{
+    constexpr int S = kWave / CH;
+    static_assert(kWave % CH == 0 && (S & (S - 1)) == 0, "a value's lanes must be a power of two");
+    const int l = threadIdx.x % kWave;
+    wave_sync();  // the wave's earlier reads of wbuf are done
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+        wbuf[2 * (k * kWave + l)] = v[k].s;
+        wbuf[2 * (k * kWave + l) + 1] = v[k].c;
+    }
+    wave_sync();
+    q = l / S;
+    const int sg = l % S;
+    dd t = dd_zero();
+#pragma unroll
+    for (int u = 0; u < kWave / S; ++u) {
+        const int x = q * kWave + sg + S * u;
+        t = dd_merge(t, dd{wbuf[2 * x], wbuf[2 * x + 1]});
+    }
+#pragma unroll
+    for (int off = S / 2; off > 0; off >>= 1) {
+        dd o;
+        o.s = __shfl_xor(t.s, off, kWave);
+        o.c = __shfl_xor(t.c, off, kWave);
+        t = dd_merge(t, o);
+    }
+    sum = t;
+    return sg == 0;
+}
+
 // Sum of `count` dd partials stored as (s, c) pairs, fixed order.
 __device__ inline dd sum_dd_partials(const double *__restrict__ p, int count, double *red)
 {

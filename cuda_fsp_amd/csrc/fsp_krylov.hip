@@ -171,17 +171,21 @@ __device__ inline void arn_put_wave(double *red, int k, dd v)
     }
 }
 
-// Block partials of slots base + q, q = 0 .. kDotChunk - 1 (those with
-// c0 + q < nk) from kDotChunk per-thread dd sums, by the transposed block
-// reduction (block_sums_dd_t; buf: kArnChunkLen doubles).
-__device__ inline void arn_chunk_partials(const ArnArgs &a, int c0, int nk, int base,
-                                          const dd (&acc)[kDotChunk], double *buf)
+// Wave sums of slots base + q, q = 0 .. kDotChunk - 1 (those with
+// c0 + q < nk) from kDotChunk per-lane dd sums, by the transposed wave
+// reduction (wave_sums_dd_t, through the wave's own kDotChunk kWave dd of
+// LDS after red), into red in the arn_put_wave layout; arn_store_partials
+// merges the waves once after the last slot.
+__device__ inline void arn_chunk_waves(int c0, int nk, int base, const dd (&acc)[kDotChunk], double *red)
 {
+    constexpr int NW = kBlock / kWave;
+    const int w = threadIdx.x / kWave;
+    double *wbuf = red + kArnRedLen + (size_t)w * (2 * kDotChunk * kWave);
     int q;
     dd v;
-    if (block_sums_dd_t<kDotChunk>(acc, buf, q, v) && c0 + q < nk) {
-        a.partials[2 * ((size_t)(base + q) * a.G + blockIdx.x)] = v.s;
-        a.partials[2 * ((size_t)(base + q) * a.G + blockIdx.x) + 1] = v.c;
+    if (wave_sums_dd_t<kDotChunk>(acc, wbuf, q, v) && c0 + q < nk) {
+        red[2 * ((base + q) * NW + w)] = v.s;
+        red[2 * ((base + q) * NW + w) + 1] = v.c;
     }
 }
 
@@ -190,10 +194,10 @@ __device__ inline void arn_chunk_partials(const ArnArgs &a, int c0, int nk, int 
 // thread re-reads its own w_{j+1}[i] (written in phase 1, so cached) and a
 // chunk of independent basis loads (column index clamped to j, always
 // valid), so few registers are live and a chunk costs one memory round trip.
-// Each chunk's block partials come from the transposed LDS reduction
-// (arn_chunk_partials, kArnChunkLen doubles after red); the sink sum (and
-// mode 1's norm) is reduced by wave butterflies into red (kArnRedLen
-// doubles), then one barrier and its block partial.
+// Each chunk's wave sums come from the transposed wave reduction
+// (arn_chunk_waves, kArnChunkLen doubles of LDS after red, a slice per
+// wave) and the sink sum (and mode 1's norm) from wave butterflies, all into
+// red (kArnRedLen doubles); then one barrier and the block partials.
 // With an FP32 basis the same pass also forms the Gram dots V_k . V_j
 // (slots nk+1..2nk) from the same basis loads; each thread sums both in
 // plain FP64 (a fixed order, so still deterministic; no bit-exact oracle
@@ -206,6 +210,9 @@ __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red
 {
     constexpr bool GRAM = gram_basis<B>();
     const int nk = a.mode == 0 ? a.j - a.istart + 1 : 1;
+    // the chunk slices may share bytes with a kernel's phase-1 LDS (a CSR
+    // product tile, a Kronecker table) that other waves may still read
+    if (a.mode == 0 && c_begin < nk) __syncthreads();
     if (a.mode == 0 && !GRAM) {
         for (int c0 = c_begin; c0 < nk; c0 += kDotChunk) {
             dd acc[kDotChunk];
@@ -223,9 +230,12 @@ __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red
                     ok[g] = tt < t.end && i < a.n;
                     const uint32_t r = ok[g] ? i : 0u;
                     y[g] = a.out[r];
+                    // only the chunk's vectors (a uniform test, so the loads
+                    // still go out together): padding loads of a short last
+                    // chunk cost L1/L2 bandwidth
 #pragma unroll
                     for (int q = 0; q < kDotChunk; ++q)
-                        vk[g][q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + c0 + q, a.j), r);
+                        vk[g][q] = c0 + q < nk ? basis<B>(a.V64, (const B *)a.Vb, a.nv, a.istart + c0 + q, r) : 0.0;
                 }
 #pragma unroll
                 for (int g = 0; g < kTileGroup; ++g) {
@@ -235,7 +245,7 @@ __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red
                         if (c0 + q < nk) acc[q] = dd_add_prod(acc[q], y[g], vk[g][q]);
                 }
             }
-            arn_chunk_partials(a, c0, nk, c0, acc, red + kArnRedLen);
+            arn_chunk_waves(c0, nk, c0, acc, red);
         }
     } else if (a.mode == 0) {
         for (int c0 = 0; c0 < nk; c0 += kDotChunk) {
@@ -253,6 +263,8 @@ __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red
                     const uint32_t r = ok[g] ? i : 0u;
                     y[g] = a.out[r];
                     xj[g] = basis<B>(a.V64, (const B *)a.Vb, a.nv, a.j, r);
+                    // padding loads past the chunk re-read V_j (a guard here
+                    // made the FP32-basis solve 15% slower)
 #pragma unroll
                     for (int q = 0; q < kDotChunk; ++q)
                         vk[g][q] = basis<B>(a.V64, (const B *)a.Vb, a.nv, min(a.istart + c0 + q, a.j), r);
@@ -273,17 +285,16 @@ __device__ inline void arn_finish(const ArnArgs &a, dd nrm, dd sink, double *red
                 da[q] = dd{acc[q], 0.0};
                 dg[q] = dd{accg[q], 0.0};
             }
-            arn_chunk_partials(a, c0, nk, c0, da, red + kArnRedLen);
-            arn_chunk_partials(a, c0, nk, nk + 1 + c0, dg, red + kArnRedLen);
+            arn_chunk_waves(c0, nk, c0, da, red);
+            arn_chunk_waves(c0, nk, nk + 1 + c0, dg, red);
         }
     }
     // the sink sum at slot nk (mode 1: its norm at slot 0, the sink at 1);
     // without a sink entry (nv == n) the finalizer reads no sink slot
     const bool has_sink = a.nv > a.n;
     if (a.mode == 0) {
-        if (!has_sink) return;
-        arn_put_wave(red, nk, sink);
-        arn_store_partials(a, nk, red, nk);
+        if (has_sink || GRAM) arn_put_wave(red, nk, sink);
+        arn_store_partials(a, GRAM ? 2 * nk : (has_sink ? nk : nk - 1), red);
     } else {
         arn_put_wave(red, 0, nrm);
         if (has_sink) arn_put_wave(red, 1, sink);
@@ -416,7 +427,8 @@ __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
             const uint32_t i = t * kBlock + threadIdx.x;
             const bool in = i < A.n;
             // vectors istart + q below j (any valid index otherwise: such a
-            // load is not used)
+            // load is not used; a guard here would cost a register and
+            // occupancy)
             double vk[kDotChunk];
             if (fuse0) {
 #pragma unroll
@@ -442,7 +454,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_hyb(HybF<V> A, ArnArgs a)
             arn_epi<B>(a, i, y, i < a.n ? X(i) : 0.0, nrm);
         }
     }
-    if (fuse0) arn_chunk_partials(a, 0, nk, 0, acc0, red + kArnRedLen);
+    if (fuse0) arn_chunk_waves(0, nk, 0, acc0, red);
     arn_finish<B>(a, nrm, sink, red, fuse0 ? kDotChunk : 0);
 }
 
