@@ -258,8 +258,11 @@ __device__ inline void ls_rows(const LsInst &I, int r0, const KryX<B> &X, double
     }
 }
 
+// At least 5 waves per SIMD (at most 96 VGPRs): the compiler's own choice,
+// 98 VGPRs at 4 rows per thread, held 4 waves and took the 512-instance
+// batch 28.9 s (FP64) / 29.0 s (mixed) against 28.6 / 27.5 s.
 template <typename B, int RPT>
-__global__ void __launch_bounds__(kBlock) k_ls_arn(LsArgs a)
+__global__ void __launch_bounds__(kBlock, 5) k_ls_arn(LsArgs a)
 {
     __shared__ double red[kLsRedLen];
     const int na = a.n_atiles[0];
