@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "fsp_device.h"
+#include "fsp_formats.h"
 
 namespace fsp {
 namespace {
@@ -156,6 +157,53 @@ __global__ void k_gm_post(const double *__restrict__ x, const double *__restrict
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         out[i] = alpha0 * (x[i] / (alpha0 - diag[i])) - ewt[i] * u[i];
+}
+
+// k_gm_post fused with the operator product for an FP64 HYB generator
+// without a sink: u_i = (A t)_i by the HYB row and, in the same pass,
+// w_i = alpha0 x_i / (alpha0 - A_ii) - ewt_i u_i with k_gm_post's
+// arithmetic (so w has the same bits), u never stored.
+template <int NU>
+__global__ void __launch_bounds__(kBlock) k_gm_apply_post(HybF<double> A, const double *__restrict__ t,
+                                                          const double *__restrict__ x,
+                                                          const double *__restrict__ ewt,
+                                                          const double *__restrict__ diag, double alpha0,
+                                                          double *__restrict__ out, uint32_t tiles,
+                                                          const GmState *gm)
+{
+    if (gm->done) return;
+    for (TileWalk w = tile_walk(tiles); w.t < w.end; w.t += w.step) {
+        const uint32_t i = w.t * kBlock + threadIdx.x;
+        if (i >= A.n) continue;
+        dd unused = dd_zero();
+        const double u = hyb_row_n<NU>(A, i, RawX{t}, unused);
+        out[i] = alpha0 * (x[i] / (alpha0 - diag[i])) - ewt[i] * u;
+    }
+}
+
+// Launches k_gm_apply_post when the generator allows it (HYB, FP64 values,
+// no sink, at most 12 slots); false: the caller takes the two-kernel path.
+static bool launch_gm_apply_post(const MatImpl &M, const double *t, const double *x, const double *ewt,
+                                 const double *diag, double alpha0, double *out, int64_t n,
+                                 const GmState *gm, hipStream_t s)
+{
+    if (M.format != FSP_HYB || M.prec != FSP_F64 || M.with_sink || M.transposed || n != M.n) return false;
+    const HybF<double> A = hyb_view<double>(M);
+    const uint32_t tiles = (uint32_t)((M.n + kBlock - 1) / kBlock);
+    const int grid = (int)tiles;  // one tile per workgroup, as the plain product
+#define FSP_GM_POST_CASE(K)                                                                   \
+    case K:                                                                                   \
+        hipLaunchKernelGGL(k_gm_apply_post<K>, dim3(grid), dim3(kBlock), 0, s, A, t, x, ewt, diag, \
+                           alpha0, out, tiles, gm);                                           \
+        return true;
+    switch (A.U) {
+        FSP_GM_POST_CASE(1) FSP_GM_POST_CASE(2) FSP_GM_POST_CASE(3) FSP_GM_POST_CASE(4)
+        FSP_GM_POST_CASE(5) FSP_GM_POST_CASE(6) FSP_GM_POST_CASE(7) FSP_GM_POST_CASE(8)
+        FSP_GM_POST_CASE(9) FSP_GM_POST_CASE(10) FSP_GM_POST_CASE(11) FSP_GM_POST_CASE(12)
+    default:
+        return false;
+    }
+#undef FSP_GM_POST_CASE
 }
 
 // Partial sums over the rows of block b of h_k = V_k . w (slot k, double-
@@ -492,9 +540,11 @@ fsp_status gmres(BdfImpl &B, double alpha0, double tol, double *y, bool *converg
             const int nk = j + 1;
             hipLaunchKernelGGL(k_gm_unscale, dim3(G), dim3(kBlock), 0, B.s, vj, B.d_gm, B.ewt, B.diag,
                                alpha0, B.tmp, n);
-            FSP_TRY(mat_apply(*B.A, B.tmp, B.u, 0, B.s));
-            hipLaunchKernelGGL(k_gm_post, dim3(G), dim3(kBlock), 0, B.s, vj, B.ewt, B.diag, B.u, alpha0,
-                               w, n, B.d_gm);
+            if (!launch_gm_apply_post(*B.A, B.tmp, vj, B.ewt, B.diag, alpha0, w, n, B.d_gm, B.s)) {
+                FSP_TRY(mat_apply(*B.A, B.tmp, B.u, 0, B.s));
+                hipLaunchKernelGGL(k_gm_post, dim3(G), dim3(kBlock), 0, B.s, vj, B.ewt, B.diag, B.u, alpha0,
+                                   w, n, B.d_gm);
+            }
             hipLaunchKernelGGL(k_gm_dots, dim3(kRedGrid), dim3(kBlock), 0, B.s, B.V, n, nk, w, n,
                                B.partials, B.d_gm);
             hipLaunchKernelGGL(k_reduce_partials, dim3(2 * nk), dim3(kBlock), 0, B.s, B.partials, kRedGrid,
