@@ -27,14 +27,17 @@ def build(force: bool = False) -> str:
             and os.path.getmtime(LIB) >= max(os.path.getmtime(p) for p in DEPS)):
         return LIB
     os.makedirs(OUT_DIR, exist_ok=True)
+    # a file of this process's own: test workers that rebuild at the same time
+    # each install a whole library (the rename is atomic)
+    tmp = f"{LIB}.{os.getpid()}.tmp"
     cmd = ["g++", "-std=c++17", "-O2", "-g", "-fPIC", "-shared",
            # fibers switch stacks with _setjmp/_longjmp
            "-U_FORTIFY_SOURCE", "-D_FORTIFY_SOURCE=0",
            "-ffp-contract=off", "-Wno-unused-function", "-Wno-unused-variable",
            "-I", HERE, "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-           "-x", "c++", *SOURCES, "-o", LIB + ".tmp"]
+           "-x", "c++", *SOURCES, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("simt build failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
-    os.replace(LIB + ".tmp", LIB)
+    os.replace(tmp, LIB)
     return LIB
