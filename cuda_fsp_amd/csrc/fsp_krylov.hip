@@ -618,20 +618,24 @@ __global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2)
                 st->H[(a.istart + k) + (size_t)a.j * M2] = cw[k];
         h = cw - a.istart;
     }
-    // kUpdRows rows per thread at a time (rows i + u G kBlock): the loads
-    // of all of them are issued before their sums, so a group of rows costs
-    // one memory round trip per 8 basis vectors instead of one per row; each
-    // row's sum still runs over k ascending
-    constexpr int kUpdRows = 3;
+    // the rows of kUpdRows tiles per thread at a time, walked as the Arnoldi
+    // row kernel walks them (so each XCD updates the rows whose w_{j+1} and
+    // V_j it just wrote): the loads of all of them are issued before their
+    // sums, so a group of rows costs one memory round trip per 8 basis
+    // vectors instead of one per row; each row's sum still runs over k
+    // ascending
+    constexpr int kUpdRows = kTileGroup;
     dd nrm = dd_zero();
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i0 = blockIdx.x * kBlock + threadIdx.x; i0 < a.nv; i0 += kUpdRows * stride) {
+    for (TileWalk tw = tile_walk(a.tiles); tw.t < tw.end; tw.t += kUpdRows * tw.step) {
         uint32_t r[kUpdRows];
+        bool ok[kUpdRows];
         double acc[kUpdRows], o[kUpdRows];
 #pragma unroll
         for (int u = 0; u < kUpdRows; ++u) {
-            const uint32_t iu = i0 + (uint32_t)u * stride;
-            r[u] = iu < a.nv ? iu : i0;
+            const uint32_t tt = tw.t + (uint32_t)u * tw.step;
+            const uint32_t i = tt * kBlock + threadIdx.x;
+            ok[u] = tt < tw.end && i < a.nv;
+            r[u] = ok[u] ? i : 0u;
             o[u] = a.out[r[u]];
             acc[u] = 0.0;
         }
@@ -656,7 +660,7 @@ __global__ void __launch_bounds__(kBlock) k_arn_update(ArnArgs a, int M2)
         }
 #pragma unroll
         for (int u = 0; u < kUpdRows; ++u) {
-            if (i0 + (uint32_t)u * stride >= a.nv) break;
+            if (!ok[u]) continue;
             const double w = -1.0 * acc[u] + o[u];
             a.out[r[u]] = w;
             nrm = dd_add_prod(nrm, w, w);
