@@ -337,6 +337,9 @@ __device__ inline bool arn_sigma(const ArnArgs &a, double *red, double &sig)
         return true;
     }
     const double s = sqrt(dd_value(sum_dd_partials(a.npart, a.G, red)));
+    // every wave has read the block sum out of red before any wave writes
+    // its dot or norm sums there
+    __syncthreads();
     const bool broke = s < st->btol;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (broke) {
