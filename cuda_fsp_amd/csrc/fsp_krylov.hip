@@ -838,8 +838,11 @@ static fsp_status prof_mark(ExpvImpl &E, int cls, double bytes)
     if (!E.profile) return FSP_OK;
     const size_t k = (size_t)E.pcount;
     while (E.pev.size() < 2 * (k + 1)) {
+        // timing only (read after the step's stream synchronisation): no
+        // system-scope fence at record, whose cache writeback would be
+        // counted in the bracketed launch
         hipEvent_t ev = nullptr;
-        FSP_HIP(hipEventCreate(&ev));
+        FSP_HIP(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
         E.pev.push_back(ev);
     }
     if (E.pcls.size() < k + 1) {
